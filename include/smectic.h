@@ -1,0 +1,172 @@
+/*
+ * smectic.h -- C ABI of libsmectic, the MI355X (gfx950) implementation of the
+ * semi-implicit pseudo-spectral Swift-Hohenberg time step of evitral/smectic3D.
+ *
+ * The reference has no library API: every driver is a standalone main() whose
+ * time loop calls FFTW-MPI plans on caller-owned arrays.  This ABI replaces the
+ * loop body of those drivers (the boundary sits at the loop, not at
+ * fftw_execute, so that pointwise and k-space work can be fused into the
+ * transform passes).  Each entry point names the reference code it replaces.
+ *
+ * Conventions
+ *  - Plain C types only.  Host pointers are borrowed for the duration of a call.
+ *  - Return 0 on success, a negative SMX_ERR_* code on failure; the message is
+ *    available from smx_last_error() (thread-local).
+ *  - A handle owns its device buffers and one HIP stream.  A handle is not
+ *    thread-safe; distinct handles may be used from distinct threads.
+ *  - Real fields cross the boundary in the reference layout: x slowest, z
+ *    contiguous, element (i,j,k) at (i*ny + j)*row_stride + k.  row_stride is
+ *    nz (dense, as in cosFFT/mesabi/cosNoAdvMorph.cpp:296) or 2*(nz/2+1)
+ *    (FFTW's padded r2c layout, periodicFFT/pf3d.cpp:312).
+ */
+#ifndef SMECTIC_H
+#define SMECTIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMX_ABI_VERSION 1
+
+/* boundary conditions */
+#define SMX_BC_PERIODIC 0 /* R2C/C2R, periodicFFT/pf3d.cpp:226-291 */
+#define SMX_BC_NEUMANN 1  /* REDFT10/REDFT01, cosFFT/mesabi/cosNoAdvMorph.cpp:249-262 */
+
+/* arithmetic of the transforms and pointwise work */
+#define SMX_FP64 0       /* everything in FP64 (the reference's arithmetic) */
+#define SMX_FP32_MIXED 1 /* FP32 passes, FP64 twiddle tables, FP64 mean mode and reductions */
+
+/* nonlinearity N(psi) */
+#define SMX_NL_QUINTIC 0 /* beta*psi^3 - gamma*psi^5, periodicFFT/pf3d.cpp:460-461 */
+#define SMX_NL_CUBIC 1   /* beta*psi^2 - gamma*psi^3 */
+
+/* advective force fed to the Stokes solve */
+#define SMX_FORCE_SH 0 /* S = psiSH * grad(psi), periodicFFT/pf3d.cpp:422-424 */
+#define SMX_FORCE_MU 1 /* S = mu * grad(psi), mu = dF/dpsi */
+
+/* error codes */
+#define SMX_OK 0
+#define SMX_ERR_INVALID (-1)     /* bad argument or configuration */
+#define SMX_ERR_UNSUPPORTED (-2) /* valid but not implemented (e.g. a grid length without a radix plan) */
+#define SMX_ERR_HIP (-3)         /* HIP runtime failure */
+#define SMX_ERR_NONFINITE (-4)   /* state contains NaN/Inf */
+#define SMX_ERR_STATE (-5)       /* call out of order (e.g. step before set_psi) */
+
+typedef struct smx_handle smx_handle;
+
+/*
+ * Step operator and grid.  Replaces the compile-time constants of the drivers:
+ * periodicFFT/pf3d.cpp:35-75 (grid, eps, alpha, beta, gamma, q0, eta, rho, dt,
+ * box L = N*2*pi/(q0*Nw)), periodicFFT/pf3dMorph.cpp:94-102 (mid-plane
+ * penalty wMid), cosFFT/mesabi/cosNoAdvMorph.cpp:90-112 (Neumann variant).
+ */
+typedef struct smx_config {
+  int abi_version; /* must be SMX_ABI_VERSION */
+  int bc;          /* SMX_BC_* */
+  int precision;   /* SMX_FP64 or SMX_FP32_MIXED */
+  int nx, ny, nz;  /* grid; each a product of 2s and 3s with a radix plan */
+  double lx, ly, lz; /* box lengths; dx = lx/nx etc. */
+  double eps, q0, alpha, beta, gamma, dt; /* L(k) = eps - alpha*(|k|^2 - q0^2)^2 */
+  int nl_model;    /* SMX_NL_* */
+  int advect;      /* 1: spectral Stokes advection (periodic only), pf3d.cpp:399-465 */
+  double eta, rho; /* Stokes viscosity and density, pf3d.cpp:54-55,210 */
+  int force_form;  /* SMX_FORCE_* */
+  double w_mid;    /* mid-plane penalty weight, pf3dMorph.cpp:102 (0 = off) */
+  int mid_plane;   /* z index of the penalty plane (reference: nz/2); <0 = off */
+  int device;      /* HIP device ordinal */
+} smx_config;
+
+/* Fill *cfg with the reference's pf3d.cpp constants at grid n^3 (Nw = 16). */
+void smx_config_default(smx_config *cfg, int n);
+
+/* Create a solver.  Replaces the set-up of pf3d.cpp:84-291 (MPI/FFTW init,
+ * wavenumbers :160-190, operator tables :195-222, 13 plans :226-291). */
+int smx_create(const smx_config *cfg, smx_handle **out);
+void smx_destroy(smx_handle *h);
+const char *smx_last_error(void);
+/* Config the handle was created with. */
+int smx_get_config(const smx_handle *h, smx_config *cfg);
+
+/* Load psi (host, reference layout) and bootstrap the Adams-Bashforth history
+ * from N(psi) without advection.  Replaces the IC scatter/load and the "First
+ * NqPast" block: pf3d.cpp:363-374, pf3dMorph.cpp:416-472,
+ * cosFFT/mesabi/cosNoAdvMorph.cpp:423-442. */
+int smx_set_psi(smx_handle *h, const double *psi, int64_t row_stride);
+
+/* Copy the current psi to the host in the reference layout.  Replaces the
+ * Gather of psi to rank 0 before a dump, pf3d.cpp:526-527,554-555. */
+int smx_get_psi(smx_handle *h, double *psi, int64_t row_stride);
+
+/* Advance nsteps CNAB2 steps.  Replaces the body of the time loop:
+ * pf3d.cpp:390-485,549 (periodic, with advection when cfg.advect),
+ * pf3dMorph.cpp:501-610 (mid-plane penalty),
+ * cosFFT/mesabi/cosNoAdvMorph.cpp:483-530,619 (Neumann).
+ * Asynchronous on the handle's stream. */
+int smx_step(smx_handle *h, int64_t nsteps);
+
+/* L1 = sum|psi^{n+1}-psi^n| / (dt * sum|psi^{n+1}|) over the last step taken
+ * by the most recent smx_step call, FP64 compensated.  Replaces the L1 block
+ * pf3d.cpp:489-519 (Reduce x2 + Bcast). */
+int smx_l1(smx_handle *h, double *l1);
+
+/* Lyapunov energy F of the current state, FP64 compensated, spectral
+ * (Parseval) quadrature of -eps/2 psi^2 + alpha/2((lap+q0^2)psi)^2 plus the
+ * model's local terms (-beta/4 psi^4 + gamma/6 psi^6 for the quintic model,
+ * energyPsi.m:217-228; -beta/3 psi^3 + gamma/4 psi^4 for the cubic one).
+ * For SMX_BC_NEUMANN the quadrature uses the cosine basis. */
+int smx_energy(smx_handle *h, double *F);
+
+/* Power-weighted mean |k| of the current state (k = 0 excluded); the layer
+ * wavenumber diagnostic used to compare runs (cf. periodicFFT/pf3dVal.cpp:214). */
+int smx_dominant_wavenumber(smx_handle *h, double *q);
+
+/* dpsi/dt = F^-1[L(k) psi_hat + N_hat] of the current state (host, reference
+ * layout).  Replaces the dTpsi transform of cosNoAdvMorphCurv.cpp:744,764 and
+ * surfTrack.cpp:611,622. */
+int smx_dtpsi(smx_handle *h, double *dtpsi, int64_t row_stride);
+
+/* surfTrack column scan at the psi = 0 crossing of every (i,j) column.
+ * Periodic: surfTrack.cpp:767-806 (scan k upward, arm at psi > arm, spectral
+ * derivatives, H ~ (psi_xx+psi_yy)/|grad psi|, K = (psi_xx psi_yy - psi_xy^2)/|grad psi|^2).
+ * Neumann: cosNoAdvMorphCurv.cpp:811-966 (scan k downward, arm at psi > arm,
+ * central finite-difference gradients, Megrabov mean curvature
+ * H = div(grad psi/|grad psi|) and Gaussian curvature K).
+ * Outputs are nx*ny arrays in (i*ny + j) order.  Like the reference's
+ * persistent surf arrays, a column without a crossing keeps the value the
+ * handle last stored for it (zero-initialised).  *found = number of columns
+ * that crossed on this call. */
+typedef struct smx_surf_opts {
+  double arm;      /* arming threshold: 0.1 (surfTrack.cpp:775) or 0.7 (cosNoAdvMorphCurv.cpp:889) */
+  double vel_sign; /* +1: v = +dtpsi/|grad psi| as in the reference; -1: v = -dtpsi/|grad psi| */
+} smx_surf_opts;
+int smx_surftrack(smx_handle *h, const smx_surf_opts *opts, double *surfz, double *vel,
+                  double *H, double *K, int64_t *found);
+
+/* Block until the handle's stream is idle. */
+int smx_sync(smx_handle *h);
+
+/* Number of steps taken since the last smx_set_psi. */
+int smx_step_count(const smx_handle *h, int64_t *n);
+
+/*
+ * Measurement.  Times every kernel launch of nsteps steps with HIP events on
+ * the handle's stream and reports, per kernel slot of the step chain, the
+ * average launch duration (ms) and the algorithmic bytes per launch.  names
+ * receives '\n'-separated kernel slot names (at most names_cap bytes).
+ * No counterpart in the reference (its only timers are the wall/CPU clocks of
+ * cosFFT/comet/cosNoAdvMorph.cpp:562-571,765-776).
+ */
+int smx_profile_step(smx_handle *h, int64_t nsteps, int max_slots, int *nslots, double *avg_ms,
+                     double *alg_bytes, char *names, int names_cap);
+
+/* Plain device copy kernel over nbytes (grid-stride float4 copy), used to
+ * calibrate hardware byte counters against a kernel of known traffic. */
+int smx_calibration_copy(smx_handle *h, int64_t nbytes, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMECTIC_H */

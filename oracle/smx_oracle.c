@@ -1,0 +1,923 @@
+/*
+ * smx_oracle.c -- FP64 CPU oracle (TEST INFRASTRUCTURE ONLY; see smx_oracle.h).
+ *
+ * A restatement of the reference time loops in plain C.  Every block cites the
+ * reference file:line it follows (paths relative to the reference root).  The
+ * FFT is this file's own FP64 Stockham autosort (radices 4, 2, 3, 5 and a
+ * direct DFT for other primes), following the DFT definitions FFTW documents:
+ * forward e^{-2 pi i jk/n}, backward e^{+2 pi i jk/n}, all unnormalised.
+ */
+#include "smx_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ======================================================================
+ * 1-D complex FFT (Stockham autosort)
+ * ==================================================================== */
+typedef struct {
+  int n, nf;
+  int rad[64];
+  cplx *tw; /* tw[m] = exp(-2 pi i m / n) */
+} plan1d;
+
+static plan1d *plan_create(int n) {
+  plan1d *p = (plan1d *)calloc(1, sizeof(plan1d));
+  p->n = n;
+  int m = n;
+  while (m % 4 == 0) { p->rad[p->nf++] = 4; m /= 4; }
+  while (m % 2 == 0) { p->rad[p->nf++] = 2; m /= 2; }
+  while (m % 3 == 0) { p->rad[p->nf++] = 3; m /= 3; }
+  while (m % 5 == 0) { p->rad[p->nf++] = 5; m /= 5; }
+  for (int f = 7; m > 1; f += 2)
+    while (m % f == 0) { p->rad[p->nf++] = f; m /= f; }
+  p->tw = (cplx *)malloc(sizeof(cplx) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    double a = -2.0 * M_PI * (double)i / (double)n;
+    p->tw[i] = cos(a) + I * sin(a);
+  }
+  return p;
+}
+
+static void plan_destroy(plan1d *p) {
+  if (!p) return;
+  free(p->tw);
+  free(p);
+}
+
+/* x: input/output (length n), w: scratch (length n). sign -1 forward, +1 backward. */
+static void fft_exec(const plan1d *p, cplx *x, cplx *w, int sign) {
+  const int n = p->n;
+  cplx *X = x, *Y = w;
+  int Ns = 1;
+  for (int f = 0; f < p->nf; ++f) {
+    const int R = p->rad[f], m = n / R, tws = n / (Ns * R);
+    for (int j = 0; j < m; ++j) {
+      const int k = j % Ns, base = (j - k) * R + k;
+      cplx a[64], b[64];
+      for (int r = 0; r < R; ++r) {
+        cplx t = p->tw[r * k * tws];
+        if (sign > 0) t = conj(t);
+        a[r] = X[j + r * m] * t;
+      }
+      if (R == 2) {
+        b[0] = a[0] + a[1];
+        b[1] = a[0] - a[1];
+      } else if (R == 4) {
+        const cplx is = (sign > 0) ? I : -I; /* e^{sign*2 pi i/4} */
+        cplx t0 = a[0] + a[2], t1 = a[0] - a[2], t2 = a[1] + a[3], t3 = (a[1] - a[3]) * is;
+        b[0] = t0 + t2;
+        b[2] = t0 - t2;
+        b[1] = t1 + t3;
+        b[3] = t1 - t3;
+      } else if (R == 3) {
+        const double s3 = 0.86602540378443864676 * (double)sign;
+        cplx t = a[1] + a[2], d = (a[1] - a[2]) * (I * s3);
+        b[0] = a[0] + t;
+        b[1] = a[0] - 0.5 * t + d;
+        b[2] = a[0] - 0.5 * t - d;
+      } else {
+        const int st = n / R;
+        for (int q = 0; q < R; ++q) {
+          cplx s = 0;
+          for (int r = 0; r < R; ++r) {
+            cplx t = p->tw[((q * r) % R) * st];
+            if (sign > 0) t = conj(t);
+            s += a[r] * t;
+          }
+          b[q] = s;
+        }
+      }
+      for (int r = 0; r < R; ++r) Y[base + r * Ns] = b[r];
+    }
+    Ns *= R;
+    cplx *t = X;
+    X = Y;
+    Y = t;
+  }
+  if (X != x) memcpy(x, X, sizeof(cplx) * (size_t)n);
+}
+
+void or_fft1d(int n, double *re_im, int sign) {
+  plan1d *p = plan_create(n);
+  cplx *w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  fft_exec(p, (cplx *)re_im, w, sign);
+  free(w);
+  plan_destroy(p);
+}
+
+/* Apply a complex FFT along one axis of a complex array: lines of length n
+ * with element stride `st`, line starts given by (outer o, inner q):
+ * base = o*ostride + q, o < nouter, q < ninner. */
+static void fft_axis(cplx *a, int n, int64_t st, int64_t nouter, int64_t ostride, int64_t ninner, int sign) {
+  plan1d *p = plan_create(n);
+#pragma omp parallel
+  {
+    cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+#pragma omp for collapse(2) schedule(static)
+    for (int64_t o = 0; o < nouter; ++o)
+      for (int64_t q = 0; q < ninner; ++q) {
+        cplx *b = a + o * ostride + q;
+        for (int i = 0; i < n; ++i) line[i] = b[i * st];
+        fft_exec(p, line, w, sign);
+        for (int i = 0; i < n; ++i) b[i * st] = line[i];
+      }
+    free(line);
+    free(w);
+  }
+  plan_destroy(p);
+}
+
+/* ======================================================================
+ * 3-D real transforms with FFTW r2c / c2r semantics
+ * ==================================================================== */
+
+/* r2c: z (half spectrum), then y, then x.  Two real z-lines share one complex
+ * FFT: Z = FFT(a + i b), A = (Z[k] + conj Z[-k])/2, B = (Z[k] - conj Z[-k])/2i. */
+int or_rfft3(int nx, int ny, int nz, const double *in, double *out_ri) {
+  const int nzh = nz / 2 + 1;
+  const int64_t nlines = (int64_t)nx * ny;
+  cplx *out = (cplx *)out_ri;
+  plan1d *p = plan_create(nz);
+#pragma omp parallel
+  {
+    cplx *z = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+    cplx *w = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < nlines; l += 2) {
+      const int two = (l + 1 < nlines);
+      const double *a = in + l * nz, *b = two ? in + (l + 1) * nz : NULL;
+      for (int i = 0; i < nz; ++i) z[i] = a[i] + I * (two ? b[i] : 0.0);
+      fft_exec(p, z, w, -1);
+      for (int k = 0; k < nzh; ++k) {
+        cplx zk = z[k], zm = conj(z[(nz - k) % nz]);
+        out[l * nzh + k] = 0.5 * (zk + zm);
+        if (two) out[(l + 1) * nzh + k] = -0.5 * I * (zk - zm);
+      }
+    }
+    free(z);
+    free(w);
+  }
+  plan_destroy(p);
+  fft_axis(out, ny, nzh, nx, (int64_t)ny * nzh, nzh, -1);  /* y */
+  fft_axis(out, nx, (int64_t)ny * nzh, 1, 0, (int64_t)ny * nzh, -1); /* x */
+  return 0;
+}
+
+/* c2r: complex backward DFTs along x and y of the half spectrum, then the 1-D
+ * c2r along z, which uses only the real parts of the kz = 0 and kz = nz/2
+ * entries (the half-complex stage of a multi-dimensional FFTW c2r). */
+int or_irfft3(int nx, int ny, int nz, const double *in_ri, double *out) {
+  const int nzh = nz / 2 + 1;
+  const int64_t nlines = (int64_t)nx * ny;
+  cplx *t = (cplx *)malloc(sizeof(cplx) * (size_t)(nlines * nzh));
+  memcpy(t, in_ri, sizeof(cplx) * (size_t)(nlines * nzh));
+  fft_axis(t, nx, (int64_t)ny * nzh, 1, 0, (int64_t)ny * nzh, +1);
+  fft_axis(t, ny, nzh, nx, (int64_t)ny * nzh, nzh, +1);
+  plan1d *p = plan_create(nz);
+#pragma omp parallel
+  {
+    cplx *z = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+    cplx *w = (cplx *)malloc(sizeof(cplx) * (size_t)nz);
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < nlines; l += 2) {
+      const int two = (l + 1 < nlines);
+      const cplx *A = t + l * nzh, *B = two ? t + (l + 1) * nzh : NULL;
+      for (int k = 0; k < nzh; ++k) {
+        cplx a = A[k], b = two ? B[k] : 0.0;
+        int self = (k == 0) || (2 * k == nz);
+        if (self) { a = creal(a); b = creal(b); }
+        z[k] = a + I * b;
+        if (!self && k > 0) z[nz - k] = conj(a) + I * conj(b);
+      }
+      fft_exec(p, z, w, +1);
+      for (int i = 0; i < nz; ++i) {
+        out[l * nz + i] = creal(z[i]);
+        if (two) out[(l + 1) * nz + i] = cimag(z[i]);
+      }
+    }
+    free(z);
+    free(w);
+  }
+  plan_destroy(p);
+  free(t);
+  return 0;
+}
+
+/* ======================================================================
+ * DCT-II (REDFT10) / DCT-III (REDFT01) via an n-point complex FFT
+ * (Makhoul's even/odd reordering and quarter-wave twiddle).
+ * REDFT10: y_k = 2 sum_j x_j cos(pi (j+1/2) k / n)
+ * REDFT01: y_k = x_0 + 2 sum_{j>=1} x_j cos(pi j (k+1/2) / n)
+ * ==================================================================== */
+static void redft10_line(const plan1d *p, const double *x, int64_t xs, double *y, int64_t ys, cplx *v,
+                         cplx *w) {
+  const int n = p->n;
+  for (int m = 0; 2 * m < n; ++m) v[m] = x[(2 * m) * xs];
+  for (int m = 0; 2 * m + 1 < n; ++m) v[n - 1 - m] = x[(2 * m + 1) * xs];
+  fft_exec(p, v, w, -1);
+  for (int k = 0; k < n; ++k) {
+    double a = -M_PI * (double)k / (2.0 * n);
+    y[k * ys] = 2.0 * creal((cos(a) + I * sin(a)) * v[k]);
+  }
+}
+
+static void redft01_line(const plan1d *p, const double *x, int64_t xs, double *y, int64_t ys, cplx *v,
+                         cplx *w) {
+  const int n = p->n;
+  for (int k = 0; k < n; ++k) {
+    double a = M_PI * (double)k / (2.0 * n);
+    double xnk = (k == 0) ? 0.0 : x[(n - k) * xs];
+    v[k] = (cos(a) + I * sin(a)) * (x[k * xs] - I * xnk);
+  }
+  fft_exec(p, v, w, +1);
+  for (int m = 0; 2 * m < n; ++m) y[(2 * m) * ys] = creal(v[m]);
+  for (int m = 0; 2 * m + 1 < n; ++m) y[(2 * m + 1) * ys] = creal(v[n - 1 - m]);
+}
+
+void or_redft10_1d(int n, const double *in, double *out) {
+  plan1d *p = plan_create(n);
+  cplx *v = (cplx *)malloc(sizeof(cplx) * n), *w = (cplx *)malloc(sizeof(cplx) * n);
+  redft10_line(p, in, 1, out, 1, v, w);
+  free(v); free(w); plan_destroy(p);
+}
+
+void or_redft01_1d(int n, const double *in, double *out) {
+  plan1d *p = plan_create(n);
+  cplx *v = (cplx *)malloc(sizeof(cplx) * n), *w = (cplx *)malloc(sizeof(cplx) * n);
+  redft01_line(p, in, 1, out, 1, v, w);
+  free(v); free(w); plan_destroy(p);
+}
+
+static void r2r_axis(double *a, int n, int64_t st, int64_t nouter, int64_t ostride, int64_t ninner, int kind) {
+  plan1d *p = plan_create(n);
+#pragma omp parallel
+  {
+    cplx *v = (cplx *)malloc(sizeof(cplx) * (size_t)n), *w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    double *line = (double *)malloc(sizeof(double) * (size_t)n);
+#pragma omp for collapse(2) schedule(static)
+    for (int64_t o = 0; o < nouter; ++o)
+      for (int64_t q = 0; q < ninner; ++q) {
+        double *b = a + o * ostride + q;
+        if (kind == 10) redft10_line(p, b, st, line, 1, v, w);
+        else redft01_line(p, b, st, line, 1, v, w);
+        for (int i = 0; i < n; ++i) b[i * st] = line[i];
+      }
+    free(v); free(w); free(line);
+  }
+  plan_destroy(p);
+}
+
+static void r2r3(int nx, int ny, int nz, const double *in, double *out, int kind) {
+  if (out != in) memcpy(out, in, sizeof(double) * (size_t)nx * ny * nz);
+  r2r_axis(out, nz, 1, (int64_t)nx * ny, nz, 1, kind);
+  r2r_axis(out, ny, nz, nx, (int64_t)ny * nz, nz, kind);
+  r2r_axis(out, nx, (int64_t)ny * nz, 1, 0, (int64_t)ny * nz, kind);
+}
+
+int or_redft10_3d(int nx, int ny, int nz, const double *in, double *out) { r2r3(nx, ny, nz, in, out, 10); return 0; }
+int or_redft01_3d(int nx, int ny, int nz, const double *in, double *out) { r2r3(nx, ny, nz, in, out, 1); return 0; }
+
+/* ======================================================================
+ * Solver state
+ * ==================================================================== */
+struct or_state {
+  smx_config c;
+  int nx, ny, nz, nzh;
+  int64_t N, Nh; /* real points; spectral entries (complex for periodic, real for Neumann) */
+  double dx, dy, dz, scale;
+  double *vqx, *vqy, *vqz;
+  double *psi, *psi_prev, *nqpast; /* nqpast: complex Nh (periodic) or real N (Neumann) */
+  int64_t steps;
+  int has_state, has_prev;
+};
+
+/* Periodic wavenumbers in FFT order, Nyquist kept at +pi/dx: pf3d.cpp:162-190. */
+static void wavenumbers_periodic(double *v, int n, double d) {
+  for (int i = 0; i < n; ++i) v[i] = (i < n / 2 + 1) ? 2.0 * M_PI * i / (d * n) : -(double)(n - i) * 2.0 * M_PI / (d * n);
+}
+/* DCT wavenumbers pi k / L: cosNoAdvMorph.cpp:216-231. */
+static void wavenumbers_dct(double *v, int n, double d) {
+  for (int i = 0; i < n; ++i) v[i] = M_PI * i / (d * n);
+}
+
+or_state *or_create(const smx_config *cfg) {
+  if (!cfg || cfg->nx <= 0 || cfg->ny <= 0 || cfg->nz <= 1) return NULL;
+  if (cfg->bc == SMX_BC_PERIODIC && (cfg->nz % 2)) return NULL;
+  or_state *s = (or_state *)calloc(1, sizeof(or_state));
+  s->c = *cfg;
+  s->nx = cfg->nx; s->ny = cfg->ny; s->nz = cfg->nz; s->nzh = cfg->nz / 2 + 1;
+  s->N = (int64_t)s->nx * s->ny * s->nz;
+  s->dx = cfg->lx / s->nx; s->dy = cfg->ly / s->ny; s->dz = cfg->lz / s->nz;
+  s->vqx = (double *)malloc(sizeof(double) * s->nx);
+  s->vqy = (double *)malloc(sizeof(double) * s->ny);
+  s->vqz = (double *)malloc(sizeof(double) * s->nz);
+  if (cfg->bc == SMX_BC_PERIODIC) {
+    s->Nh = (int64_t)s->nx * s->ny * s->nzh;
+    s->scale = 1.0 / (double)s->N; /* pf3d.cpp:62 */
+    wavenumbers_periodic(s->vqx, s->nx, s->dx);
+    wavenumbers_periodic(s->vqy, s->ny, s->dy);
+    wavenumbers_periodic(s->vqz, s->nz, s->dz);
+    s->nqpast = (double *)calloc((size_t)(2 * s->Nh), sizeof(double));
+  } else {
+    s->Nh = s->N;
+    s->scale = 0.125 / (double)s->N; /* cosNoAdvMorph.cpp:112 */
+    wavenumbers_dct(s->vqx, s->nx, s->dx);
+    wavenumbers_dct(s->vqy, s->ny, s->dy);
+    wavenumbers_dct(s->vqz, s->nz, s->dz);
+    s->nqpast = (double *)calloc((size_t)s->N, sizeof(double));
+  }
+  s->psi = (double *)calloc((size_t)s->N, sizeof(double));
+  s->psi_prev = (double *)calloc((size_t)s->N, sizeof(double));
+  return s;
+}
+
+void or_destroy(or_state *s) {
+  if (!s) return;
+  free(s->vqx); free(s->vqy); free(s->vqz);
+  free(s->psi); free(s->psi_prev); free(s->nqpast);
+  free(s);
+}
+
+/* opSH = alpha (|k|^2 - q0^2)^2, aLin = eps - opSH: pf3d.cpp:205-206, pf3dMorph.cpp:236-238 */
+static inline double op_sh(const or_state *s, double k2) {
+  double d = k2 - s->c.q0 * s->c.q0;
+  return s->c.alpha * d * d;
+}
+
+/* Local nonlinearity: pf3d.cpp:368-369 (quintic) or the cubic model. */
+static inline double nl(const or_state *s, double p) {
+  if (s->c.nl_model == SMX_NL_CUBIC) return s->c.beta * p * p - s->c.gamma * p * p * p;
+  double p2 = p * p;
+  return s->c.beta * p2 * p - s->c.gamma * p2 * p2 * p;
+}
+
+/* N(psi) without advection, with the pf3dMorph mid-plane penalty
+ * 2 wMid (psi_{k-1} - 2 psi_k + psi_{k+1})/dz^2 at k == mid_plane
+ * (pf3dMorph.cpp:456-462, 572-581). */
+static void nl_field(const or_state *s, const double *psi, double *out) {
+  const int nz = s->nz;
+  const int mid = (s->c.w_mid != 0.0) ? s->c.mid_plane : -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t l = 0; l < (int64_t)s->nx * s->ny; ++l) {
+    const double *p = psi + l * nz;
+    double *o = out + l * nz;
+    for (int k = 0; k < nz; ++k) o[k] = nl(s, p[k]);
+    if (mid > 0 && mid < nz - 1)
+      o[mid] += 2.0 * s->c.w_mid * (p[mid - 1] - 2.0 * p[mid] + p[mid + 1]) / (s->dz * s->dz);
+  }
+}
+
+int or_set_psi(or_state *s, const double *psi) {
+  memcpy(s->psi, psi, sizeof(double) * (size_t)s->N);
+  memcpy(s->psi_prev, psi, sizeof(double) * (size_t)s->N);
+  double *nr = (double *)malloc(sizeof(double) * (size_t)s->N);
+  nl_field(s, s->psi, nr);
+  if (s->c.bc == SMX_BC_PERIODIC)
+    or_rfft3(s->nx, s->ny, s->nz, nr, s->nqpast); /* "First NqPast", pf3d.cpp:363-374 */
+  else
+    or_redft10_3d(s->nx, s->ny, s->nz, nr, s->nqpast); /* cosNoAdvMorph.cpp:436-442 */
+  free(nr);
+  s->steps = 0;
+  s->has_state = 1;
+  s->has_prev = 0;
+  return 0;
+}
+
+int or_get_psi(const or_state *s, double *psi) {
+  memcpy(psi, s->psi, sizeof(double) * (size_t)s->N);
+  return 0;
+}
+
+int or_get_nqpast(const or_state *s, double *out) {
+  memcpy(out, s->nqpast, sizeof(double) * (size_t)(s->c.bc == SMX_BC_PERIODIC ? 2 * s->Nh : s->N));
+  return 0;
+}
+
+int64_t or_step_count(const or_state *s) { return s->steps; }
+
+#define KIDX(i, j, k) (((int64_t)(i) * s->ny + (j)) * s->nzh + (k))
+
+/* Advective term -U.grad(psi) of pf3d.cpp:399-451 added into nr. */
+static void periodic_advection(or_state *s, const cplx *psiq, double *nr) {
+  const int64_t N = s->N, Nh = s->Nh;
+  const double sc = s->scale;
+  cplx *gq[3], *shq = (cplx *)malloc(sizeof(cplx) * Nh);
+  double *g[3], *sh = (double *)malloc(sizeof(double) * N);
+  for (int a = 0; a < 3; ++a) {
+    gq[a] = (cplx *)malloc(sizeof(cplx) * Nh);
+    g[a] = (double *)malloc(sizeof(double) * N);
+  }
+  /* spectral gradient and psiSH: pf3d.cpp:399-408 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nzh; ++k) {
+        int64_t id = KIDX(i, j, k);
+        double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+        gq[0][id] = sc * I * s->vqx[i] * psiq[id];
+        gq[1][id] = sc * I * s->vqy[j] * psiq[id];
+        gq[2][id] = sc * I * s->vqz[k] * psiq[id];
+        shq[id] = sc * op_sh(s, k2) * psiq[id];
+      }
+  for (int a = 0; a < 3; ++a) or_irfft3(s->nx, s->ny, s->nz, (double *)gq[a], g[a]); /* :410-412 */
+  or_irfft3(s->nx, s->ny, s->nz, (double *)shq, sh);                                 /* :413 */
+  /* force S = psiSH grad psi (pf3d.cpp:417-425) or mu grad psi with
+   * mu = psiSH - eps psi - N(psi) (the variational derivative of F). */
+  double *S[3];
+  for (int a = 0; a < 3; ++a) S[a] = (double *)malloc(sizeof(double) * N);
+#pragma omp parallel for schedule(static)
+  for (int64_t x = 0; x < N; ++x) {
+    double f = sh[x];
+    if (s->c.force_form == SMX_FORCE_MU) f = sh[x] - s->c.eps * s->psi[x] - nl(s, s->psi[x]);
+    for (int a = 0; a < 3; ++a) S[a][x] = f * g[a][x];
+  }
+  cplx *Sq[3];
+  for (int a = 0; a < 3; ++a) {
+    Sq[a] = (cplx *)malloc(sizeof(cplx) * Nh);
+    or_rfft3(s->nx, s->ny, s->nz, S[a], (double *)Sq[a]); /* :427-429 */
+  }
+  /* Stokes: U = s CM1 (S - CM2 (k.S)), CM1 = 1/(rho eta |k|^2), CM2 = k/|k|^2, zero at k = 0
+   * (pf3d.cpp:209-222, 433-447). */
+  const double reta = s->c.rho * s->c.eta;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nzh; ++k) {
+        int64_t id = KIDX(i, j, k);
+        double kv[3] = {s->vqx[i], s->vqy[j], s->vqz[k]};
+        double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+        if (id == 0 || k2 == 0.0) {
+          for (int a = 0; a < 3; ++a) gq[a][id] = 0.0;
+          continue;
+        }
+        cplx dot = kv[0] * Sq[0][id] + kv[1] * Sq[1][id] + kv[2] * Sq[2][id];
+        double cm1 = 1.0 / (reta * k2);
+        for (int a = 0; a < 3; ++a) gq[a][id] = sc * cm1 * (Sq[a][id] - (kv[a] / k2) * dot);
+      }
+  double *U[3];
+  for (int a = 0; a < 3; ++a) {
+    U[a] = S[a];
+    or_irfft3(s->nx, s->ny, s->nz, (double *)gq[a], U[a]); /* :449-451 */
+  }
+  /* N -= U.grad psi: pf3d.cpp:455-465 */
+#pragma omp parallel for schedule(static)
+  for (int64_t x = 0; x < N; ++x) nr[x] -= U[0][x] * g[0][x] + U[1][x] * g[1][x] + U[2][x] * g[2][x];
+  for (int a = 0; a < 3; ++a) {
+    free(gq[a]); free(g[a]); free(S[a]); free(Sq[a]);
+  }
+  free(shq); free(sh);
+}
+
+/* One periodic step, pf3d.cpp:395-485,549. */
+static void periodic_step(or_state *s) {
+  const int64_t N = s->N, Nh = s->Nh;
+  const double sc = s->scale, dtd2 = 0.5 * s->c.dt;
+  cplx *psiq = (cplx *)malloc(sizeof(cplx) * Nh);
+  cplx *nq = (cplx *)malloc(sizeof(cplx) * Nh);
+  double *nr = (double *)malloc(sizeof(double) * N);
+  or_rfft3(s->nx, s->ny, s->nz, s->psi, (double *)psiq); /* :395 */
+  nl_field(s, s->psi, nr);
+  if (s->c.advect) periodic_advection(s, psiq, nr);
+  or_rfft3(s->nx, s->ny, s->nz, nr, (double *)nq); /* :467 */
+  cplx *nqp = (cplx *)s->nqpast;
+  /* CNAB2: psiqNew = s (C1 psiq + dt/2 (3 Nq - NqPast)) / C2, pf3d.cpp:471-479 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nzh; ++k) {
+        int64_t id = KIDX(i, j, k);
+        double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+        double aLin = s->c.eps - op_sh(s, k2);
+        double C1 = 1.0 + dtd2 * aLin, C2 = 1.0 - dtd2 * aLin;
+        psiq[id] = sc * (C1 * psiq[id] + dtd2 * (3.0 * nq[id] - nqp[id])) / C2;
+        nqp[id] = nq[id]; /* :485 */
+      }
+  double *t = s->psi_prev;
+  s->psi_prev = s->psi;
+  s->psi = t;
+  or_irfft3(s->nx, s->ny, s->nz, (double *)psiq, s->psi); /* :481 */
+  free(psiq); free(nq); free(nr);
+}
+
+/* One Neumann step, cosFFT/mesabi/cosNoAdvMorph.cpp:490-530,619. */
+static void neumann_step(or_state *s) {
+  const int64_t N = s->N;
+  const double sc = s->scale, dtd2 = 0.5 * s->c.dt;
+  double *nr = (double *)malloc(sizeof(double) * N);
+  double *nq = (double *)malloc(sizeof(double) * N);
+  double *psiq = (double *)malloc(sizeof(double) * N);
+  nl_field(s, s->psi, nr);                               /* :494-504 */
+  or_redft10_3d(s->nx, s->ny, s->nz, nr, nq);            /* :508 */
+  or_redft10_3d(s->nx, s->ny, s->nz, s->psi, psiq);      /* :512 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int64_t id = ((int64_t)i * s->ny + j) * s->nz + k;
+        double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+        double aLin = s->c.eps - op_sh(s, k2); /* :431-433 */
+        double C1 = 1.0 + dtd2 * aLin, C2 = 1.0 - dtd2 * aLin;
+        psiq[id] = sc * (C1 * psiq[id] + dtd2 * (3.0 * nq[id] - s->nqpast[id])) / C2; /* :524-525 */
+        s->nqpast[id] = nq[id]; /* :490 of the next step */
+      }
+  double *t = s->psi_prev;
+  s->psi_prev = s->psi;
+  s->psi = t;
+  or_redft01_3d(s->nx, s->ny, s->nz, psiq, s->psi); /* :530 */
+  free(nr); free(nq); free(psiq);
+}
+
+int or_step(or_state *s, int64_t nsteps) {
+  if (!s->has_state) return SMX_ERR_STATE;
+  for (int64_t n = 0; n < nsteps; ++n) {
+    if (s->c.bc == SMX_BC_PERIODIC) periodic_step(s);
+    else neumann_step(s);
+    s->steps++;
+    s->has_prev = 1;
+  }
+  return 0;
+}
+
+/* L1 = sum|psiNew - psi| / (dt sum|psiNew|), pf3d.cpp:494-509 */
+double or_l1(const or_state *s) {
+  if (!s->has_prev) return NAN;
+  double a = 0.0, b = 0.0;
+  for (int64_t x = 0; x < s->N; ++x) {
+    a += fabs(s->psi[x] - s->psi_prev[x]);
+    b += fabs(s->psi[x]);
+  }
+  return a / (s->c.dt * b);
+}
+
+/* Local energy density terms of the model: energyPsi.m:225-228 (quintic:
+ * -beta/4 psi^4 + gamma/6 psi^6) or the cubic model (-beta/3 psi^3 + gamma/4 psi^4). */
+static inline double f_local(const or_state *s, double p) {
+  double p2 = p * p;
+  if (s->c.nl_model == SMX_NL_CUBIC) return -s->c.beta / 3.0 * p2 * p + s->c.gamma / 4.0 * p2 * p2;
+  return -s->c.beta / 4.0 * p2 * p2 + s->c.gamma / 6.0 * p2 * p2 * p2;
+}
+
+/* Spectral energy: quadratic part by Parseval, local part by the rectangle rule. */
+double or_energy(const or_state *s) {
+  const double dV = s->dx * s->dy * s->dz, q02 = s->c.q0 * s->c.q0;
+  double quad = 0.0, loc = 0.0;
+  if (s->c.bc == SMX_BC_PERIODIC) {
+    cplx *q = (cplx *)malloc(sizeof(cplx) * s->Nh);
+    or_rfft3(s->nx, s->ny, s->nz, s->psi, (double *)q);
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nzh; ++k) {
+          double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+          double w = (k == 0 || 2 * k == s->nz) ? 1.0 : 2.0;
+          double m = 0.5 * s->c.alpha * (q02 - k2) * (q02 - k2) - 0.5 * s->c.eps;
+          cplx v = q[KIDX(i, j, k)];
+          quad += w * m * (creal(v) * creal(v) + cimag(v) * cimag(v));
+        }
+    quad *= dV / (double)s->N; /* sum_x psi^2 = (1/N) sum_k |psiq_k|^2 */
+    free(q);
+  } else {
+    double *q = (double *)malloc(sizeof(double) * s->N);
+    or_redft10_3d(s->nx, s->ny, s->nz, s->psi, q);
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nz; ++k) {
+          double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+          double w = (i == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k == 0 ? 0.5 : 1.0);
+          double m = 0.5 * s->c.alpha * (q02 - k2) * (q02 - k2) - 0.5 * s->c.eps;
+          double v = q[((int64_t)i * s->ny + j) * s->nz + k];
+          quad += w * m * v * v;
+        }
+    quad *= dV * s->scale; /* sum_x psi^2 = s sum_k w_k y_k^2, s = 1/(8N) */
+    free(q);
+  }
+  for (int64_t x = 0; x < s->N; ++x) loc += f_local(s, s->psi[x]);
+  return quad + loc * dV;
+}
+
+/* energyPsi.m:131-234: periodic-wrap second differences, trapz over
+ * 0..L-dx on each axis (end points weighted 1/2). */
+double or_energy_fd_trapz(const or_state *s) {
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  const double q02 = s->c.q0 * s->c.q0;
+  const double idx2 = 1.0 / (s->dx * s->dx), idy2 = 1.0 / (s->dy * s->dy), idz2 = 1.0 / (s->dz * s->dz);
+  double tot = 0.0;
+  for (int i = 0; i < nx; ++i) {
+    int im = (i + nx - 1) % nx, ip = (i + 1) % nx;
+    double wi = (i == 0 || i == nx - 1) ? 0.5 : 1.0;
+    for (int j = 0; j < ny; ++j) {
+      int jm = (j + ny - 1) % ny, jp = (j + 1) % ny;
+      double wj = (j == 0 || j == ny - 1) ? 0.5 : 1.0;
+      for (int k = 0; k < nz; ++k) {
+        int km = (k + nz - 1) % nz, kp = (k + 1) % nz;
+        double wk = (k == 0 || k == nz - 1) ? 0.5 : 1.0;
+#define P(a, b, c) s->psi[((int64_t)(a) * ny + (b)) * nz + (c)]
+        double p = P(i, j, k);
+        double lap = idz2 * (P(i, j, km) - 2 * p + P(i, j, kp)) + idy2 * (P(i, jm, k) - 2 * p + P(i, jp, k)) +
+                     idx2 * (P(im, j, k) - 2 * p + P(ip, j, k));
+#undef P
+        double t = q02 * p + lap;
+        double e = -0.5 * s->c.eps * p * p + 0.5 * s->c.alpha * t * t + f_local(s, p);
+        tot += wi * wj * wk * e;
+      }
+    }
+  }
+  return tot * s->dx * s->dy * s->dz;
+}
+
+double or_dominant_wavenumber(const or_state *s) {
+  double num = 0.0, den = 0.0;
+  if (s->c.bc == SMX_BC_PERIODIC) {
+    cplx *q = (cplx *)malloc(sizeof(cplx) * s->Nh);
+    or_rfft3(s->nx, s->ny, s->nz, s->psi, (double *)q);
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nzh; ++k) {
+          if (i == 0 && j == 0 && k == 0) continue;
+          double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+          double w = (k == 0 || 2 * k == s->nz) ? 1.0 : 2.0;
+          cplx v = q[KIDX(i, j, k)];
+          double pw = w * (creal(v) * creal(v) + cimag(v) * cimag(v));
+          num += pw * sqrt(k2);
+          den += pw;
+        }
+    free(q);
+  } else {
+    double *q = (double *)malloc(sizeof(double) * s->N);
+    or_redft10_3d(s->nx, s->ny, s->nz, s->psi, q);
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nz; ++k) {
+          if (i == 0 && j == 0 && k == 0) continue;
+          double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+          double w = (i == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k == 0 ? 0.5 : 1.0);
+          double v = q[((int64_t)i * s->ny + j) * s->nz + k];
+          num += w * v * v * sqrt(k2);
+          den += w * v * v;
+        }
+    free(q);
+  }
+  return den > 0 ? num / den : 0.0;
+}
+
+/* dpsi/dt = F^-1[s (aLin psiq + Nq)] of the current state:
+ * surfTrack.cpp:611,622 (periodic), cosNoAdvMorphCurv.cpp:744,764 (Neumann). */
+int or_dtpsi(or_state *s, double *out) {
+  const int64_t N = s->N;
+  double *nr = (double *)malloc(sizeof(double) * N);
+  nl_field(s, s->psi, nr);
+  if (s->c.bc == SMX_BC_PERIODIC) {
+    cplx *psiq = (cplx *)malloc(sizeof(cplx) * s->Nh), *nq = (cplx *)malloc(sizeof(cplx) * s->Nh);
+    or_rfft3(s->nx, s->ny, s->nz, s->psi, (double *)psiq);
+    if (s->c.advect) periodic_advection(s, psiq, nr);
+    or_rfft3(s->nx, s->ny, s->nz, nr, (double *)nq);
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nzh; ++k) {
+          int64_t id = KIDX(i, j, k);
+          double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+          nq[id] = s->scale * ((s->c.eps - op_sh(s, k2)) * psiq[id] + nq[id]);
+        }
+    or_irfft3(s->nx, s->ny, s->nz, (double *)nq, out);
+    free(psiq); free(nq);
+  } else {
+    double *psiq = (double *)malloc(sizeof(double) * N), *nq = (double *)malloc(sizeof(double) * N);
+    or_redft10_3d(s->nx, s->ny, s->nz, s->psi, psiq);
+    or_redft10_3d(s->nx, s->ny, s->nz, nr, nq);
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nz; ++k) {
+          int64_t id = ((int64_t)i * s->ny + j) * s->nz + k;
+          double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+          nq[id] = s->scale * ((s->c.eps - op_sh(s, k2)) * psiq[id] + nq[id]);
+        }
+    or_redft01_3d(s->nx, s->ny, s->nz, nq, out);
+    free(psiq); free(nq);
+  }
+  free(nr);
+  return 0;
+}
+
+/* Periodic surfTrack on the current state (surfTrack.cpp:611-626 spectral
+ * derivatives, :767-806 column scan). */
+static int surftrack_periodic(or_state *s, double arm, double vsgn, double *surfz, double *vel, double *H,
+                              double *Kc, int64_t *found) {
+  const int64_t N = s->N, Nh = s->Nh;
+  const double sc = s->scale;
+  cplx *psiq = (cplx *)malloc(sizeof(cplx) * Nh), *t = (cplx *)malloc(sizeof(cplx) * Nh);
+  double *f[8];
+  for (int a = 0; a < 8; ++a) f[a] = (double *)malloc(sizeof(double) * N);
+  /* f: 0 gx, 1 gy, 2 gz, 3 curvH numerator, 4 dxx, 5 dyy, 6 dxy, 7 dtpsi */
+  or_rfft3(s->nx, s->ny, s->nz, s->psi, (double *)psiq);
+  for (int a = 0; a < 7; ++a) {
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nzh; ++k) {
+          int64_t id = KIDX(i, j, k);
+          double kx = s->vqx[i], ky = s->vqy[j], kz = s->vqz[k];
+          cplx m = 0;
+          switch (a) {
+          case 0: m = I * kx; break;
+          case 1: m = I * ky; break;
+          case 2: m = I * kz; break;
+          case 3: m = (I * kx) * (I * kx) + (I * ky) * (I * ky); break; /* :612-613 */
+          case 4: m = (I * kx) * (I * kx); break;                       /* :615 */
+          case 5: m = (I * ky) * (I * ky); break;                       /* :616 */
+          case 6: m = (I * ky) * (I * kx); break;                       /* :617 */
+          }
+          t[id] = sc * m * psiq[id];
+        }
+    or_irfft3(s->nx, s->ny, s->nz, (double *)t, f[a]);
+  }
+  or_dtpsi(s, f[7]);
+  int64_t nf = 0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      int track = 0;
+      const int64_t col = ((int64_t)i * s->ny + j) * s->nz;
+      for (int k = 0; k < s->nz; ++k) {
+        int64_t id = col + k;
+        if (s->psi[id] > arm && track == 0) track = 1;
+        if (s->psi[id] < 0 && track == 1) {
+          if (fabs(s->psi[id]) > fabs(s->psi[id - 1])) id = id - 1;
+          double g = sqrt(f[0][id] * f[0][id] + f[1][id] * f[1][id] + f[2][id] * f[2][id]);
+          int64_t c2 = (int64_t)i * s->ny + j;
+          surfz[c2] = k; /* the reference writes k, not the shifted index (:785) */
+          vel[c2] = vsgn * f[7][id] / g;
+          H[c2] = f[3][id] / g;
+          Kc[c2] = (f[4][id] * f[5][id] - f[6][id] * f[6][id]) / (g * g);
+          track = 2;
+          nf++;
+        }
+      }
+    }
+  *found = nf;
+  for (int a = 0; a < 8; ++a) free(f[a]);
+  free(psiq); free(t);
+  return 0;
+}
+
+/* Neumann surfTrack on the current state (cosNoAdvMorphCurv.cpp:744-762
+ * spectral second derivatives, :811-873 central differences, one-sided x2 at
+ * the walls, :877-961 column scan with Megrabov H and K). */
+static int surftrack_neumann(or_state *s, double arm, double vsgn, double *surfz, double *vel, double *H,
+                             double *Kc, int64_t *found) {
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  const int64_t N = s->N;
+  double *psiq = (double *)malloc(sizeof(double) * N), *t = (double *)malloc(sizeof(double) * N);
+  double *dxx = (double *)malloc(sizeof(double) * N), *dyy = (double *)malloc(sizeof(double) * N),
+         *dzz = (double *)malloc(sizeof(double) * N), *dt = (double *)malloc(sizeof(double) * N);
+  double *gx = (double *)malloc(sizeof(double) * N), *gy = (double *)malloc(sizeof(double) * N),
+         *gz = (double *)malloc(sizeof(double) * N);
+  or_redft10_3d(nx, ny, nz, s->psi, psiq);
+  for (int a = 0; a < 3; ++a) {
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+          int64_t id = ((int64_t)i * ny + j) * nz + k;
+          double q = a == 0 ? s->vqx[i] : (a == 1 ? s->vqy[j] : s->vqz[k]);
+          t[id] = -s->scale * q * q * psiq[id]; /* :746-750 */
+        }
+    or_redft01_3d(nx, ny, nz, t, a == 0 ? dxx : (a == 1 ? dyy : dzz));
+  }
+  or_dtpsi(s, dt);
+  const double tdx = 2 * s->dx, tdy = 2 * s->dy, tdz = 2 * s->dz; /* :167-169 */
+#define P(a, b, c) s->psi[((int64_t)(a) * ny + (b)) * nz + (c)]
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        int64_t id = ((int64_t)i * ny + j) * nz + k;
+        /* x from the current state (the reference uses a stale gathered copy, :811-841) */
+        if (i > 0 && i < nx - 1) gx[id] = (P(i + 1, j, k) - P(i - 1, j, k)) / tdx;
+        else if (i == 0) gx[id] = 2 * (P(1, j, k) - P(0, j, k)) / tdx;
+        else gx[id] = 2 * (P(i, j, k) - P(i - 1, j, k)) / tdx;
+        if (j != 0 && j != ny - 1) gy[id] = (P(i, j + 1, k) - P(i, j - 1, k)) / tdy; /* :859-865 */
+        else if (j == 0) gy[id] = 2 * (P(i, 1, k) - P(i, 0, k)) / tdy;
+        else gy[id] = 2 * (P(i, j, k) - P(i, j - 1, k)) / tdy;
+        if (k != 0 && k != nz - 1) gz[id] = (P(i, j, k + 1) - P(i, j, k - 1)) / tdz; /* :866-872 */
+        else if (k == 0) gz[id] = 2 * (P(i, j, 1) - P(i, j, 0)) / tdz;
+        else gz[id] = 2 * (P(i, j, k) - P(i, j, k - 1)) / tdz;
+      }
+  int64_t nf = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      int track = 0;
+      const int64_t col = ((int64_t)i * ny + j) * nz, c2 = (int64_t)i * ny + j;
+      for (int k = nz - 1; k > -1; --k) { /* :884 */
+        int64_t id = col + k;
+        if (s->psi[id] > arm && track == 0) track = 1;
+        if (s->psi[id] < 0.0 && track == 1) {
+          int k2 = k;
+          if (fabs(s->psi[id]) > fabs(s->psi[id + 1])) { id = id + 1; k2 = k + 1; } /* :898-902 */
+          surfz[c2] = k2;
+          double g = sqrt(gx[id] * gx[id] + gy[id] * gy[id] + gz[id] * gz[id]);
+          vel[c2] = vsgn * dt[id] / g; /* :909 */
+          double dxy;
+#define GX(jj, kk) gx[((int64_t)i * ny + (jj)) * nz + (kk)]
+          if (j > 0 && j < ny - 1) dxy = (GX(j + 1, k2) - GX(j - 1, k2)) / tdy; /* :913-924 */
+          else if (j == 0) dxy = 2 * (GX(1, k2) - GX(0, k2)) / tdy;
+          else dxy = 2 * (GX(j, k2) - GX(j - 1, k2)) / tdy;
+#undef GX
+          /* :926-940 chooses the stencil by k while centring it on index k2; a
+           * centred stencil that would leave the column falls back to one-sided. */
+          double dxz, dyz;
+          int lo = (int)(id - col) - 1, hi = (int)(id - col) + 1;
+          if (k > 0 && k < nz - 1 && lo >= 0 && hi <= nz - 1) {
+            dxz = (gx[id + 1] - gx[id - 1]) / tdz;
+            dyz = (gy[id + 1] - gy[id - 1]) / tdz;
+          } else if (k == 0 || lo < 0) {
+            dxz = 2 * (gx[id + 1] - gx[id]) / tdz;
+            dyz = 2 * (gy[id + 1] - gy[id]) / tdz;
+          } else {
+            dxz = 2 * (gx[id] - gx[id - 1]) / tdz;
+            dyz = 2 * (gy[id] - gy[id - 1]) / tdz;
+          }
+          double px = gx[id], py = gy[id], pz = gz[id];
+          double fxx = dxx[id], fyy = dyy[id], fzz = dzz[id];
+          H[c2] = ((py * py + pz * pz) * fxx + (px * px + pz * pz) * fyy + (px * px + py * py) * fzz -
+                   2 * (px * py * dxy + px * pz * dxz + py * pz * dyz)) /
+                  (g * g * g); /* :945-952 */
+          Kc[c2] = (pz * pz * (fxx * fyy - dxy * dxy) + px * px * (fyy * fzz - dyz * dyz) +
+                    py * py * (fxx * fzz - dxz * dxz) +
+                    2 * (py * dxy * (pz * dxz - px * fzz) + px * dxz * (py * dyz - pz * fyy) +
+                         pz * dyz * (px * dxy - py * fxx))) /
+                   (g * g * g * g); /* :954-961 */
+          track = 2;
+          nf++;
+        }
+      }
+    }
+#undef P
+  *found = nf;
+  free(psiq); free(t); free(dxx); free(dyy); free(dzz); free(dt); free(gx); free(gy); free(gz);
+  return 0;
+}
+
+int or_surftrack(or_state *s, double arm, double vel_sign, double *surfz, double *vel, double *H, double *K,
+                 int64_t *found) {
+  if (!s->has_state) return SMX_ERR_STATE;
+  if (s->c.bc == SMX_BC_PERIODIC) return surftrack_periodic(s, arm, vel_sign, surfz, vel, H, K, found);
+  return surftrack_neumann(s, arm, vel_sign, surfz, vel, H, K, found);
+}
+
+/* ======================================================================
+ * Initial conditions
+ * ==================================================================== */
+
+/* Counter-based SplitMix64: the idx-th output of the stream seeded with
+ * seed ^ box; u = (x >> 11) * 2^-53 in [0,1). */
+double or_splitmix_uniform(uint64_t seed, uint64_t box, uint64_t idx) {
+  uint64_t z = (seed ^ box) + (idx + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void or_ic_noise(int nx, int ny, int nz, uint64_t seed, uint64_t box, double amp, double *psi) {
+  const int64_t N = (int64_t)nx * ny * nz;
+  for (int64_t i = 0; i < N; ++i) psi[i] = amp * (2.0 * or_splitmix_uniform(seed, box, (uint64_t)i) - 1.0);
+}
+
+/* psi0 = a cos(q0 sqrt(max(x^2+y^2-z^2, 0) + 4)) about the box centre plus
+ * uniform noise of amplitude `noise`. */
+void or_ic_hyperboloid(int nx, int ny, int nz, double dx, double dy, double dz, double q0, double a,
+                       double noise, uint64_t seed, double *psi) {
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        int64_t id = ((int64_t)i * ny + j) * nz + k;
+        double x = (i - nx / 2) * dx, y = (j - ny / 2) * dy, z = (k - nz / 2) * dz;
+        double r2 = x * x + y * y - z * z;
+        if (r2 < 0) r2 = 0;
+        psi[id] = a * cos(q0 * sqrt(r2 + 4.0)) + noise * (2.0 * or_splitmix_uniform(seed, 0, (uint64_t)id) - 1.0);
+      }
+}
+
+/* pf3dVal.cpp:214-326 */
+int or_growth_sweep(const smx_config *cfg, double mq0, double dmq, int nq, double *growth) {
+  or_state *s = or_create(cfg);
+  if (!s) return SMX_ERR_INVALID;
+  double *psi = (double *)malloc(sizeof(double) * s->N);
+  for (int q = 0; q < nq; ++q) {
+    double mq = mq0 + q * dmq;
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nz; ++k) psi[((int64_t)i * s->ny + j) * s->nz + k] = 1e-4 * cos(mq * k * s->dz);
+    or_set_psi(s, psi);
+    or_step(s, 3);
+    int64_t c = ((int64_t)(s->nx / 2) * s->ny + s->ny / 2) * s->nz + s->nz / 2;
+    growth[q] = (s->psi[c] - s->psi_prev[c]) / (s->c.dt * s->psi_prev[c]); /* :320 */
+  }
+  free(psi);
+  or_destroy(s);
+  return 0;
+}

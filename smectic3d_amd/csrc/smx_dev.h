@@ -1,0 +1,248 @@
+// smx_dev.h -- device building blocks for libsmectic (gfx950).
+//
+//  * cplx<T>: 2-component complex in registers / LDS / HBM.
+//  * dft<R,SIGN>: in-register radix-R DFT, R in {2,3,4,6,8,12,16}.
+//  * BlockFFT: batched length-L Stockham autosort FFT of NC columns held in an
+//    LDS tile laid out [pos][col] (col fastest).  Thread (col, tc) owns E =
+//    L/TPC elements; pass p with radix R computes E/R butterflies per thread.
+//    Inputs of the first pass sit at positions tc + TPC*u + r*(L/R_first);
+//    outputs of the last pass at tc + TPC*u + r*(L/R_last), so a forward
+//    transform followed by an inverse transform with the radix list reversed
+//    needs no data exchange in between.
+//
+// Stockham pass (Govindaraju et al., SC'08, "High performance discrete Fourier
+// transforms on graphics processors"): butterfly j reads X[j + r L/R],
+// multiplies by w^{r (j mod Ns)} with w = e^{sign 2 pi i/(Ns R)}, does a
+// radix-R DFT and writes Y[(j/Ns) Ns R + (j mod Ns) + r Ns].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smx {
+
+template <typename T>
+struct __attribute__((aligned(2 * sizeof(T)))) cplx {
+  T x, y;
+};
+
+template <typename T> __device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ cplx<T> operator*(T s, cplx<T> a) { return mk(s * a.x, s * a.y); }
+template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __device__ __forceinline__ cplx<T> conjg(cplx<T> a) { return mk(a.x, -a.y); }
+// multiply by SIGN*i
+template <int SIGN, typename T> __device__ __forceinline__ cplx<T> mul_si(cplx<T> a) {
+  return SIGN > 0 ? mk(-a.y, a.x) : mk(a.y, -a.x);
+}
+
+// cos(2*pi*m/48) and sin(2*pi*m/48), m = 0..47, rounded from extended precision
+__device__ constexpr double kRootCos48[48] = {1.0, 0.9914448613738104, 0.9659258262890683, 0.9238795325112867, 0.8660254037844386, 0.7933533402912352, 0.7071067811865476, 0.6087614290087207, 0.5, 0.3826834323650898, 0.25881904510252074, 0.1305261922200516, 0.0, -0.1305261922200516, -0.25881904510252074, -0.3826834323650898, -0.5, -0.6087614290087207, -0.7071067811865476, -0.7933533402912352, -0.8660254037844386, -0.9238795325112867, -0.9659258262890683, -0.9914448613738104, -1.0, -0.9914448613738104, -0.9659258262890683, -0.9238795325112867, -0.8660254037844386, -0.7933533402912352, -0.7071067811865476, -0.6087614290087207, -0.5, -0.3826834323650898, -0.25881904510252074, -0.1305261922200516, 0.0, 0.1305261922200516, 0.25881904510252074, 0.3826834323650898, 0.5, 0.6087614290087207, 0.7071067811865476, 0.7933533402912352, 0.8660254037844386, 0.9238795325112867, 0.9659258262890683, 0.9914448613738104};
+__device__ constexpr double kRootSin48[48] = {0.0, 0.1305261922200516, 0.25881904510252074, 0.3826834323650898, 0.5, 0.6087614290087207, 0.7071067811865476, 0.7933533402912352, 0.8660254037844386, 0.9238795325112867, 0.9659258262890683, 0.9914448613738104, 1.0, 0.9914448613738104, 0.9659258262890683, 0.9238795325112867, 0.8660254037844386, 0.7933533402912352, 0.7071067811865476, 0.6087614290087207, 0.5, 0.3826834323650898, 0.25881904510252074, 0.1305261922200516, 0.0, -0.1305261922200516, -0.25881904510252074, -0.3826834323650898, -0.5, -0.6087614290087207, -0.7071067811865476, -0.7933533402912352, -0.8660254037844386, -0.9238795325112867, -0.9659258262890683, -0.9914448613738104, -1.0, -0.9914448613738104, -0.9659258262890683, -0.9238795325112867, -0.8660254037844386, -0.7933533402912352, -0.7071067811865476, -0.6087614290087207, -0.5, -0.3826834323650898, -0.25881904510252074, -0.1305261922200516};
+
+// e^{SIGN 2 pi i q / R} times a, for compile-time q, R (R divides 48)
+template <int R, int Q, int SIGN, typename T>
+__device__ __forceinline__ cplx<T> rot(cplx<T> a) {
+  constexpr int m = ((Q % R) * (48 / R)) % 48;
+  if constexpr (m == 0) return a;
+  else if constexpr (m == 12) return mul_si<SIGN>(a);
+  else if constexpr (m == 24) return mk(-a.x, -a.y);
+  else if constexpr (m == 36) return mul_si<-SIGN>(a);
+  else {
+    const T c = (T)kRootCos48[m];
+    const T s = (T)(SIGN * kRootSin48[m]);
+    return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+  }
+}
+
+template <int R, int SIGN, typename T> __device__ __forceinline__ void dft(cplx<T>* a);
+
+template <int R, int SIGN, int Q, typename T>
+__device__ __forceinline__ void combine_even_odd(cplx<T>* a, const cplx<T>* e, const cplx<T>* o) {
+  if constexpr (Q < R / 2) {
+    cplx<T> t = rot<R, Q, SIGN>(o[Q]);
+    a[Q] = e[Q] + t;
+    a[Q + R / 2] = e[Q] - t;
+    combine_even_odd<R, SIGN, Q + 1>(a, e, o);
+  }
+}
+
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft(cplx<T>* a) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    cplx<T> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+  } else if constexpr (R == 3) {
+    const T s3 = (T)(SIGN * 0.8660254037844386);
+    cplx<T> t = a[1] + a[2], d = a[1] - a[2];
+    cplx<T> m = mk(a[0].x - (T)0.5 * t.x, a[0].y - (T)0.5 * t.y);
+    cplx<T> sd = mk(-s3 * d.y, s3 * d.x);
+    a[0] = a[0] + t;
+    a[1] = m + sd;
+    a[2] = m - sd;
+  } else if constexpr (R == 4) {
+    cplx<T> t0 = a[0] + a[2], t1 = a[0] - a[2], t2 = a[1] + a[3], t3 = mul_si<SIGN>(a[1] - a[3]);
+    a[0] = t0 + t2;
+    a[2] = t0 - t2;
+    a[1] = t1 + t3;
+    a[3] = t1 - t3;
+  } else {
+    static_assert(R % 2 == 0, "radix");
+    cplx<T> e[R / 2], o[R / 2];
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      e[q] = a[2 * q];
+      o[q] = a[2 * q + 1];
+    }
+    dft<R / 2, SIGN>(e);
+    dft<R / 2, SIGN>(o);
+    combine_even_odd<R, SIGN, 0>(a, e, o);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Radix plans.  Fwd and Inv list the same radices in opposite orders.
+template <int... Rs> struct Radices {};
+template <int R0, int... Rs> struct First { static constexpr int value = R0; };
+template <int R0, int... Rs> struct Last { static constexpr int value = Last<Rs...>::value; };
+template <int R0> struct Last<R0> { static constexpr int value = R0; };
+template <typename P> struct RadInfo;
+template <int... Rs> struct RadInfo<Radices<Rs...>> {
+  static constexpr int first = First<Rs...>::value;
+  static constexpr int last = Last<Rs...>::value;
+};
+
+// L -> (E, forward radices, inverse radices)
+template <int L> struct Plan;
+#define SMX_PLAN(L_, E_, FWD, INV)   \
+  template <> struct Plan<L_> {     \
+    static constexpr int E = E_;    \
+    using Fwd = Radices<FWD>;       \
+    using Inv = Radices<INV>;       \
+  };
+#define SMX_COMMA ,
+SMX_PLAN(8, 8, 8, 8)
+SMX_PLAN(12, 12, 4 SMX_COMMA 3, 3 SMX_COMMA 4)
+SMX_PLAN(16, 16, 16, 16)
+SMX_PLAN(24, 12, 4 SMX_COMMA 6, 6 SMX_COMMA 4)
+SMX_PLAN(32, 8, 8 SMX_COMMA 4, 4 SMX_COMMA 8)
+SMX_PLAN(48, 12, 4 SMX_COMMA 12, 12 SMX_COMMA 4)
+SMX_PLAN(64, 16, 16 SMX_COMMA 4, 4 SMX_COMMA 16)
+SMX_PLAN(96, 12, 4 SMX_COMMA 4 SMX_COMMA 6, 6 SMX_COMMA 4 SMX_COMMA 4)
+SMX_PLAN(128, 16, 16 SMX_COMMA 8, 8 SMX_COMMA 16)
+SMX_PLAN(192, 12, 4 SMX_COMMA 4 SMX_COMMA 12, 12 SMX_COMMA 4 SMX_COMMA 4)
+SMX_PLAN(256, 16, 16 SMX_COMMA 16, 16 SMX_COMMA 16)
+SMX_PLAN(384, 12, 4 SMX_COMMA 4 SMX_COMMA 4 SMX_COMMA 6, 6 SMX_COMMA 4 SMX_COMMA 4 SMX_COMMA 4)
+SMX_PLAN(512, 16, 16 SMX_COMMA 16 SMX_COMMA 2, 2 SMX_COMMA 16 SMX_COMMA 16)
+#undef SMX_PLAN
+
+// ---------------------------------------------------------------------------
+// Block FFT over an LDS tile [pos][col] with row pitch NCP (>= number of columns).
+template <typename T, int L, int E, int NCP>
+struct Tile {
+  static constexpr int TPC = L / E;
+  // element e of a thread for a pass of radix R: butterfly u = e / R, leg r = e % R
+  template <int R> __device__ static __forceinline__ int pos_in(int tc, int e) {
+    return tc + TPC * (e / R) + (e % R) * (L / R);
+  }
+  template <int R> __device__ static __forceinline__ void load(cplx<T>* v, const cplx<T>* lds, int col, int tc) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = lds[pos_in<R>(tc, e) * NCP + col];
+  }
+  template <int R> __device__ static __forceinline__ void store(const cplx<T>* v, cplx<T>* lds, int col, int tc) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[pos_in<R>(tc, e) * NCP + col] = v[e];
+  }
+};
+
+template <typename T, int L, int E, int NCP, int SIGN, int Ns, int R, int... Rest>
+__device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, int tc, const cplx<T>* __restrict__ tw) {
+  constexpr int TPC = L / E;
+  static_assert(E % R == 0, "E must be a multiple of every radix");
+#pragma unroll
+  for (int u = 0; u < E / R; ++u) {
+    const int b = tc + TPC * u;
+    if constexpr (Ns > 1) {
+      const int k = b % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cplx<T> w = tw[r * k * (L / (Ns * R))];
+        if (SIGN > 0) w.y = -w.y;
+        v[u * R + r] = cmul(v[u * R + r], w);
+      }
+    }
+    dft<R, SIGN>(&v[u * R]);
+  }
+  if constexpr (sizeof...(Rest) > 0) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int b = tc + TPC * u;
+      const int k = b % Ns;
+      const int base = (b - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[(base + r * Ns) * NCP + col] = v[u * R + r];
+    }
+    __syncthreads();
+    constexpr int Rn = First<Rest...>::value;
+    Tile<T, L, E, NCP>::template load<Rn>(v, lds, col, tc);
+    fft_passes<T, L, E, NCP, SIGN, Ns * R, Rest...>(v, lds, col, tc, tw);
+  }
+}
+
+template <typename T, int L, int NCP, int SIGN, typename Rad> struct BlockFFT;
+template <typename T, int L, int NCP, int SIGN, int... Rs>
+struct BlockFFT<T, L, NCP, SIGN, Radices<Rs...>> {
+  static constexpr int E = Plan<L>::E;
+  // v: first-pass input positions on entry; last-pass output positions on exit.
+  // The LDS tile is used for the exchanges; all threads must have finished any
+  // earlier reads of it (the first exchange starts with a barrier).
+  __device__ static __forceinline__ void run(cplx<T>* v, cplx<T>* lds, int col, int tc, const cplx<T>* __restrict__ tw) {
+    fft_passes<T, L, E, NCP, SIGN, 1, Rs...>(v, lds, col, tc, tw);
+  }
+};
+
+// forward (SIGN = -1) with Plan::Fwd, inverse (SIGN = +1) with Plan::Inv
+template <typename T, int L, int NCP> using FwdFFT = BlockFFT<T, L, NCP, -1, typename Plan<L>::Fwd>;
+template <typename T, int L, int NCP> using InvFFT = BlockFFT<T, L, NCP, +1, typename Plan<L>::Inv>;
+template <int L> constexpr int fwd_first() { return RadInfo<typename Plan<L>::Fwd>::first; }
+template <int L> constexpr int fwd_last() { return RadInfo<typename Plan<L>::Fwd>::last; }
+template <int L> constexpr int inv_first() { return RadInfo<typename Plan<L>::Inv>::first; }
+template <int L> constexpr int inv_last() { return RadInfo<typename Plan<L>::Inv>::last; }
+
+// ---------------------------------------------------------------------------
+// FP64 block reduction helpers (wavefront shuffles, then LDS across waves).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sum over the block; result valid in thread 0.  scratch: >= max(nwaves, 64)
+// doubles when blockDim.x is not a multiple of 64, else >= nwaves doubles.
+// Shuffles are used only when every wave of the block is full.
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((blockDim.x & 63) != 0) {  // small blocks (tiny grids): plain LDS sum, blockDim.x <= 64 here
+    scratch[threadIdx.x] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+      for (int i = 0; i < (int)blockDim.x; ++i) s += scratch[i];
+    return s;
+  }
+  v = wave_sum(v);
+  if (lane == 0) scratch[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < nw; ++i) s += scratch[i];
+  return s;
+}
+
+}  // namespace smx

@@ -1,0 +1,3 @@
+// Instantiates the FP32 pass kernels (FP32-mixed path).
+#include "smx_launch.h"
+template struct smx::PeriodicOps<float>;

@@ -1,0 +1,94 @@
+// smx_launch.h -- runtime-length -> template-instance dispatch for the pass kernels.
+// Instantiated once per precision in smx_inst_f32.hip / smx_inst_f64.hip.
+#pragma once
+#include "smx_periodic.h"
+
+namespace smx {
+
+constexpr int kNC = 16;  // columns (pencils) per tile
+
+// Grid lengths with a radix plan.  Every axis length must be one of these.
+#define SMX_FOR_EACH_LENGTH(X) \
+  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+inline bool length_supported(int n) {
+#define SMX_CASE(L) if (n == L) return true;
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return false;
+}
+
+template <typename T>
+struct PeriodicOps {
+  // y pass, in place allowed.  sign -1 forward, +1 inverse.
+  static hipError_t ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
+                          const cplx<T>* tw, hipStream_t st);
+  // x pass (mode: XMode), tiles over (y, kz).
+  static hipError_t xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx<T>* rbuf, const cplx<T>* psi0,
+                          const PParams<T>& p, hipStream_t st);
+  // z pass.  in_mode: ZIn, out_mode: ZOut bit set, op: ZOp.
+  static hipError_t zpass(int in_mode, int out_mode, int op, const cplx<T>* in, cplx<T>* out,
+                          const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st);
+  static int zpass_blocks(const PParams<T>& p) { return (int)((p.nxy + kNC - 1) / kNC); }
+};
+
+template <typename T>
+hipError_t PeriodicOps<T>::ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
+                                 const cplx<T>* tw, hipStream_t st) {
+  const dim3 grid((unsigned)((npencils + kNC - 1) / kNC));
+#define SMX_CASE(L)                                                                           \
+  if (ny == L) {                                                                              \
+    const dim3 block(kNC * (L / Plan<L>::E));                                                 \
+    if (sign < 0) hipLaunchKernelGGL((k_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw); \
+    else hipLaunchKernelGGL((k_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw);          \
+    return hipGetLastError();                                                                 \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx<T>* rbuf, const cplx<T>* psi0,
+                                 const PParams<T>& p, hipStream_t st) {
+  const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
+#define SMX_CASE(L)                                                                                   \
+  if (p.nx == L) {                                                                                    \
+    const dim3 block(kNC * (L / Plan<L>::E));                                                         \
+    switch (mode) {                                                                                   \
+      case X_STEP: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_STEP>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      case X_FWD_SCALED: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      case X_BOOT: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      case X_INV: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      default: return hipErrorInvalidValue;                                                           \
+    }                                                                                                 \
+    return hipGetLastError();                                                                         \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T>* in, cplx<T>* out,
+                                 const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st) {
+  const dim3 grid((unsigned)zpass_blocks(p));
+#define SMX_CASE(L)                                                                                  \
+  if (p.nz == L) {                                                                                   \
+    const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                  \
+    if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)                                              \
+      hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
+    else if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                         \
+      hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
+    else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
+      hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
+    else                                                                                             \
+      return hipErrorInvalidValue;                                                                   \
+    return hipGetLastError();                                                                        \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace smx

@@ -1,0 +1,294 @@
+// smx_periodic.h -- periodic (R2C/C2R) pass kernels of the Swift-Hohenberg step.
+//
+// HBM layout of every spectral-stage array: [kz][x or kx][y or ky], y fastest,
+// kz in [0, nz/2] (nz/2+1 planes).  A step is four in-place launches on the
+// stage buffer A plus the CNAB2 carry R:
+//   k_ypass<+1>  (x,ky,kz) -> (x,y,kz)       inverse y
+//   k_zpass      (x,y,kz)  -> (x,y,kz)       z C2R, N(psi), z R2C
+//   k_ypass<-1>  (x,y,kz)  -> (x,ky,kz)      forward y
+//   k_xpass      (x,ky,kz) -> (x,ky,kz)      forward x, CNAB2 update, inverse x
+// Real fields (host I/O, diagnostics) are dense [x][y][z] in FP64.
+#pragma once
+#include <type_traits>
+
+#include "smx_dev.h"
+
+namespace smx {
+
+// Runtime parameters shared by the periodic kernels.
+template <typename T>
+struct PParams {
+  int nx, ny, nz, nzh;
+  int64_t nxy;      // nx*ny (columns of a kz plane)
+  T scale;          // 1/(nx ny nz), pf3d.cpp:62
+  T eps, alpha, q02, beta, gamma, dtd2;
+  int nl_model;     // 0 quintic, 1 cubic
+  int mid;          // mid-plane penalty z index, <0 off
+  T pen;            // 2 w_mid / dz^2
+  const T* kx;      // wavenumbers, FFT order, Nyquist +pi/dx (pf3d.cpp:162-190)
+  const T* ky;
+  const T* kz;
+  const cplx<T>* twx;  // e^{-2 pi i m/n} tables, m < n
+  const cplx<T>* twy;
+  const cplx<T>* twz;
+  // FP64 mean (k = 0) mode of the mixed path: partial sums written by the z
+  // pass, consumed by the x pass.  Unused (nullptr) in FP64 mode.
+  double* partial;
+  int npartial;
+  double* mean_state;  // [0] = R(k=0) in FP64, [1] = psi_hat(0) in FP64
+  double m_scale, m_dtd2, m_c1, m_c2;  // FP64 scale, dt/2, C1(0), C2(0) for the mean mode
+};
+
+// Local nonlinearity: beta psi^3 - gamma psi^5 (pf3d.cpp:460-461) or the cubic model.
+template <typename T>
+__device__ __forceinline__ T nonlin(const PParams<T>& p, T s) {
+  if (p.nl_model == 1) return s * s * (p.beta - p.gamma * s);
+  T s2 = s * s;
+  return s2 * s * (p.beta - p.gamma * s2);
+}
+
+// ---------------------------------------------------------------------------
+// y pass: contiguous pencils of length L = ny; pencils g = kz*nx + x.
+template <typename T, int L, int SIGN, int NC>
+__global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_ypass(const cplx<T>* in, cplx<T>* out,
+                                                               int64_t npencils,
+                                                               const cplx<T>* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
+  __shared__ cplx<T> S[L * NCP];
+  const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
+  const int64_t p0 = (int64_t)blockIdx.x * NC;
+  const int nvalid = (int)((npencils - p0) < NC ? (npencils - p0) : NC);
+  const cplx<T>* src = in + p0 * L;
+  cplx<T>* dst = out + p0 * L;
+  for (int i = tid; i < NC * L; i += NT) {
+    const int c = i / L, pos = i - c * L;
+    S[pos * NCP + c] = (c < nvalid) ? src[i] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  cplx<T> v[E];
+  using FFT = BlockFFT<T, L, NCP, SIGN, typename std::conditional<(SIGN < 0), typename Plan<L>::Fwd, typename Plan<L>::Inv>::type>;
+  constexpr int RF = (SIGN < 0) ? fwd_first<L>() : inv_first<L>();
+  constexpr int RL = (SIGN < 0) ? fwd_last<L>() : inv_last<L>();
+  Tile<T, L, E, NCP>::template load<RF>(v, S, col, tc);
+  FFT::run(v, S, col, tc, tw);
+  __syncthreads();
+  Tile<T, L, E, NCP>::template store<RL>(v, S, col, tc);
+  __syncthreads();
+  for (int i = tid; i < NC * L; i += NT) {
+    const int c = i / L, pos = i - c * L;
+    if (c < nvalid) dst[i] = S[pos * NCP + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1, rows x.
+enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
+
+template <typename T, int L, int NC, int MODE>
+__global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
+                                                               cplx<T>* __restrict__ rbuf,
+                                                               const cplx<T>* __restrict__ psi0,
+                                                               PParams<T> p) {
+  constexpr int E = Plan<L>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
+  __shared__ cplx<T> S[L * NCP];
+  __shared__ double red[64];
+  const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
+  const int kz = blockIdx.y, y0 = blockIdx.x * NC;
+  const int64_t plane = (int64_t)kz * p.nxy;
+  for (int i = tid; i < NC * L; i += NT) {
+    const int pos = i / NC, c = i - pos * NC;
+    S[pos * NCP + c] = in[plane + (int64_t)pos * p.ny + y0 + c];
+  }
+  // FP64 mean mode (mixed path): the block owning k = 0 sums the z-pass partials.
+  double nsum = 0.0;
+  const bool owner0 = (p.partial != nullptr) && kz == 0 && y0 == 0;
+  if (owner0) {
+    double a = 0.0;
+    for (int i = tid; i < p.npartial; i += NT) a += p.partial[i];
+    nsum = block_sum(a, red);  // valid in thread 0 (= col 0, tc 0)
+  }
+  __syncthreads();
+  cplx<T> v[E];
+  if constexpr (MODE == X_INV) {
+    Tile<T, L, E, NCP>::template load<inv_first<L>()>(v, S, col, tc);
+  } else {
+    Tile<T, L, E, NCP>::template load<fwd_first<L>()>(v, S, col, tc);
+    FwdFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
+    const T kyv = p.ky[y0 + col], kzv = p.kz[kz];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e);
+      const int64_t idx = plane + (int64_t)kxi * p.ny + y0 + col;
+      if constexpr (MODE == X_FWD_SCALED) {
+        v[e] = p.scale * v[e];
+        if (owner0 && tid == 0 && kxi == 0) {
+          p.mean_state[1] = p.m_scale * nsum;  // FP64 mean of the transformed field
+          v[e] = mk((T)p.mean_state[1], (T)0);
+        }
+      } else {
+        const T kxv = p.kx[kxi];
+        const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+        const T d = k2 - p.q02;
+        const T aLin = p.eps - p.alpha * d * d;  // pf3d.cpp:205-206
+        const T C1 = (T)1 + p.dtd2 * aLin, C2 = (T)1 - p.dtd2 * aLin;  // :214-215
+        const cplx<T> nn = p.scale * v[e];
+        if constexpr (MODE == X_STEP) {
+          // psi_new = (C1 psi + dt/2 (3 N - N_past))/C2 carried as R = C1 psi - dt/2 N_past
+          // (pf3d.cpp:476-478): psi_new = (R + 3 dt/2 N)/C2, R' = C1 psi_new - dt/2 N.
+          const cplx<T> r = rbuf[idx];
+          const T ic2 = (T)1 / C2;
+          cplx<T> ps = mk(ic2 * (r.x + (T)3 * p.dtd2 * nn.x), ic2 * (r.y + (T)3 * p.dtd2 * nn.y));
+          cplx<T> rn = mk(C1 * ps.x - p.dtd2 * nn.x, C1 * ps.y - p.dtd2 * nn.y);
+          if (owner0 && tid == 0 && kxi == 0) {
+            // k = 0 in FP64: N_hat(0) = sum of N over the grid.
+            const double dd = p.m_dtd2, n0 = p.m_scale * nsum;
+            const double ps0 = (p.mean_state[0] + 3.0 * dd * n0) / p.m_c2;
+            p.mean_state[0] = p.m_c1 * ps0 - dd * n0;
+            p.mean_state[1] = ps0;
+            ps = mk((T)ps0, (T)0);
+            rn = mk((T)p.mean_state[0], (T)0);
+          }
+          rbuf[idx] = rn;
+          v[e] = ps;
+        } else {  // X_BOOT: R = C1 psi0 - dt/2 N_past with N_past = N(psi0) (pf3d.cpp:363-374)
+          cplx<T> ps = psi0[idx];
+          cplx<T> rn = mk(C1 * ps.x - p.dtd2 * nn.x, C1 * ps.y - p.dtd2 * nn.y);
+          if (owner0 && tid == 0 && kxi == 0) {
+            const double ps0 = p.mean_state[1];  // set from the FP64 sum of psi0
+            p.mean_state[0] = p.m_c1 * ps0 - p.m_dtd2 * p.m_scale * nsum;
+            ps = mk((T)ps0, (T)0);
+            rn = mk((T)p.mean_state[0], (T)0);
+          }
+          rbuf[idx] = rn;
+          v[e] = ps;
+        }
+      }
+    }
+  }
+  if constexpr (MODE == X_FWD_SCALED) {
+    __syncthreads();
+    Tile<T, L, E, NCP>::template store<fwd_last<L>()>(v, S, col, tc);
+  } else {
+    InvFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
+    __syncthreads();
+    Tile<T, L, E, NCP>::template store<inv_last<L>()>(v, S, col, tc);
+  }
+  __syncthreads();
+  for (int i = tid; i < NC * L; i += NT) {
+    const int pos = i / NC, c = i - pos * NC;
+    out[plane + (int64_t)pos * p.ny + y0 + c] = S[pos * NCP + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z pass.  Tile: columns f0 .. f0+NC-1 of the (x,y) plane; column pairs
+// (q, q + NC/2) share one length-nz complex FFT (z = psi_a + i psi_b).
+enum ZIn { Z_IN_SPEC = 0, Z_IN_REAL = 1 };
+enum ZOut { Z_OUT_SPEC = 1, Z_OUT_REAL = 2 };
+enum ZOp { Z_OP_ID = 0, Z_OP_NL = 1 };
+
+template <typename T, int L, int NC, int IN, int OUT>
+__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cplx<T>* in, cplx<T>* out,
+                                                                       const double* __restrict__ real_in,
+                                                                       double* __restrict__ real_out,
+                                                                       int op, PParams<T> p) {
+  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
+  constexpr int NZH = L / 2 + 1, SP = NC + 1, FP = NPR + 1;
+  __shared__ cplx<T> S[NZH * SP];
+  __shared__ cplx<T> F[L * FP];
+  __shared__ double red[64];
+  const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
+  const int64_t f0 = (int64_t)blockIdx.x * NC;
+  const int nvalid = (int)((p.nxy - f0) < NC ? (p.nxy - f0) : NC);
+  cplx<T> v[E];
+  if constexpr (IN == Z_IN_SPEC) {
+    for (int i = tid; i < NZH * NC; i += NT) {
+      const int k = i / NC, c = i - k * NC;
+      S[k * SP + c] = (c < nvalid) ? in[(int64_t)k * p.nxy + f0 + c] : mk<T>(0, 0);
+    }
+    __syncthreads();
+    // Hermitian extension of the pair; only the real parts of kz = 0 and
+    // kz = nz/2 enter (the half-complex stage of an FFTW c2r).
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
+      if (n <= L / 2) {
+        cplx<T> A = S[n * SP + q], B = S[n * SP + q + NPR];
+        if (n == 0 || n == L / 2) { A.y = 0; B.y = 0; }
+        v[e] = mk(A.x - B.y, A.y + B.x);
+      } else {
+        const int m = L - n;
+        const cplx<T> A = S[m * SP + q], B = S[m * SP + q + NPR];
+        v[e] = mk(A.x + B.y, B.x - A.y);
+      }
+    }
+    InvFFT<T, L, FP>::run(v, F, q, tc, p.twz);
+  } else {
+    for (int i = tid; i < NC * L; i += NT) {
+      const int c = i / L, n = i - c * L;
+      const T val = (c < nvalid) ? (T)real_in[(f0 + c) * L + n] : (T)0;
+      if (c < NPR) F[n * FP + c].x = val;
+      else F[n * FP + c - NPR].y = val;
+    }
+    __syncthreads();
+    Tile<T, L, E, FP>::template load<fwd_first<L>()>(v, F, q, tc);
+  }
+  // v now holds (psi_a, psi_b) at positions pos_in<fwd_first>.
+  constexpr int RP = fwd_first<L>();
+  static_assert(fwd_first<L>() == inv_last<L>(), "plan symmetry");
+  if constexpr ((OUT & Z_OUT_REAL) != 0) {
+    __syncthreads();
+    Tile<T, L, E, FP>::template store<RP>(v, F, q, tc);
+    __syncthreads();
+    for (int i = tid; i < NC * L; i += NT) {
+      const int c = i / L, n = i - c * L;
+      if (c < nvalid) real_out[(f0 + c) * L + n] = (double)(c < NPR ? F[n * FP + c].x : F[n * FP + c - NPR].y);
+    }
+  }
+  if constexpr ((OUT & Z_OUT_SPEC) != 0) {
+    const bool pen = (op == Z_OP_NL) && p.mid > 0 && p.mid < L - 1;
+    if (pen) {
+      __syncthreads();
+      Tile<T, L, E, FP>::template store<RP>(v, F, q, tc);
+      __syncthreads();
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (op == Z_OP_NL) {
+        const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
+        T a = nonlin(p, v[e].x), b = nonlin(p, v[e].y);
+        if (pen && n == p.mid) {
+          // 2 wMid (psi_{k-1} - 2 psi_k + psi_{k+1})/dz^2 at k = mid (pf3dMorph.cpp:572-581)
+          const cplx<T> lo = F[(n - 1) * FP + q], hi = F[(n + 1) * FP + q];
+          a += p.pen * (lo.x - (T)2 * v[e].x + hi.x);
+          b += p.pen * (lo.y - (T)2 * v[e].y + hi.y);
+        }
+        v[e] = mk(a, b);
+      }
+      acc += (double)v[e].x + (double)v[e].y;
+    }
+    if (p.partial != nullptr) {
+      const double s = block_sum(acc, red);
+      if (tid == 0) p.partial[blockIdx.x] = s;
+    }
+    FwdFFT<T, L, FP>::run(v, F, q, tc, p.twz);
+    __syncthreads();
+    Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
+    __syncthreads();
+    // split W = X + iY:  X = (W_k + conj W_{-k})/2,  Y = (W_k - conj W_{-k})/(2i)
+    for (int i = tid; i < NZH * NPR; i += NT) {
+      const int k = i / NPR, qq = i - k * NPR;
+      const cplx<T> W = F[k * FP + qq], Wm = F[((L - k) % L) * FP + qq];
+      S[k * SP + qq] = mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y));
+      S[k * SP + qq + NPR] = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
+    }
+    __syncthreads();
+    for (int i = tid; i < NZH * NC; i += NT) {
+      const int k = i / NC, c = i - k * NC;
+      if (c < nvalid) out[(int64_t)k * p.nxy + f0 + c] = S[k * SP + c];
+    }
+  }
+}
+
+}  // namespace smx

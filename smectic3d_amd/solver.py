@@ -1,0 +1,116 @@
+"""Host-side mirror of the reference time loop over the libsmectic C ABI.
+
+``Solver`` plays the role of one reference driver's state (psi, NqPast, the
+operator tables and plans) with the step operator's parameters given at run
+time instead of compile time (periodicFFT/pf3d.cpp:35-75).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._lib import check, lib
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Solver:
+    def __init__(self, cfg):
+        if not isinstance(cfg, _abi.SmxConfig):
+            raise TypeError("cfg must be an SmxConfig")
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        check(lib().smx_create(C.byref(cfg), C.byref(self._h)))
+        self.shape = (cfg.nx, cfg.ny, cfg.nz)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().smx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state ----
+    def set_psi(self, psi, row_stride=None):
+        """psi: (nx, ny, nz) array, or a flat array in the reference layout with row_stride."""
+        nz = self.cfg.nz
+        if row_stride is None:
+            psi = np.ascontiguousarray(psi, dtype=np.float64).reshape(self.shape)
+            row_stride = nz
+        else:
+            psi = np.ascontiguousarray(psi, dtype=np.float64)
+        check(lib().smx_set_psi(self._h, _ptr(psi), row_stride))
+
+    def get_psi(self, row_stride=None):
+        nx, ny, nz = self.shape
+        if row_stride is None:
+            out = np.empty(self.shape)
+            check(lib().smx_get_psi(self._h, _ptr(out), nz))
+        else:
+            out = np.zeros(nx * ny * row_stride)
+            check(lib().smx_get_psi(self._h, _ptr(out), row_stride))
+        return out
+
+    def step(self, n=1):
+        check(lib().smx_step(self._h, int(n)))
+
+    def sync(self):
+        check(lib().smx_sync(self._h))
+
+    def step_count(self):
+        n = C.c_int64()
+        check(lib().smx_step_count(self._h, C.byref(n)))
+        return n.value
+
+    # ---- diagnostics ----
+    def _scalar(self, fn):
+        v = C.c_double()
+        check(fn(self._h, C.byref(v)))
+        return v.value
+
+    def l1(self):
+        return self._scalar(lib().smx_l1)
+
+    def energy(self):
+        return self._scalar(lib().smx_energy)
+
+    def dominant_wavenumber(self):
+        return self._scalar(lib().smx_dominant_wavenumber)
+
+    def dtpsi(self):
+        out = np.empty(self.shape)
+        check(lib().smx_dtpsi(self._h, _ptr(out), self.cfg.nz))
+        return out
+
+    def surftrack(self, arm, vel_sign=1.0):
+        nx, ny, _ = self.shape
+        outs = [np.zeros(nx * ny) for _ in range(4)]
+        found = C.c_int64()
+        opts = _abi.SmxSurfOpts(arm, vel_sign)
+        check(lib().smx_surftrack(self._h, C.byref(opts), *[_ptr(o) for o in outs], C.byref(found)))
+        return outs, found.value
+
+    def profile_step(self, nsteps):
+        ms = np.zeros(16)
+        by = np.zeros(16)
+        n = C.c_int()
+        names = C.create_string_buffer(1024)
+        check(lib().smx_profile_step(self._h, int(nsteps), 16, C.byref(n), _ptr(ms), _ptr(by), names, 1024))
+        k = n.value
+        return list(zip(names.value.decode().split("\n")[:k], ms[:k], by[:k]))
+
+    def calibration_copy(self, nbytes, reps=20):
+        v = C.c_double()
+        check(lib().smx_calibration_copy(self._h, int(nbytes), int(reps), C.byref(v)))
+        return v.value
