@@ -1,0 +1,174 @@
+"""Benchmark of the periodic Swift-Hohenberg step on one MI355X.
+
+Workload (pf3dMorph-style periodic run): 256^3 grid, L = 32*2*pi per axis
+(8 points per wavelength), q0 = 1, eps = 0.3, beta = 0, gamma = 1, dt = 0.1,
+cubic nonlinearity, CNAB2; psi0 = 0.4 cos(q0 sqrt(max(x^2+y^2-z^2, 0) + 4))
+about the box centre plus uniform noise of amplitude 1e-3 (SplitMix64 seed 0).
+Both the FP64 path and the FP32-mixed path are run from the same initial
+state; `value` is the FP32-mixed throughput, the FP64 figures and the error of
+the mixed path against the FP64 path are reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SH steps/s at 256³ (FP64 vs FP32-mixed, ≤1e-6 rel); adjoint grads/s; %HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_cfg(precision, n=256):
+    from smectic3d_amd import _abi as A
+    L = n / 8 * 2 * math.pi
+    return A.make_config(n, lx=L, ly=L, lz=L, eps=0.3, q0=1.0, beta=0.0, gamma=1.0, dt=0.1,
+                         nl_model=A.SMX_NL_CUBIC, precision=precision)
+
+
+def hyperboloid_ic(n, dx, q0=1.0, a=0.4, noise=1e-3, seed=0):
+    """psi0 = a cos(q0 sqrt(max(x^2+y^2-z^2,0)+4)) + U[-noise, noise] (SplitMix64, counter = linear index)."""
+    c = (np.arange(n) - n // 2) * dx
+    x = c[:, None, None]
+    y = c[None, :, None]
+    z = c[None, None, :]
+    r2 = np.maximum(x * x + y * y - z * z, 0.0)
+    psi = a * np.cos(q0 * np.sqrt(r2 + 4.0))
+    idx = np.arange(n * n * n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        zz = np.uint64(seed) + (idx + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        zz = (zz ^ (zz >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        zz = (zz ^ (zz >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        zz = zz ^ (zz >> np.uint64(31))
+    u = (zz >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return psi + noise * (2.0 * u.reshape(n, n, n) - 1.0)
+
+
+def run_path(precision, psi0, steps, warmup, prof_steps):
+    from smectic3d_amd import Solver
+    cfg = workload_cfg(precision)
+    s = Solver(cfg)
+    s.set_psi(psi0)
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    t1 = time.perf_counter()
+    out = {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps}
+    out["psi"] = s.get_psi()
+    out["energy"] = s.energy()
+    out["qdom"] = s.dominant_wavenumber()
+    out["l1"] = s.l1()
+    out["kernels"] = s.profile_step(prof_steps)  # advances the state; after the final readouts
+    return out
+
+
+def cpu_baseline(psi0, budget_s=12.0):
+    """The FP64 oracle (a C port of the reference loop, OpenMP threads) timed on the host,
+    on a bounded sample of the same workload: whole steps at 256^3 until ~budget_s."""
+    from oracle import oracle as O
+    from smectic3d_amd import _abi as A
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = O.Oracle(workload_cfg(A.SMX_FP64))
+    o.set_psi(psi0)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if n >= 2 and el > budget_s:
+            break
+    return {"value": n / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 steps of the 256^3 workload (oracle/smx_oracle.c, {threads} OpenMP threads, own FFT, not FFTW)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--prof-steps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
+              "(multi-GPU sharding is a design note, not an implemented path).", file=sys.stderr)
+        sys.exit(2)
+    from smectic3d_amd import _abi as A
+
+    n = 256
+    dx = (n / 8 * 2 * math.pi) / n
+    psi0 = hyperboloid_ic(n, dx)
+    log(f"workload 256^3, steps={a.steps} warmup={a.warmup}")
+    r32 = run_path(A.SMX_FP32_MIXED, psi0, a.steps, a.warmup, a.prof_steps)
+    log(f"fp32-mixed: {r32['steps_per_s']:.1f} steps/s")
+    r64 = run_path(A.SMX_FP64, psi0, a.steps, a.warmup, a.prof_steps)
+    log(f"fp64: {r64['steps_per_s']:.1f} steps/s")
+
+    err_psi = float(np.linalg.norm(r32["psi"] - r64["psi"]) / np.linalg.norm(r64["psi"]))
+    err_e = abs(r32["energy"] - r64["energy"]) / abs(r64["energy"])
+    err_q = abs(r32["qdom"] - r64["qdom"]) / abs(r64["qdom"])
+
+    # roofline of the dominant kernel (largest share of the step time) of the FP32-mixed path
+    ks = r32["kernels"]
+    name, ms, alg = max(ks, key=lambda k: k[1])
+    achieved = alg / (ms * 1e-3) / 1e9
+    nzh = n // 2 + 1
+    s_r, s_c = n ** 3 * 4, n * n * nzh * 8
+    step_alg = 2 * s_r + 5 * s_c  # compulsory bytes of the pure-SH step DAG (FP32)
+
+    line = {
+        "metric": METRIC,
+        "value": r32["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "periodic SH 256^3, hyperboloid IC + 1e-3 noise, L=32*2pi, q0=1, eps=0.3, beta=0, "
+                        "gamma=1, dt=0.1, cubic NL, CNAB2 (FP32-mixed headline; FP64 beside it)",
+            "grid": [n, n, n], "precision": "fp32-mixed",
+        },
+        "fp64": {"value": r64["steps_per_s"], "unit": "steps/s", "ms_per_step": r64["ms_per_step"]},
+        "speedup_fp32_mixed_vs_fp64": r32["steps_per_s"] / r64["steps_per_s"],
+        "error_fp32_mixed_vs_fp64": {"psi_rel_l2": err_psi, "energy_rel": err_e, "qdom_rel": err_q,
+                                     "steps": a.steps + a.warmup},
+        "step_roofline_fp32": {"alg_bytes_per_step": step_alg,
+                               "achieved_GBs": step_alg / (r32["ms_per_step"] * 1e-3) / 1e9,
+                               "frac": step_alg / (r32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "kernels_fp32": [{"name": k, "avg_ms": m, "alg_bytes": b, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks],
+        "kernels_fp64": [{"name": k, "avg_ms": m, "alg_bytes": b, "GBs": b / (m * 1e-3) / 1e9}
+                         for k, m, b in r64["kernels"]],
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(psi0)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

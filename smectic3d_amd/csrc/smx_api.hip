@@ -1,5 +1,5 @@
-// smx_api.hip -- host side of libsmectic: handle, device buffers, step chain,
-// FP64 diagnostics and the C ABI declared in include/smectic.h.
+// smx_api.hip -- host side of libsmectic: handle, device buffers, step chains,
+// FP64 diagnostics, surfTrack and the C ABI declared in include/smectic.h.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/smectic.h"
+#include "smx_diag.h"
 #include "smx_launch.h"
 
 using namespace smx;
@@ -30,160 +31,46 @@ static int fail(int code, const std::string& msg) {
     if (r_ != SMX_OK) return r_; \
   } while (0)
 
-// ---------------------------------------------------------------------------
-// diagnostics kernels (FP64)
-namespace {
-
-// Per-block FP64 Kahan partials of a k-space reduction over the normalised
-// half spectrum psi_hat, layout [kz][kx][ky]:
-//   out[3b+0] = sum w (alpha/2 (q0^2 - k^2)^2 - eps/2) |psi_hat|^2
-//   out[3b+1] = sum_{k != 0} w |psi_hat|^2
-//   out[3b+2] = sum_{k != 0} w |k| |psi_hat|^2
-// w = 1 on the kz = 0 and kz = nz/2 planes, 2 elsewhere (Hermitian half).
-struct Kahan {
-  double s = 0.0, c = 0.0;
-  __device__ __forceinline__ void add(double x) {
-    double y = x - c, t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-};
-
-__global__ void k_reduce_spec(const cplx<double>* __restrict__ a, int64_t n, int nx, int ny, int nz,
-                              const double* __restrict__ kx, const double* __restrict__ ky,
-                              const double* __restrict__ kz, double eps, double alpha, double q02,
-                              double* __restrict__ out) {
-  __shared__ double red[64];
-  Kahan s0, s1, s2;
-  const int64_t nxy = (int64_t)nx * ny;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int iz = (int)(i / nxy);
-    const int64_t r = i - iz * nxy;
-    const int ix = (int)(r / ny), iy = (int)(r - (int64_t)ix * ny);
-    const double k2 = kx[ix] * kx[ix] + ky[iy] * ky[iy] + kz[iz] * kz[iz];
-    const double w = (iz == 0 || 2 * iz == nz) ? 1.0 : 2.0;
-    const double p2 = w * (a[i].x * a[i].x + a[i].y * a[i].y);
-    const double d = q02 - k2;
-    s0.add((0.5 * alpha * d * d - 0.5 * eps) * p2);
-    if (i != 0) {
-      s1.add(p2);
-      s2.add(sqrt(k2) * p2);
-    }
-  }
-  double t0 = block_sum(s0.s, red);
-  double t1 = block_sum(s1.s, red);
-  double t2 = block_sum(s2.s, red);
-  if (threadIdx.x == 0) {
-    out[3 * blockIdx.x + 0] = t0;
-    out[3 * blockIdx.x + 1] = t1;
-    out[3 * blockIdx.x + 2] = t2;
-  }
-}
-
-// Per-block partial of sum f_local(psi) over a real field.
-__global__ void k_reduce_local(const double* __restrict__ psi, int64_t n, int nl_model, double beta, double gamma,
-                               double* __restrict__ out) {
-  __shared__ double red[64];
-  Kahan s;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double p = psi[i], p2 = p * p;
-    // energyPsi.m:225-228 (quintic) or the cubic model
-    const double f = (nl_model == SMX_NL_CUBIC) ? (-beta / 3.0 * p2 * p + gamma / 4.0 * p2 * p2)
-                                                : (-beta / 4.0 * p2 * p2 + gamma / 6.0 * p2 * p2 * p2);
-    s.add(f);
-  }
-  double t = block_sum(s.s, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = t;
-}
-
-// Per-block partials of sum |a - b| and sum |a| (pf3d.cpp:494-502).
-__global__ void k_reduce_l1(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
-                            double* __restrict__ out) {
-  __shared__ double red[64];
-  Kahan s0, s1;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    s0.add(fabs(a[i] - b[i]));
-    s1.add(fabs(a[i]));
-  }
-  double t0 = block_sum(s0.s, red);
-  double t1 = block_sum(s1.s, red);
-  if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = t0;
-    out[2 * blockIdx.x + 1] = t1;
-  }
-}
-
-// Non-finite check over a complex array: out[0] += count.
-template <typename T>
-__global__ void k_count_nonfinite(const cplx<T>* __restrict__ a, int64_t n, unsigned int* __restrict__ out) {
-  unsigned int c = 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    c += (!isfinite((double)a[i].x) || !isfinite((double)a[i].y)) ? 1u : 0u;
-  if (c) atomicAdd(out, c);
-}
-
-__global__ void k_f32_to_f64(const cplx<float>* __restrict__ a, cplx<double>* __restrict__ b, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = mk((double)a[i].x, (double)a[i].y);
-}
-
-// dpsi/dt in k-space: out = aLin(k) psi_hat + N_hat (both normalised), layout [kz][kx][ky]
-// (cosNoAdvMorphCurv.cpp:744, surfTrack.cpp:611).
-__global__ void k_dtpsi_combine(const cplx<double>* __restrict__ psi, cplx<double>* __restrict__ nq, int64_t n,
-                                int nx, int ny, const double* __restrict__ kx, const double* __restrict__ ky,
-                                const double* __restrict__ kz, double eps, double alpha, double q02) {
-  const int64_t nxy = (int64_t)nx * ny;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int iz = (int)(i / nxy);
-    const int64_t r = i - iz * nxy;
-    const int ix = (int)(r / ny), iy = (int)(r - (int64_t)ix * ny);
-    const double k2 = kx[ix] * kx[ix] + ky[iy] * ky[iy] + kz[iz] * kz[iz];
-    const double d = k2 - q02;
-    const double aLin = eps - alpha * d * d;
-    nq[i] = mk(aLin * psi[i].x + nq[i].x, aLin * psi[i].y + nq[i].y);
-  }
-}
-
-__global__ void k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
-}
-
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // handle
 template <typename T>
 struct DevTables {
-  T* kx = nullptr;
+  T* kx = nullptr;  // wavenumbers (periodic: FFT order, Nyquist +pi/dx; Neumann: pi k / L)
   T* ky = nullptr;
   T* kz = nullptr;
-  cplx<T>* twx = nullptr;
+  cplx<T>* twx = nullptr;  // e^{-2 pi i m/n}
   cplx<T>* twy = nullptr;
   cplx<T>* twz = nullptr;
+  cplx<T>* qtwx = nullptr;  // e^{-i pi k/2n} (Neumann)
+  cplx<T>* qtwy = nullptr;
+  cplx<T>* qtwz = nullptr;
 };
 
 struct smx_handle {
   smx_config cfg;
   hipStream_t st = nullptr;
-  bool fp32 = false;
-  int nzh = 0;
-  int64_t N = 0, Nh = 0, nxy = 0, npencils = 0;
+  bool fp32 = false, neumann = false;
+  int nplanes = 0;           // spectral planes along z: nz/2+1 (periodic) or nz (Neumann)
+  int64_t N = 0, nspec = 0;  // real points; spectral entries (complex periodic, real Neumann)
+  int64_t nxy = 0, npencils = 0;
+  size_t wbytes = 0;  // bytes of one spectral entry in the working precision
   double dx = 0, dy = 0, dz = 0;
-  void* A = nullptr;  // stage buffer, working precision
-  void* R = nullptr;  // CNAB2 carry R = C1 psi_hat - dt/2 N_hat_past
-  void* P = nullptr;  // bootstrap spectrum
-  cplx<double>* D1 = nullptr;  // FP64 diagnostics
-  cplx<double>* D2 = nullptr;
+  void* A = nullptr;   // stage buffer, working precision
+  void* R = nullptr;   // CNAB2 carry R = C1 psi_hat - dt/2 N_hat_past
+  void* P = nullptr;   // bootstrap spectrum
+  void* D1 = nullptr;  // FP64 diagnostic spectra
+  void* D2 = nullptr;
   double* real_io = nullptr;
   double* real_prev = nullptr;
   double* partial = nullptr;  // z-pass FP64 partial sums (mixed path mean mode)
   double* mean_state = nullptr;
-  double* red = nullptr;      // reduction partials
+  double* red = nullptr;  // reduction partials
   unsigned int* flag = nullptr;
+  double* surf = nullptr;    // 4 x nxy persistent surfTrack outputs (zero-initialised)
+  double* sfield = nullptr;  // surfTrack real fields, 8 x N (allocated on first use)
   DevTables<float> t32;
   DevTables<double> t64;
   std::vector<void*> allocs;
@@ -204,13 +91,19 @@ static std::vector<double> wavenumbers_periodic(int n, double d) {
   for (int i = 0; i < n; ++i) v[i] = (i < n / 2 + 1) ? 2.0 * M_PI * i / (d * n) : -(double)(n - i) * 2.0 * M_PI / (d * n);
   return v;
 }
+// DCT wavenumbers pi k / L (cosNoAdvMorph.cpp:216-231).
+static std::vector<double> wavenumbers_dct(int n, double d) {
+  std::vector<double> v(n);
+  for (int i = 0; i < n; ++i) v[i] = M_PI * i / (d * n);
+  return v;
+}
 
-// e^{-2 pi i m / n}, m < n, evaluated in extended precision and rounded once.
-static std::vector<cplx<double>> twiddles(int n) {
+// e^{-2 pi i m / den}, m < n, evaluated in extended precision and rounded once.
+static std::vector<cplx<double>> roots(int n, int den) {
   std::vector<cplx<double>> t(n);
   const long double pi = 3.141592653589793238462643383279502884L;
   for (int m = 0; m < n; ++m) {
-    long double a = -2.0L * pi * (long double)m / (long double)n;
+    long double a = -2.0L * pi * (long double)m / (long double)den;
     t[m].x = (double)cosl(a);
     t[m].y = (double)sinl(a);
   }
@@ -226,8 +119,8 @@ static int upload_tables(smx_handle* h, DevTables<T>& t) {
     HIPCHK(hipMemcpy(*dst, tmp.data(), sizeof(T) * tmp.size(), hipMemcpyHostToDevice));
     return SMX_OK;
   };
-  auto put_tw = [&](cplx<T>** dst, int n) -> int {
-    auto tw = twiddles(n);
+  auto put_tw = [&](cplx<T>** dst, int n, int den) -> int {
+    auto tw = roots(n, den);
     std::vector<cplx<T>> tmp(n);
     for (int i = 0; i < n; ++i) {
       tmp[i].x = (T)tw[i].x;
@@ -237,25 +130,37 @@ static int upload_tables(smx_handle* h, DevTables<T>& t) {
     HIPCHK(hipMemcpy(*dst, tmp.data(), sizeof(cplx<T>) * n, hipMemcpyHostToDevice));
     return SMX_OK;
   };
-  CHK(put_real(&t.kx, wavenumbers_periodic(c.nx, h->dx)));
-  CHK(put_real(&t.ky, wavenumbers_periodic(c.ny, h->dy)));
-  CHK(put_real(&t.kz, wavenumbers_periodic(c.nz, h->dz)));
-  CHK(put_tw(&t.twx, c.nx));
-  CHK(put_tw(&t.twy, c.ny));
-  CHK(put_tw(&t.twz, c.nz));
+  auto wn = h->neumann ? wavenumbers_dct : wavenumbers_periodic;
+  CHK(put_real(&t.kx, wn(c.nx, h->dx)));
+  CHK(put_real(&t.ky, wn(c.ny, h->dy)));
+  CHK(put_real(&t.kz, wn(c.nz, h->dz)));
+  CHK(put_tw(&t.twx, c.nx, c.nx));
+  CHK(put_tw(&t.twy, c.ny, c.ny));
+  CHK(put_tw(&t.twz, c.nz, c.nz));
+  if (h->neumann) {
+    CHK(put_tw(&t.qtwx, c.nx, 4 * c.nx));
+    CHK(put_tw(&t.qtwy, c.ny, 4 * c.ny));
+    CHK(put_tw(&t.qtwz, c.nz, 4 * c.nz));
+  }
   return SMX_OK;
 }
 
+template <typename T> static DevTables<T>& tables(smx_handle* h);
+template <> DevTables<float>& tables<float>(smx_handle* h) { return h->t32; }
+template <> DevTables<double>& tables<double>(smx_handle* h) { return h->t64; }
+
 template <typename T>
-static PParams<T> params(const smx_handle* h, const DevTables<T>& t, bool mixed_mean) {
+static PParams<T> params(smx_handle* h, bool mixed_mean) {
+  const DevTables<T>& t = tables<T>(h);
   const smx_config& c = h->cfg;
   PParams<T> p;
   p.nx = c.nx;
   p.ny = c.ny;
   p.nz = c.nz;
-  p.nzh = h->nzh;
+  p.nzh = h->nplanes;
   p.nxy = h->nxy;
-  p.scale = (T)(1.0 / (double)h->N);
+  // unnormalised transforms: C2R o R2C = N (pf3d.cpp:62); REDFT01 o REDFT10 = 8N (cosNoAdvMorph.cpp:112)
+  p.scale = (T)(h->neumann ? 0.125 / (double)h->N : 1.0 / (double)h->N);
   p.eps = (T)c.eps;
   p.alpha = (T)c.alpha;
   p.q02 = (T)(c.q0 * c.q0);
@@ -272,103 +177,179 @@ static PParams<T> params(const smx_handle* h, const DevTables<T>& t, bool mixed_
   p.twy = t.twy;
   p.twz = t.twz;
   p.partial = mixed_mean ? h->partial : nullptr;
-  p.npartial = PeriodicOps<T>::zpass_blocks(p);
+  p.npartial = (int)((h->nxy + kNC - 1) / kNC);
   p.mean_state = h->mean_state;
   const double q02 = c.q0 * c.q0, aLin0 = c.eps - c.alpha * q02 * q02;
-  p.m_scale = 1.0 / (double)h->N;
+  p.m_scale = 1.0 / (double)h->N;  // k = 0 coefficient of either transform, normalised
   p.m_dtd2 = 0.5 * c.dt;
   p.m_c1 = 1.0 + p.m_dtd2 * aLin0;
   p.m_c2 = 1.0 - p.m_dtd2 * aLin0;
   return p;
 }
 
-#define HIPERR(x)                                                                                  \
-  do {                                                                                             \
-    hipError_t e_ = (x);                                                                           \
-    if (e_ != hipSuccess) return fail(SMX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-// ---- working-precision chains ----
+// ---- pass facade over both boundary conditions ----
 template <typename T>
-static DevTables<T>& tables(smx_handle* h);
-template <>
-DevTables<float>& tables<float>(smx_handle* h) { return h->t32; }
-template <>
-DevTables<double>& tables<double>(smx_handle* h) { return h->t64; }
+static int ypass(smx_handle* h, const void* in, void* out, int dir) {
+  const DevTables<T>& t = tables<T>(h);
+  hipError_t e = h->neumann
+                     ? NeumannOps<T>::ypass((const T*)in, (T*)out, h->cfg.ny, h->npencils, dir, t.twy, t.qtwy, h->st)
+                     : PeriodicOps<T>::ypass((const cplx<T>*)in, (cplx<T>*)out, h->cfg.ny, h->npencils, dir, t.twy,
+                                             h->st);
+  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("y pass: ") + hipGetErrorString(e));
+  return SMX_OK;
+}
+
+template <typename T>
+static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, void* out, void* r, const void* psi0) {
+  hipError_t e = h->neumann ? NeumannOps<T>::xpass(mode, (const T*)in, (T*)out, (T*)r, (const T*)psi0, p,
+                                                   tables<T>(h).qtwx, h->st)
+                            : PeriodicOps<T>::xpass(mode, (const cplx<T>*)in, (cplx<T>*)out, (cplx<T>*)r,
+                                                    (const cplx<T>*)psi0, p, h->st);
+  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("x pass: ") + hipGetErrorString(e));
+  return SMX_OK;
+}
+
+template <typename T>
+static int zpass(smx_handle* h, const PParams<T>& p, int in_mode, int out_mode, int op, const void* in, void* out,
+                 const double* real_in, double* real_out) {
+  hipError_t e = h->neumann ? NeumannOps<T>::zpass(in_mode, out_mode, op, (const T*)in, (T*)out, real_in, real_out,
+                                                   p, tables<T>(h).qtwz, h->st)
+                            : PeriodicOps<T>::zpass(in_mode, out_mode, op, (const cplx<T>*)in, (cplx<T>*)out,
+                                                    real_in, real_out, p, h->st);
+  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("z pass: ") + hipGetErrorString(e));
+  return SMX_OK;
+}
+
+template <typename T>
+static int herm(smx_handle* h, const PParams<T>& p, void* a) {
+  if (h->neumann) return SMX_OK;  // real-to-real transforms carry no redundant modes
+  hipError_t e = PeriodicOps<T>::herm_project((cplx<T>*)a, p, h->st);
+  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("herm: ") + hipGetErrorString(e));
+  return SMX_OK;
+}
 
 // Bootstrap from real psi in h->real_io: P = psi_hat (normalised), R = C1 psi_hat - dt/2 N_hat(psi),
-// A = x-inverse(psi_hat).  pf3d.cpp:363-374, pf3dMorph.cpp:451-472.
+// A = x-inverse(psi_hat).  pf3d.cpp:363-374, pf3dMorph.cpp:451-472, cosNoAdvMorph.cpp:423-442.
 template <typename T>
 static int bootstrap(smx_handle* h) {
-  PParams<T> p = params<T>(h, tables<T>(h), h->fp32);
-  cplx<T>* A = (cplx<T>*)h->A;
-  cplx<T>* R = (cplx<T>*)h->R;
-  cplx<T>* P = (cplx<T>*)h->P;
-  using Ops = PeriodicOps<T>;
-  HIPERR(Ops::zpass(Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, A, h->real_io, nullptr, p, h->st));
-  HIPERR(Ops::ypass(A, A, p.ny, h->npencils, -1, p.twy, h->st));
-  HIPERR(Ops::xpass(X_FWD_SCALED, A, P, nullptr, nullptr, p, h->st));
-  HIPERR(Ops::zpass(Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, A, h->real_io, nullptr, p, h->st));
-  HIPERR(Ops::ypass(A, A, p.ny, h->npencils, -1, p.twy, h->st));
-  HIPERR(Ops::xpass(X_BOOT, A, A, R, P, p, h->st));
+  PParams<T> p = params<T>(h, h->fp32);
+  CHK(zpass<T>(h, p, Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, h->A, h->real_io, nullptr));
+  CHK(ypass<T>(h, h->A, h->A, -1));
+  CHK(xpass<T>(h, p, X_FWD_SCALED, h->A, h->P, nullptr, nullptr));
+  CHK(zpass<T>(h, p, Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, h->A, h->real_io, nullptr));
+  CHK(ypass<T>(h, h->A, h->A, -1));
+  CHK(xpass<T>(h, p, X_BOOT, h->A, h->A, h->R, h->P));
+  CHK(herm<T>(h, p, h->R));
   return SMX_OK;
 }
 
 // Kernel slots of one step, in launch order.
-static const char* kSlotNames[] = {"k_ypass_inv", "k_zpass_nl", "k_ypass_fwd", "k_xpass_step"};
-constexpr int kNumSlots = 4;
+static const char* kSlotNamesP[] = {"k_ypass_inv", "k_zpass_nl", "k_ypass_fwd", "k_xpass_step", "k_herm_project"};
+static const char* kSlotNamesN[] = {"k_dct_ypass_inv", "k_dct_zpass_nl", "k_dct_ypass_fwd", "k_dct_xpass_step"};
+static int num_slots(const smx_handle* h) { return h->neumann ? 4 : 5; }
 
 template <typename T>
 static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
-  cplx<T>* A = (cplx<T>*)h->A;
-  cplx<T>* R = (cplx<T>*)h->R;
-  using Ops = PeriodicOps<T>;
-  if (ev) HIPERR(hipEventRecord(ev[0], h->st));
-  HIPERR(Ops::ypass(A, A, p.ny, h->npencils, +1, p.twy, h->st));
-  if (ev) HIPERR(hipEventRecord(ev[1], h->st));
-  HIPERR(Ops::zpass(Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, A, A, nullptr, nullptr, p, h->st));
-  if (ev) HIPERR(hipEventRecord(ev[2], h->st));
-  HIPERR(Ops::ypass(A, A, p.ny, h->npencils, -1, p.twy, h->st));
-  if (ev) HIPERR(hipEventRecord(ev[3], h->st));
-  HIPERR(Ops::xpass(X_STEP, A, A, R, nullptr, p, h->st));
-  if (ev) HIPERR(hipEventRecord(ev[4], h->st));
+  if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
+  CHK(ypass<T>(h, h->A, h->A, +1));
+  if (ev) HIPCHK(hipEventRecord(ev[1], h->st));
+  CHK(zpass<T>(h, p, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr));
+  if (ev) HIPCHK(hipEventRecord(ev[2], h->st));
+  CHK(ypass<T>(h, h->A, h->A, -1));
+  if (ev) HIPCHK(hipEventRecord(ev[3], h->st));
+  CHK(xpass<T>(h, p, X_STEP, h->A, h->A, h->R, nullptr));
+  if (ev) HIPCHK(hipEventRecord(ev[4], h->st));
+  if (!h->neumann) {
+    CHK(herm<T>(h, p, h->R));
+    if (ev) HIPCHK(hipEventRecord(ev[5], h->st));
+  }
   return SMX_OK;
 }
 
 // ---- FP64 diagnostics on a copy of the state ----
 static int state_to_d1(smx_handle* h) {
+  const int64_t ncomp = h->neumann ? h->nspec : 2 * h->nspec;  // real components
   if (h->fp32) {
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const cplx<float>*)h->A, h->D1,
-                       h->Nh);
-    HIPERR(hipGetLastError());
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)h->A,
+                       (double*)h->D1, ncomp);
+    HIPCHK(hipGetLastError());
   } else {
-    HIPERR(hipMemcpyAsync(h->D1, h->A, sizeof(cplx<double>) * h->Nh, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->D1, h->A, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
   }
   return SMX_OK;
 }
 
 // Real psi of the current state into `out` (dense [x][y][z], FP64).
 static int c2r_state(smx_handle* h, double* out) {
-  PParams<double> p = params<double>(h, h->t64, false);
+  PParams<double> p = params<double>(h, false);
   CHK(state_to_d1(h));
-  HIPERR(PeriodicOps<double>::ypass(h->D1, h->D1, p.ny, h->npencils, +1, p.twy, h->st));
-  HIPERR(PeriodicOps<double>::zpass(Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out, p, h->st));
+  CHK(ypass<double>(h, h->D1, h->D1, +1));
+  CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out));
   return SMX_OK;
 }
 
-// Normalised full half spectrum psi_hat of the current state into D2.
+// Inverse chain of a normalised full spectrum in D1 (destroyed) into a real field.
+static int c2r_spectrum_d1(smx_handle* h, double* out) {
+  PParams<double> p = params<double>(h, false);
+  CHK(xpass<double>(h, p, X_INV, h->D1, h->D1, nullptr, nullptr));
+  CHK(ypass<double>(h, h->D1, h->D1, +1));
+  CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out));
+  return SMX_OK;
+}
+
+// Normalised full spectrum psi_hat of the current state into D2.
 static int spectrum_state(smx_handle* h) {
-  PParams<double> p = params<double>(h, h->t64, false);
-  p.scale = 1.0 / (double)h->cfg.nx;  // the stage buffer holds the x-inverse of psi_hat
+  PParams<double> p = params<double>(h, false);
+  // the stage buffer holds the x-inverse of psi_hat: undo it (C2R o R2C = nx, REDFT01 o REDFT10 = 2 nx)
+  p.scale = h->neumann ? 0.5 / (double)h->cfg.nx : 1.0 / (double)h->cfg.nx;
   CHK(state_to_d1(h));
-  HIPERR(PeriodicOps<double>::xpass(X_FWD_SCALED, h->D1, h->D2, nullptr, nullptr, p, h->st));
+  CHK(xpass<double>(h, p, X_FWD_SCALED, h->D1, h->D2, nullptr, nullptr));
+  return SMX_OK;
+}
+
+// Normalised N_hat of the real field in h->real_io into D1.
+static int nl_spectrum_of_real_io(smx_handle* h) {
+  PParams<double> p = params<double>(h, false);
+  CHK(zpass<double>(h, p, Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, h->D1, h->real_io, nullptr));
+  CHK(ypass<double>(h, h->D1, h->D1, -1));
+  CHK(xpass<double>(h, p, X_FWD_SCALED, h->D1, h->D1, nullptr, nullptr));
+  return SMX_OK;
+}
+
+static KParams kparams(smx_handle* h) {
+  const smx_config& c = h->cfg;
+  KParams k;
+  k.nx = c.nx;
+  k.ny = c.ny;
+  k.nz = c.nz;
+  k.nplanes = h->nplanes;
+  k.n = h->nspec;
+  k.neumann = h->neumann ? 1 : 0;
+  k.kx = h->t64.kx;
+  k.ky = h->t64.ky;
+  k.kz = h->t64.kz;
+  k.eps = c.eps;
+  k.alpha = c.alpha;
+  k.q02 = c.q0 * c.q0;
+  return k;
+}
+
+static int kmul(smx_handle* h, int mode, const void* in, void* out) {
+  KParams k = kparams(h);
+  if (h->neumann)
+    hipLaunchKernelGGL(k_kmul_real, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const double*)in, (double*)out, k,
+                       mode);
+  else
+    hipLaunchKernelGGL(k_kmul_cplx, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const cplx<double>*)in,
+                       (cplx<double>*)out, k, mode);
+  HIPCHK(hipGetLastError());
   return SMX_OK;
 }
 
 static int host_sum(smx_handle* h, int nper, int nblocks, std::vector<double>& out) {
   std::vector<double> tmp((size_t)nper * nblocks);
-  HIPERR(hipMemcpyAsync(tmp.data(), h->red, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, h->st));
-  HIPERR(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpyAsync(tmp.data(), h->red, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   out.assign(nper, 0.0);
   std::vector<double> c(nper, 0.0);
   for (int b = 0; b < nblocks; ++b)
@@ -381,28 +362,49 @@ static int host_sum(smx_handle* h, int nper, int nblocks, std::vector<double>& o
 }
 
 static int check_finite(smx_handle* h) {
-  HIPERR(hipMemsetAsync(h->flag, 0, sizeof(unsigned int), h->st));
+  HIPCHK(hipMemsetAsync(h->flag, 0, sizeof(unsigned int), h->st));
+  const int64_t ncomp = h->neumann ? h->nspec : 2 * h->nspec;
   if (h->fp32)
-    hipLaunchKernelGGL(k_count_nonfinite<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st,
-                       (const cplx<float>*)h->R, h->Nh, h->flag);
+    hipLaunchKernelGGL(k_count_nonfinite<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)h->R,
+                       ncomp, h->flag);
   else
-    hipLaunchKernelGGL(k_count_nonfinite<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st,
-                       (const cplx<double>*)h->R, h->Nh, h->flag);
-  HIPERR(hipGetLastError());
+    hipLaunchKernelGGL(k_count_nonfinite<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const double*)h->R,
+                       ncomp, h->flag);
+  HIPCHK(hipGetLastError());
   unsigned int bad = 0;
-  HIPERR(hipMemcpyAsync(&bad, h->flag, sizeof(bad), hipMemcpyDeviceToHost, h->st));
-  HIPERR(hipStreamSynchronize(h->st));
-  if (bad) return fail(SMX_ERR_NONFINITE, std::to_string(bad) + " non-finite spectral entries in the state");
+  HIPCHK(hipMemcpyAsync(&bad, h->flag, sizeof(bad), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (bad) return fail(SMX_ERR_NONFINITE, std::to_string(bad) + " non-finite values in the spectral state");
   return SMX_OK;
+}
+
+static int spectral_sums(smx_handle* h, std::vector<double>& s) {
+  CHK(spectrum_state(h));
+  KParams k = kparams(h);
+  if (h->neumann)
+    hipLaunchKernelGGL(k_reduce_spec_real, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const double*)h->D2, k,
+                       h->red);
+  else
+    hipLaunchKernelGGL(k_reduce_spec_cplx, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st,
+                       (const cplx<double>*)h->D2, k, h->red);
+  HIPCHK(hipGetLastError());
+  return host_sum(h, 3, kRedBlocks, s);
+}
+
+// dpsi/dt of the current state into `out` (FP64 real, dense [x][y][z]); uses real_io as scratch.
+static int dtpsi_field(smx_handle* h, double* out) {
+  CHK(spectrum_state(h));  // D2 = psi_hat
+  CHK(c2r_state(h, h->real_io));
+  CHK(nl_spectrum_of_real_io(h));        // D1 = N_hat
+  CHK(kmul(h, KM_DTPSI, h->D2, h->D1));  // D1 = aLin psi_hat + N_hat
+  return c2r_spectrum_d1(h, out);
 }
 
 template <typename T>
 static int run_steps(smx_handle* h, int64_t nsteps) {
-  PParams<T> p = params<T>(h, tables<T>(h), h->fp32);
+  PParams<T> p = params<T>(h, h->fp32);
   for (int64_t n = 0; n < nsteps; ++n) {
-    if (n == nsteps - 1) {
-      CHK(c2r_state(h, h->real_prev));  // psi^n for the L1 criterion of the last step
-    }
+    if (n == nsteps - 1) CHK(c2r_state(h, h->real_prev));  // psi^n for the L1 criterion of the last step
     CHK(step_once<T>(h, p, nullptr));
   }
   return SMX_OK;
@@ -410,16 +412,20 @@ static int run_steps(smx_handle* h, int64_t nsteps) {
 
 template <typename T>
 static int profile_T(smx_handle* h, int64_t nsteps, std::vector<double>& acc) {
-  PParams<T> p = params<T>(h, tables<T>(h), h->fp32);
-  std::vector<hipEvent_t> ev(kNumSlots + 1);
-  for (auto& e : ev) HIPERR(hipEventCreate(&e));
-  acc.assign(kNumSlots, 0.0);
+  PParams<T> p = params<T>(h, h->fp32);
+  const int ns = num_slots(h);
+  std::vector<hipEvent_t> ev(ns + 1, nullptr);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  acc.assign(ns, 0.0);
   int rc = SMX_OK;
   for (int64_t n = 0; n < nsteps && rc == SMX_OK; ++n) {
     rc = step_once<T>(h, p, ev.data());
     if (rc) break;
-    if (hipEventSynchronize(ev[kNumSlots]) != hipSuccess) { rc = fail(SMX_ERR_HIP, "event sync"); break; }
-    for (int s = 0; s < kNumSlots; ++s) {
+    if (hipEventSynchronize(ev[ns]) != hipSuccess) {
+      rc = fail(SMX_ERR_HIP, "event sync");
+      break;
+    }
+    for (int s = 0; s < ns; ++s) {
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, ev[s], ev[s + 1]);
       acc[s] += ms;
@@ -470,24 +476,29 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
   if (c.precision != SMX_FP64 && c.precision != SMX_FP32_MIXED) return fail(SMX_ERR_INVALID, "bad precision");
   if (c.nl_model != SMX_NL_QUINTIC && c.nl_model != SMX_NL_CUBIC) return fail(SMX_ERR_INVALID, "bad nl_model");
   if (!(c.lx > 0 && c.ly > 0 && c.lz > 0 && c.dt > 0)) return fail(SMX_ERR_INVALID, "lengths and dt must be > 0");
-  if (c.bc == SMX_BC_NEUMANN) return fail(SMX_ERR_UNSUPPORTED, "Neumann (DCT) path not built yet");
+  if (c.advect && c.bc == SMX_BC_NEUMANN)
+    return fail(SMX_ERR_UNSUPPORTED, "advection is periodic only (the reference's Neumann drivers do not advect)");
   if (c.advect) return fail(SMX_ERR_UNSUPPORTED, "advection not built yet");
   for (int n : {c.nx, c.ny, c.nz})
     if (!length_supported(n))
       return fail(SMX_ERR_UNSUPPORTED, "grid length " + std::to_string(n) +
                                            " has no radix plan (supported: 16 24 32 48 64 96 128 192 256 384 512)");
   if (c.ny % kNC) return fail(SMX_ERR_UNSUPPORTED, "ny must be a multiple of 16");
+  if (c.w_mid != 0.0 && c.bc == SMX_BC_NEUMANN)
+    return fail(SMX_ERR_UNSUPPORTED, "the mid-plane penalty belongs to the periodic pf3dMorph driver");
   if (c.w_mid != 0.0 && (c.mid_plane < 1 || c.mid_plane > c.nz - 2))
     return fail(SMX_ERR_INVALID, "mid_plane must lie in [1, nz-2] when w_mid != 0");
 
   smx_handle* h = new smx_handle();
   h->cfg = c;
   h->fp32 = (c.precision == SMX_FP32_MIXED);
-  h->nzh = c.nz / 2 + 1;
+  h->neumann = (c.bc == SMX_BC_NEUMANN);
+  h->nplanes = h->neumann ? c.nz : c.nz / 2 + 1;
   h->nxy = (int64_t)c.nx * c.ny;
   h->N = h->nxy * c.nz;
-  h->Nh = h->nxy * h->nzh;
-  h->npencils = (int64_t)h->nzh * c.nx;
+  h->nspec = h->nxy * h->nplanes;
+  h->npencils = (int64_t)h->nplanes * c.nx;
+  h->wbytes = (h->fp32 ? 4 : 8) * (h->neumann ? 1 : 2);
   h->dx = c.lx / c.nx;
   h->dy = c.ly / c.ny;
   h->dz = c.lz / c.nz;
@@ -499,22 +510,23 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
   if (e != hipSuccess) return bail(fail(SMX_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
   e = hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking);
   if (e != hipSuccess) return bail(fail(SMX_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
-  const size_t wsz = h->fp32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const size_t dbytes = (h->neumann ? 8 : 16) * (size_t)h->nspec;
   int rc = SMX_OK;
-  if ((rc = dalloc(h, &h->A, wsz * h->Nh)) || (rc = dalloc(h, &h->R, wsz * h->Nh)) ||
-      (rc = dalloc(h, &h->P, wsz * h->Nh)) || (rc = dalloc(h, (void**)&h->D1, sizeof(cplx<double>) * h->Nh)) ||
-      (rc = dalloc(h, (void**)&h->D2, sizeof(cplx<double>) * h->Nh)) ||
-      (rc = dalloc(h, (void**)&h->real_io, sizeof(double) * h->N)) ||
+  if ((rc = dalloc(h, &h->A, h->wbytes * h->nspec)) || (rc = dalloc(h, &h->R, h->wbytes * h->nspec)) ||
+      (rc = dalloc(h, &h->P, h->wbytes * h->nspec)) || (rc = dalloc(h, &h->D1, dbytes)) ||
+      (rc = dalloc(h, &h->D2, dbytes)) || (rc = dalloc(h, (void**)&h->real_io, sizeof(double) * h->N)) ||
       (rc = dalloc(h, (void**)&h->real_prev, sizeof(double) * h->N)) ||
-      (rc = dalloc(h, (void**)&h->partial, sizeof(double) * (h->nxy / 8 + 64))) ||
+      (rc = dalloc(h, (void**)&h->partial, sizeof(double) * (h->nxy / kNC + 64))) ||
       (rc = dalloc(h, (void**)&h->mean_state, sizeof(double) * 8)) ||
       (rc = dalloc(h, (void**)&h->red, sizeof(double) * 4 * kRedBlocks)) ||
-      (rc = dalloc(h, (void**)&h->flag, sizeof(unsigned int) * 4)))
+      (rc = dalloc(h, (void**)&h->flag, sizeof(unsigned int) * 4)) ||
+      (rc = dalloc(h, (void**)&h->surf, sizeof(double) * 4 * h->nxy)))
     return bail(rc);
   if ((rc = upload_tables<double>(h, h->t64))) return bail(rc);
   if (h->fp32 && (rc = upload_tables<float>(h, h->t32))) return bail(rc);
-  e = hipMemset(h->mean_state, 0, sizeof(double) * 8);
-  if (e != hipSuccess) return bail(fail(SMX_ERR_HIP, "hipMemset"));
+  if ((e = hipMemset(h->mean_state, 0, sizeof(double) * 8)) != hipSuccess ||
+      (e = hipMemset(h->surf, 0, sizeof(double) * 4 * h->nxy)) != hipSuccess)
+    return bail(fail(SMX_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e)));
   *out = h;
   return SMX_OK;
 }
@@ -538,8 +550,9 @@ int smx_set_psi(smx_handle* h, const double* psi, int64_t row_stride) {
   const smx_config& c = h->cfg;
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
-  HIPCHK(hipMemcpy2DAsync(h->real_io, sizeof(double) * c.nz, psi, sizeof(double) * row_stride,
-                          sizeof(double) * c.nz, (size_t)h->nxy, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpy2DAsync(h->real_io, sizeof(double) * c.nz, psi, sizeof(double) * row_stride, sizeof(double) * c.nz,
+                          (size_t)h->nxy, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemsetAsync(h->mean_state, 0, sizeof(double) * 8, h->st));
   CHK(h->fp32 ? bootstrap<float>(h) : bootstrap<double>(h));
   HIPCHK(hipStreamSynchronize(h->st));
   h->has_state = true;
@@ -555,8 +568,8 @@ int smx_get_psi(smx_handle* h, double* psi, int64_t row_stride) {
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
   CHK(c2r_state(h, h->real_io));
-  HIPCHK(hipMemcpy2DAsync(psi, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz,
-                          sizeof(double) * c.nz, (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpy2DAsync(psi, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz, sizeof(double) * c.nz,
+                          (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   return SMX_OK;
 }
@@ -600,15 +613,6 @@ int smx_l1(smx_handle* h, double* l1) {
   return SMX_OK;
 }
 
-static int spectral_sums(smx_handle* h, std::vector<double>& s) {
-  CHK(spectrum_state(h));
-  const smx_config& c = h->cfg;
-  hipLaunchKernelGGL(k_reduce_spec, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->D2, h->Nh, c.nx, c.ny, c.nz,
-                     h->t64.kx, h->t64.ky, h->t64.kz, c.eps, c.alpha, c.q0 * c.q0, h->red);
-  HIPCHK(hipGetLastError());
-  return host_sum(h, 3, kRedBlocks, s);
-}
-
 int smx_energy(smx_handle* h, double* F) {
   if (!h || !F) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
@@ -622,7 +626,8 @@ int smx_energy(smx_handle* h, double* F) {
   HIPCHK(hipGetLastError());
   CHK(host_sum(h, 1, kRedBlocks, l));
   const double V = c.lx * c.ly * c.lz, dV = h->dx * h->dy * h->dz;
-  *F = V * s[0] + dV * l[0];  // Parseval: int psi^2 = V sum_k |psi_hat_k|^2
+  // Parseval: int psi^2 = V sum_k w |psi_hat_k|^2 (periodic), 8 V sum_k w psi_hat_k^2 (cosine basis)
+  *F = (h->neumann ? 8.0 * V : V) * s[0] + dV * l[0];
   return SMX_OK;
 }
 
@@ -642,19 +647,7 @@ int smx_dtpsi(smx_handle* h, double* out, int64_t row_stride) {
   const smx_config& c = h->cfg;
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
-  PParams<double> p = params<double>(h, h->t64, false);
-  using Ops = PeriodicOps<double>;
-  CHK(spectrum_state(h));                     // D2 = psi_hat
-  CHK(c2r_state(h, h->real_io));              // real psi
-  HIPCHK(Ops::zpass(Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, h->D1, h->real_io, nullptr, p, h->st));
-  HIPCHK(Ops::ypass(h->D1, h->D1, p.ny, h->npencils, -1, p.twy, h->st));
-  HIPCHK(Ops::xpass(X_FWD_SCALED, h->D1, h->D1, nullptr, nullptr, p, h->st));  // N_hat (normalised)
-  hipLaunchKernelGGL(k_dtpsi_combine, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->D2, h->D1, h->Nh, c.nx, c.ny,
-                     h->t64.kx, h->t64.ky, h->t64.kz, c.eps, c.alpha, c.q0 * c.q0);
-  HIPCHK(hipGetLastError());
-  HIPCHK(Ops::xpass(X_INV, h->D1, h->D1, nullptr, nullptr, p, h->st));
-  HIPCHK(Ops::ypass(h->D1, h->D1, p.ny, h->npencils, +1, p.twy, h->st));
-  HIPCHK(Ops::zpass(Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, h->real_io, p, h->st));
+  CHK(dtpsi_field(h, h->real_io));
   HIPCHK(hipMemcpy2DAsync(out, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz, sizeof(double) * c.nz,
                           (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -663,27 +656,80 @@ int smx_dtpsi(smx_handle* h, double* out, int64_t row_stride) {
 
 int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, double* vel, double* H, double* K,
                   int64_t* found) {
-  (void)opts; (void)surfz; (void)vel; (void)H; (void)K; (void)found;
-  if (!h) return fail(SMX_ERR_INVALID, "null handle");
-  return fail(SMX_ERR_UNSUPPORTED, "surfTrack not built yet");
+  if (!h || !opts || !surfz || !vel || !H || !K) return fail(SMX_ERR_INVALID, "null argument");
+  if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  HIPCHK(hipSetDevice(h->cfg.device));
+  const int64_t N = h->N;
+  if (!h->sfield) CHK(dalloc(h, (void**)&h->sfield, sizeof(double) * 8 * N));
+  double* f = h->sfield;
+  // f[0] psi, f[1] dtpsi, then per boundary condition:
+  //  periodic: f[2..4] spectral gradient, f[5] psi_xx + psi_yy, f[6] psi_xx, f[7] psi_yy, psi_xy in real_io
+  //            (surfTrack.cpp:611-626)
+  //  Neumann:  f[2] psi_xx, f[3] psi_yy, f[4] psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750)
+  CHK(dtpsi_field(h, f + N));
+  CHK(c2r_state(h, f));
+  CHK(spectrum_state(h));
+  if (!h->neumann) {
+    const int modes[] = {KM_IKX, KM_IKY, KM_IKZ, KM_LAP_XY, KM_DXX, KM_DYY, KM_DXY};
+    double* dst[] = {f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N, f + 6 * N, f + 7 * N, h->real_io};
+    for (int i = 0; i < 7; ++i) {
+      CHK(kmul(h, modes[i], h->D2, h->D1));
+      CHK(c2r_spectrum_d1(h, dst[i]));
+    }
+  } else {
+    const int modes[] = {KM_DXX, KM_DYY, KM_DZZ};
+    for (int i = 0; i < 3; ++i) {
+      CHK(kmul(h, modes[i], h->D2, h->D1));
+      CHK(c2r_spectrum_d1(h, f + (2 + i) * N));
+    }
+  }
+  HIPCHK(hipMemsetAsync(h->flag, 0, sizeof(unsigned int), h->st));
+  SurfParams sp;
+  sp.nx = h->cfg.nx;
+  sp.ny = h->cfg.ny;
+  sp.nz = h->cfg.nz;
+  sp.arm = opts->arm;
+  sp.vsign = opts->vel_sign;
+  sp.tdx = 2 * h->dx;  // cosNoAdvMorphCurv.cpp:167-169
+  sp.tdy = 2 * h->dy;
+  sp.tdz = 2 * h->dz;
+  const int64_t nxy = h->nxy;
+  const unsigned blocks = (unsigned)((nxy + 255) / 256);
+  if (h->neumann)
+    hipLaunchKernelGGL(k_surf_neumann, dim3(blocks), dim3(256), 0, h->st, f, f + N, f + 2 * N, f + 3 * N, f + 4 * N,
+                       sp, h->surf, h->flag);
+  else
+    hipLaunchKernelGGL(k_surf_periodic, dim3(blocks), dim3(256), 0, h->st, f, f + N, f + 2 * N, f + 3 * N, f + 4 * N,
+                       f + 5 * N, f + 6 * N, f + 7 * N, h->real_io, sp, h->surf, h->flag);
+  HIPCHK(hipGetLastError());
+  double* outs[4] = {surfz, vel, H, K};
+  for (int i = 0; i < 4; ++i)
+    HIPCHK(hipMemcpyAsync(outs[i], h->surf + i * nxy, sizeof(double) * nxy, hipMemcpyDeviceToHost, h->st));
+  unsigned int nf = 0;
+  HIPCHK(hipMemcpyAsync(&nf, h->flag, sizeof(nf), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (found) *found = nf;
+  return SMX_OK;
 }
 
 int smx_profile_step(smx_handle* h, int64_t nsteps, int max_slots, int* nslots, double* avg_ms, double* alg_bytes,
                      char* names, int names_cap) {
   if (!h || !nslots || !avg_ms || !alg_bytes) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
-  if (max_slots < kNumSlots) return fail(SMX_ERR_INVALID, "max_slots too small");
+  const int ns = num_slots(h);
+  if (max_slots < ns) return fail(SMX_ERR_INVALID, "max_slots too small");
   HIPCHK(hipSetDevice(h->cfg.device));
   std::vector<double> acc;
   CHK(h->fp32 ? profile_T<float>(h, nsteps, acc) : profile_T<double>(h, nsteps, acc));
-  const double sc = (h->fp32 ? 8.0 : 16.0) * (double)h->Nh;  // bytes of one spectral-stage array
-  const double bytes[kNumSlots] = {2 * sc, 2 * sc, 2 * sc, 4 * sc};
-  *nslots = kNumSlots;
+  const double sc = (double)h->wbytes * (double)h->nspec;  // bytes of one spectral-stage array
+  const double plane = (double)h->wbytes * (double)h->nxy;
+  const double bytes[5] = {2 * sc, 2 * sc, 2 * sc, 4 * sc, 2 * 2 * plane};
+  *nslots = ns;
   std::string nm;
-  for (int s = 0; s < kNumSlots; ++s) {
+  for (int s = 0; s < ns; ++s) {
     avg_ms[s] = acc[s];
     alg_bytes[s] = bytes[s];
-    nm += kSlotNames[s];
+    nm += h->neumann ? kSlotNamesN[s] : kSlotNamesP[s];
     nm += "\n";
   }
   if (names && names_cap > 0) {

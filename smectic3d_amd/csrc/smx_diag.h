@@ -1,0 +1,320 @@
+// smx_diag.h -- FP64 diagnostics of libsmectic: compensated reductions (L1,
+// energy, dominant wavenumber), k-space multipliers for derivative fields, and
+// the surfTrack column scans.  Spectra are normalised and laid out
+// [kz][kx][ky] (complex, periodic half spectrum) or [kz][kx][ky] (real, DCT).
+#pragma once
+#include "smx_dev.h"
+
+namespace smx {
+
+struct Kahan {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    double y = x - c, t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+struct KParams {
+  int nx, ny, nz, nplanes;
+  int64_t n;  // spectral entries
+  int neumann;
+  const double* kx;
+  const double* ky;
+  const double* kz;
+  double eps, alpha, q02;
+};
+
+// k-space multipliers
+enum KMode {
+  KM_DTPSI = 0,   // out = aLin(k) in + out         (cosNoAdvMorphCurv.cpp:744, surfTrack.cpp:611)
+  KM_IKX = 1,     // out = i kx in                 (pf3d.cpp:404)
+  KM_IKY = 2,     //                               (:405)
+  KM_IKZ = 3,     //                               (:406)
+  KM_DXX = 4,     // out = -kx^2 in                (surfTrack.cpp:615, cosNoAdvMorphCurv.cpp:746)
+  KM_DYY = 5,     //                               (surfTrack.cpp:616, cosNoAdvMorphCurv.cpp:748)
+  KM_DZZ = 6,     //                               (cosNoAdvMorphCurv.cpp:750)
+  KM_DXY = 7,     // out = (i ky)(i kx) in         (surfTrack.cpp:617)
+  KM_LAP_XY = 8,  // out = -(kx^2 + ky^2) in       (surfTrack.cpp:612-613)
+};
+
+__device__ __forceinline__ void kidx(const KParams& k, int64_t i, int& iz, int& ix, int& iy) {
+  const int64_t nxy = (int64_t)k.nx * k.ny;
+  iz = (int)(i / nxy);
+  const int64_t r = i - (int64_t)iz * nxy;
+  ix = (int)(r / k.ny);
+  iy = (int)(r - (int64_t)ix * k.ny);
+}
+
+__device__ __forceinline__ double alin(const KParams& k, double k2) {
+  const double d = k2 - k.q02;
+  return k.eps - k.alpha * d * d;
+}
+
+__global__ void k_kmul_cplx(const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out, KParams k, int mode) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k.n; i += (int64_t)gridDim.x * blockDim.x) {
+    int iz, ix, iy;
+    kidx(k, i, iz, ix, iy);
+    const double qx = k.kx[ix], qy = k.ky[iy], qz = k.kz[iz];
+    const cplx<double> a = in[i];
+    cplx<double> r;
+    switch (mode) {
+      case KM_DTPSI: {
+        const double m = alin(k, qx * qx + qy * qy + qz * qz);
+        r = mk(m * a.x + out[i].x, m * a.y + out[i].y);
+        break;
+      }
+      case KM_IKX: r = mk(-qx * a.y, qx * a.x); break;
+      case KM_IKY: r = mk(-qy * a.y, qy * a.x); break;
+      case KM_IKZ: r = mk(-qz * a.y, qz * a.x); break;
+      case KM_DXX: r = (-qx * qx) * a; break;
+      case KM_DYY: r = (-qy * qy) * a; break;
+      case KM_DZZ: r = (-qz * qz) * a; break;
+      case KM_DXY: r = (-qx * qy) * a; break;
+      default: r = (-(qx * qx) - qy * qy) * a; break;
+    }
+    out[i] = r;
+  }
+}
+
+__global__ void k_kmul_real(const double* __restrict__ in, double* __restrict__ out, KParams k, int mode) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k.n; i += (int64_t)gridDim.x * blockDim.x) {
+    int iz, ix, iy;
+    kidx(k, i, iz, ix, iy);
+    const double qx = k.kx[ix], qy = k.ky[iy], qz = k.kz[iz];
+    const double a = in[i];
+    double r;
+    switch (mode) {
+      case KM_DTPSI: r = alin(k, qx * qx + qy * qy + qz * qz) * a + out[i]; break;
+      case KM_DXX: r = -qx * qx * a; break;
+      case KM_DYY: r = -qy * qy * a; break;
+      case KM_DZZ: r = -qz * qz * a; break;
+      default: r = (-(qx * qx) - qy * qy) * a; break;
+    }
+    out[i] = r;
+  }
+}
+
+// Per-block Kahan partials over the normalised half spectrum:
+//   out[3b+0] = sum w (alpha/2 (q0^2 - k^2)^2 - eps/2) |psi_hat|^2
+//   out[3b+1] = sum_{k != 0} w |psi_hat|^2,  out[3b+2] = sum_{k != 0} w |k| |psi_hat|^2
+// w = 1 on the kz = 0 and kz = nz/2 planes, 2 elsewhere (Hermitian half).
+__global__ void k_reduce_spec_cplx(const cplx<double>* __restrict__ a, KParams k, double* __restrict__ out) {
+  __shared__ double red[64];
+  Kahan s0, s1, s2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k.n; i += (int64_t)gridDim.x * blockDim.x) {
+    int iz, ix, iy;
+    kidx(k, i, iz, ix, iy);
+    const double k2 = k.kx[ix] * k.kx[ix] + k.ky[iy] * k.ky[iy] + k.kz[iz] * k.kz[iz];
+    const double w = (iz == 0 || 2 * iz == k.nz) ? 1.0 : 2.0;
+    const double p2 = w * (a[i].x * a[i].x + a[i].y * a[i].y);
+    const double d = k.q02 - k2;
+    s0.add((0.5 * k.alpha * d * d - 0.5 * k.eps) * p2);
+    if (i != 0) {
+      s1.add(p2);
+      s2.add(sqrt(k2) * p2);
+    }
+  }
+  const double t0 = block_sum(s0.s, red), t1 = block_sum(s1.s, red), t2 = block_sum(s2.s, red);
+  if (threadIdx.x == 0) {
+    out[3 * blockIdx.x + 0] = t0;
+    out[3 * blockIdx.x + 1] = t1;
+    out[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+// Same over a DCT spectrum; w = prod over axes of (1/2 if k_axis = 0 else 1)
+// (discrete Parseval of REDFT10: sum x^2 = (1/2n)(y_0^2/2 + sum_{k>0} y_k^2) per axis).
+__global__ void k_reduce_spec_real(const double* __restrict__ a, KParams k, double* __restrict__ out) {
+  __shared__ double red[64];
+  Kahan s0, s1, s2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k.n; i += (int64_t)gridDim.x * blockDim.x) {
+    int iz, ix, iy;
+    kidx(k, i, iz, ix, iy);
+    const double k2 = k.kx[ix] * k.kx[ix] + k.ky[iy] * k.ky[iy] + k.kz[iz] * k.kz[iz];
+    const double w = (ix == 0 ? 0.5 : 1.0) * (iy == 0 ? 0.5 : 1.0) * (iz == 0 ? 0.5 : 1.0);
+    const double p2 = w * a[i] * a[i];
+    const double d = k.q02 - k2;
+    s0.add((0.5 * k.alpha * d * d - 0.5 * k.eps) * p2);
+    if (i != 0) {
+      s1.add(p2);
+      s2.add(sqrt(k2) * p2);
+    }
+  }
+  const double t0 = block_sum(s0.s, red), t1 = block_sum(s1.s, red), t2 = block_sum(s2.s, red);
+  if (threadIdx.x == 0) {
+    out[3 * blockIdx.x + 0] = t0;
+    out[3 * blockIdx.x + 1] = t1;
+    out[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+// Per-block partial of sum f_local(psi): energyPsi.m:225-228 (quintic) or the cubic model.
+__global__ void k_reduce_local(const double* __restrict__ psi, int64_t n, int nl_model, double beta, double gamma,
+                               double* __restrict__ out) {
+  __shared__ double red[64];
+  Kahan s;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double p = psi[i], p2 = p * p;
+    const double f = (nl_model == 1) ? (-beta / 3.0 * p2 * p + gamma / 4.0 * p2 * p2)
+                                     : (-beta / 4.0 * p2 * p2 + gamma / 6.0 * p2 * p2 * p2);
+    s.add(f);
+  }
+  const double t = block_sum(s.s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// Per-block partials of sum |a - b| and sum |a| (pf3d.cpp:494-502).
+__global__ void k_reduce_l1(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                            double* __restrict__ out) {
+  __shared__ double red[64];
+  Kahan s0, s1;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    s0.add(fabs(a[i] - b[i]));
+    s1.add(fabs(a[i]));
+  }
+  const double t0 = block_sum(s0.s, red), t1 = block_sum(s1.s, red);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = t0;
+    out[2 * blockIdx.x + 1] = t1;
+  }
+}
+
+template <typename T>
+__global__ void k_count_nonfinite(const T* __restrict__ a, int64_t n, unsigned int* __restrict__ out) {
+  unsigned int c = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += isfinite((double)a[i]) ? 0u : 1u;
+  if (c) atomicAdd(out, c);
+}
+
+__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
+__global__ void k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+// ---------------------------------------------------------------------------
+// surfTrack.  Fields are FP64 real, dense [x][y][z]; one thread per (i,j) column.
+// out: 4 arrays of nx*ny (surfz, vel, H, K) written only for columns that cross.
+struct SurfParams {
+  int nx, ny, nz;
+  double arm, vsign;
+  double tdx, tdy, tdz;  // 2 dx, 2 dy, 2 dz
+};
+
+// surfTrack.cpp:767-806: scan k upward, arm at psi > arm, first psi < 0 after
+// arming; derivatives taken at k-1 when |psi(k)| > |psi(k-1)|; reports k.
+__global__ void k_surf_periodic(const double* __restrict__ psi, const double* __restrict__ dt,
+                                const double* __restrict__ gx, const double* __restrict__ gy,
+                                const double* __restrict__ gz, const double* __restrict__ hnum,
+                                const double* __restrict__ dxx, const double* __restrict__ dyy,
+                                const double* __restrict__ dxy, SurfParams sp, double* __restrict__ out,
+                                unsigned int* __restrict__ nfound) {
+  const int64_t nxy = (int64_t)sp.nx * sp.ny;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= nxy) return;
+  const int64_t col = c * sp.nz;
+  int track = 0;
+  for (int k = 0; k < sp.nz; ++k) {
+    const double v = psi[col + k];
+    if (v > sp.arm && track == 0) track = 1;
+    if (v < 0.0 && track == 1) {
+      int64_t id = col + k;
+      if (fabs(v) > fabs(psi[id - 1])) id -= 1;
+      const double g = sqrt(gx[id] * gx[id] + gy[id] * gy[id] + gz[id] * gz[id]);
+      out[c] = (double)k;
+      out[nxy + c] = sp.vsign * dt[id] / g;
+      out[2 * nxy + c] = hnum[id] / g;
+      out[3 * nxy + c] = (dxx[id] * dyy[id] - dxy[id] * dxy[id]) / (g * g);
+      atomicAdd(nfound, 1u);
+      break;
+    }
+  }
+}
+
+// Central differences with one-sided (x2) stencils at the walls,
+// cosNoAdvMorphCurv.cpp:811-873.
+__device__ __forceinline__ double fd_x(const double* p, const SurfParams& s, int i, int j, int k) {
+  auto P = [&](int a) { return p[((int64_t)a * s.ny + j) * s.nz + k]; };
+  if (i > 0 && i < s.nx - 1) return (P(i + 1) - P(i - 1)) / s.tdx;
+  if (i == 0) return 2.0 * (P(1) - P(0)) / s.tdx;
+  return 2.0 * (P(i) - P(i - 1)) / s.tdx;
+}
+__device__ __forceinline__ double fd_y(const double* p, const SurfParams& s, int i, int j, int k) {
+  auto P = [&](int b) { return p[((int64_t)i * s.ny + b) * s.nz + k]; };
+  if (j > 0 && j < s.ny - 1) return (P(j + 1) - P(j - 1)) / s.tdy;
+  if (j == 0) return 2.0 * (P(1) - P(0)) / s.tdy;
+  return 2.0 * (P(j) - P(j - 1)) / s.tdy;
+}
+__device__ __forceinline__ double fd_z(const double* p, const SurfParams& s, int i, int j, int k) {
+  auto P = [&](int c) { return p[((int64_t)i * s.ny + j) * s.nz + c]; };
+  if (k > 0 && k < s.nz - 1) return (P(k + 1) - P(k - 1)) / s.tdz;
+  if (k == 0) return 2.0 * (P(1) - P(0)) / s.tdz;
+  return 2.0 * (P(k) - P(k - 1)) / s.tdz;
+}
+
+// cosNoAdvMorphCurv.cpp:877-966: scan k downward, arm at psi > arm, first psi < 0;
+// index moved to k+1 when |psi(k)| > |psi(k+1)|; Megrabov H = div(grad psi/|grad psi|)
+// and Gaussian curvature K from the FD gradient and the spectral second derivatives.
+__global__ void k_surf_neumann(const double* __restrict__ psi, const double* __restrict__ dt,
+                               const double* __restrict__ dxx, const double* __restrict__ dyy,
+                               const double* __restrict__ dzz, SurfParams sp, double* __restrict__ out,
+                               unsigned int* __restrict__ nfound) {
+  const int64_t nxy = (int64_t)sp.nx * sp.ny;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= nxy) return;
+  const int i = (int)(c / sp.ny), j = (int)(c - (int64_t)i * sp.ny);
+  const int64_t col = c * sp.nz;
+  int track = 0;
+  for (int k = sp.nz - 1; k > -1; --k) {
+    const double v = psi[col + k];
+    if (v > sp.arm && track == 0) track = 1;
+    if (v < 0.0 && track == 1) {
+      int k2 = k;
+      if (fabs(v) > fabs(psi[col + k + 1])) k2 = k + 1;
+      const int64_t id = col + k2;
+      const double px = fd_x(psi, sp, i, j, k2), py = fd_y(psi, sp, i, j, k2), pz = fd_z(psi, sp, i, j, k2);
+      const double g = sqrt(px * px + py * py + pz * pz);
+      double d_xy;
+      if (j > 0 && j < sp.ny - 1) d_xy = (fd_x(psi, sp, i, j + 1, k2) - fd_x(psi, sp, i, j - 1, k2)) / sp.tdy;
+      else if (j == 0) d_xy = 2.0 * (fd_x(psi, sp, i, 1, k2) - fd_x(psi, sp, i, 0, k2)) / sp.tdy;
+      else d_xy = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j - 1, k2)) / sp.tdy;
+      // the reference picks the z stencil by k while centring it on k2 (:926-940); a
+      // centred stencil that would leave the column falls back to one-sided
+      double d_xz, d_yz;
+      if (k > 0 && k < sp.nz - 1 && k2 - 1 >= 0 && k2 + 1 <= sp.nz - 1) {
+        d_xz = (fd_x(psi, sp, i, j, k2 + 1) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
+        d_yz = (fd_y(psi, sp, i, j, k2 + 1) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
+      } else if (k == 0 || k2 - 1 < 0) {
+        d_xz = 2.0 * (fd_x(psi, sp, i, j, k2 + 1) - fd_x(psi, sp, i, j, k2)) / sp.tdz;
+        d_yz = 2.0 * (fd_y(psi, sp, i, j, k2 + 1) - fd_y(psi, sp, i, j, k2)) / sp.tdz;
+      } else {
+        d_xz = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
+        d_yz = 2.0 * (fd_y(psi, sp, i, j, k2) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
+      }
+      const double fxx = dxx[id], fyy = dyy[id], fzz = dzz[id];
+      const double Hc = ((py * py + pz * pz) * fxx + (px * px + pz * pz) * fyy + (px * px + py * py) * fzz -
+                         2.0 * (px * py * d_xy + px * pz * d_xz + py * pz * d_yz)) /
+                        (g * g * g);
+      const double Kc = (pz * pz * (fxx * fyy - d_xy * d_xy) + px * px * (fyy * fzz - d_yz * d_yz) +
+                         py * py * (fxx * fzz - d_xz * d_xz) +
+                         2.0 * (py * d_xy * (pz * d_xz - px * fzz) + px * d_xz * (py * d_yz - pz * fyy) +
+                                pz * d_yz * (px * d_xy - py * fxx))) /
+                        (g * g * g * g);
+      out[c] = (double)k2;
+      out[nxy + c] = sp.vsign * dt[id] / g;
+      out[2 * nxy + c] = Hc;
+      out[3 * nxy + c] = Kc;
+      atomicAdd(nfound, 1u);
+      break;
+    }
+  }
+}
+
+}  // namespace smx

@@ -1,3 +1,4 @@
 // Instantiates the FP32 pass kernels (FP32-mixed path).
 #include "smx_launch.h"
 template struct smx::PeriodicOps<float>;
+template struct smx::NeumannOps<float>;

@@ -1,3 +1,4 @@
 // Instantiates the FP64 pass kernels (FP64 path and all diagnostics).
 #include "smx_launch.h"
 template struct smx::PeriodicOps<double>;
+template struct smx::NeumannOps<double>;

@@ -1,7 +1,9 @@
 // smx_launch.h -- runtime-length -> template-instance dispatch for the pass kernels.
 // Instantiated once per precision in smx_inst_f32.hip / smx_inst_f64.hip.
 #pragma once
-#include "smx_periodic.h"
+#include <algorithm>
+
+#include "smx_neumann.h"
 
 namespace smx {
 
@@ -30,6 +32,13 @@ struct PeriodicOps {
   static hipError_t zpass(int in_mode, int out_mode, int op, const cplx<T>* in, cplx<T>* out,
                           const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st);
   static int zpass_blocks(const PParams<T>& p) { return (int)((p.nxy + kNC - 1) / kNC); }
+  // Hermitian projection of the kz = 0 and kz = nz/2 planes of a full spectrum.
+  static hipError_t herm_project(cplx<T>* a, const PParams<T>& p, hipStream_t st) {
+    const int64_t work = 2 * p.nxy;
+    const unsigned blocks = (unsigned)std::min<int64_t>((work + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_herm_project<T>), dim3(blocks), dim3(256), 0, st, a, p.nx, p.ny, p.nzh, p.nz);
+    return hipGetLastError();
+  }
 };
 
 template <typename T>
@@ -82,6 +91,78 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
       hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
     else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
       hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
+    else                                                                                             \
+      return hipErrorInvalidValue;                                                                   \
+    return hipGetLastError();                                                                        \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+
+template <typename T>
+struct NeumannOps {
+  // y pass on real pencils (npencils = nz*nx), dir -1: DCT-II, +1: DCT-III.
+  static hipError_t ypass(const T* in, T* out, int ny, int64_t npencils, int dir, const cplx<T>* tw,
+                          const cplx<T>* qtw, hipStream_t st);
+  static hipError_t xpass(int mode, const T* in, T* out, T* rbuf, const T* psi0, const PParams<T>& p,
+                          const cplx<T>* qtw, hipStream_t st);
+  static hipError_t zpass(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
+                          double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st);
+  static int zpass_blocks(const PParams<T>& p) { return (int)((p.nxy + kNC - 1) / kNC); }
+};
+
+template <typename T>
+hipError_t NeumannOps<T>::ypass(const T* in, T* out, int ny, int64_t npencils, int dir, const cplx<T>* tw,
+                                const cplx<T>* qtw, hipStream_t st) {
+  const dim3 grid((unsigned)((npencils + kNC - 1) / kNC));
+#define SMX_CASE(L)                                                                                 \
+  if (ny == L) {                                                                                    \
+    const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                 \
+    if (dir < 0) hipLaunchKernelGGL((k_dct_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw); \
+    else hipLaunchKernelGGL((k_dct_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw);          \
+    return hipGetLastError();                                                                       \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t NeumannOps<T>::xpass(int mode, const T* in, T* out, T* rbuf, const T* psi0, const PParams<T>& p,
+                                const cplx<T>* qtw, hipStream_t st) {
+  const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
+#define SMX_CASE(L)                                                                                   \
+  if (p.nx == L) {                                                                                    \
+    const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                   \
+    switch (mode) {                                                                                   \
+      case X_STEP: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_STEP>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      case X_FWD_SCALED: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      case X_BOOT: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      case X_INV: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      default: return hipErrorInvalidValue;                                                           \
+    }                                                                                                 \
+    return hipGetLastError();                                                                         \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t NeumannOps<T>::zpass(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
+                                double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st) {
+  const dim3 grid((unsigned)zpass_blocks(p));
+#define SMX_CASE(L)                                                                                  \
+  if (p.nz == L) {                                                                                   \
+    const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                  \
+    if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)                                              \
+      hipLaunchKernelGGL((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+    else if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                         \
+      hipLaunchKernelGGL((k_dct_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+    else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
+      hipLaunchKernelGGL((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
     else                                                                                             \
       return hipErrorInvalidValue;                                                                   \
     return hipGetLastError();                                                                        \

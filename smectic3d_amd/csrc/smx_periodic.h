@@ -181,6 +181,38 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
 }
 
 // ---------------------------------------------------------------------------
+// Hermitian projection of the self-conjugate planes kz = 0 and kz = nz/2 of a
+// full spectrum [kz][kx][ky]: X(k) <- (X(k) + conj X(-k))/2.  The reference
+// re-derives psi_hat from a real field every step (R2C of psi, pf3d.cpp:395),
+// which keeps these planes Hermitian; the carried CNAB2 state has to be
+// projected explicitly, or the rounding-level anti-Hermitian part -- invisible
+// to the C2R, which reads only real parts there -- grows by
+// (1 + dt L/2)/(1 - dt L/2) per step in the unstable band L(k) > 0.
+// One thread per conjugate pair; self-conjugate entries lose their imaginary part.
+template <typename T>
+__global__ void k_herm_project(cplx<T>* __restrict__ a, int nx, int ny, int nzh, int nz) {
+  const int64_t nxy = (int64_t)nx * ny;
+  const int nplanes = (2 * (nzh - 1) == nz) ? 2 : 1;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nplanes * nxy;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int pl = (int)(t / nxy);
+    const int64_t i = t - pl * nxy;
+    const int ix = (int)(i / ny), iy = (int)(i - (int64_t)ix * ny);
+    const int jx = (nx - ix) % nx, jy = (ny - iy) % ny;
+    const int64_t j = (int64_t)jx * ny + jy;
+    cplx<T>* p = a + (int64_t)(pl == 0 ? 0 : nzh - 1) * nxy;
+    if (i == j) {
+      p[i].y = (T)0;
+    } else if (i < j) {
+      const cplx<T> u = p[i], w = p[j];
+      const cplx<T> m = mk((T)0.5 * (u.x + w.x), (T)0.5 * (u.y - w.y));
+      p[i] = m;
+      p[j] = conjg(m);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // z pass.  Tile: columns f0 .. f0+NC-1 of the (x,y) plane; column pairs
 // (q, q + NC/2) share one length-nz complex FFT (z = psi_a + i psi_b).
 enum ZIn { Z_IN_SPEC = 0, Z_IN_REAL = 1 };

@@ -149,3 +149,65 @@ def test_errors():
             s.step(1)  # no state yet
         with pytest.raises(SmxError):
             s.get_psi()
+
+
+LENGTHS = [16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512]
+
+
+def _axis_shapes():
+    out = []
+    for n in LENGTHS:
+        out.append((n, 16, 16))
+        if n % 16 == 0:
+            out.append((16, n, 16))
+        out.append((16, 16, n))
+    return out
+
+
+@pytest.mark.parametrize("shape", _axis_shapes())
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+def test_every_length_parity(shape, prec):
+    """Every radix plan in every axis role (x pass, y pass, z pass) against the oracle."""
+    say("length parity", shape, prec)
+    cfg = cfg_reference(*shape, eps=0.5, beta=0.3, dt=0.05, nl_model=A.SMX_NL_CUBIC, precision=prec)
+    psi = noise(shape, 17, 0.3)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(5)
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        r0 = rel_l2(s.get_psi(), psi)
+        s.step(5)
+        e = rel_l2(s.get_psi(), orc.get_psi())
+    say("  roundtrip", r0, "5-step", e)
+    tol = 1e-11 if prec == A.SMX_FP64 else 2e-5
+    assert r0 < tol and e < tol
+
+
+def test_long_run_hermitian_planes():
+    """2000 steps of the hyperboloid workload at 64^3 (dx = 2pi/8, eps=0.3, dt=0.1, cubic).
+
+    Regression for the carried CNAB2 state: the kz = 0 and kz = nz/2 planes must
+    stay Hermitian, or their invisible anti-Hermitian part grows in the unstable
+    band and the energy diverges while psi still looks right."""
+    import bench
+    say("long run 64^3 x 2000 hyperboloid")
+    n = 64
+    dx = (n / 8 * 2 * math.pi) / n
+    psi = bench.hyperboloid_ic(n, dx)
+    orc = O.Oracle(bench.workload_cfg(A.SMX_FP64, n))
+    orc.set_psi(psi)
+    orc.step(2000)
+    oe, op = orc.energy(), orc.get_psi()
+    with Solver(bench.workload_cfg(A.SMX_FP64, n)) as s:
+        s.set_psi(psi)
+        s.step(2000)
+        e, ge = rel_l2(s.get_psi(), op), s.energy()
+        say("  fp64 rel L2", e, "rel energy", abs(ge - oe) / abs(oe))
+        assert e < 1e-10 and abs(ge - oe) / abs(oe) < 1e-12
+    with Solver(bench.workload_cfg(A.SMX_FP32_MIXED, n)) as s:
+        s.set_psi(psi)
+        s.step(2000)
+        e, ge = rel_l2(s.get_psi(), op), s.energy()
+        say("  fp32-mixed rel L2", e, "rel energy", abs(ge - oe) / abs(oe))
+        assert np.isfinite(ge) and abs(ge - oe) / abs(oe) < 1e-3

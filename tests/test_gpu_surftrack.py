@@ -1,0 +1,76 @@
+"""GPU surfTrack (periodic: surfTrack.cpp:767-806; Neumann: cosNoAdvMorphCurv.cpp:811-966)
+against the FP64 oracle on identical states."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+from oracle import oracle as O
+from smectic3d_amd import Solver
+from smectic3d_amd import _abi as A
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _compare(s, orc, arm, vsign):
+    (gz, gv, gh, gk), gf = s.surftrack(arm, vsign)
+    (oz, ov, oh, ok), of = orc.surftrack(arm, vsign)
+    say("  found", gf, of)
+    assert gf == of and gf > 0
+    np.testing.assert_array_equal(gz, oz)
+    for a, b in ((gv, ov), (gh, oh), (gk, ok)):
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max())
+
+
+def test_surftrack_neumann_matches_oracle():
+    nx, ny, nz = 32, 32, 48
+    dx = 2 * math.pi / 8
+    cfg = A.make_config(nx, ny, nz, bc=A.SMX_BC_NEUMANN, nw=8, eps=0.5, beta=0.0, gamma=1.0, dt=0.05,
+                        nl_model=A.SMX_NL_CUBIC)
+    psi = O.ic_hyperboloid(nx, ny, nz, dx, dx, dx, 1.0, 0.9, 1e-3, 0)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(40)
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        s.step(40)
+        say("neumann surftrack")
+        _compare(s, orc, 0.7, -1.0)
+
+
+def test_surftrack_periodic_matches_oracle():
+    nx, ny, nz = 32, 32, 48
+    dx = 2 * math.pi / 8
+    cfg = A.make_config(nx, ny, nz, nw=8, eps=0.5, beta=0.0, gamma=1.0, dt=0.05, nl_model=A.SMX_NL_CUBIC)
+    psi = O.ic_hyperboloid(nx, ny, nz, dx, dx, dx, 1.0, 0.9, 1e-3, 0)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(40)
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        s.step(40)
+        say("periodic surftrack")
+        _compare(s, orc, 0.1, 1.0)
+
+
+def test_surftrack_columns_without_crossing_keep_previous_values():
+    nx, ny, nz = 16, 16, 32
+    cfg = A.make_config(nx, ny, nz, bc=A.SMX_BC_NEUMANN, lx=8.0, ly=8.0, lz=16.0, eps=0.3, beta=0.0,
+                        gamma=1.0, nl_model=A.SMX_NL_CUBIC)
+    z = (np.arange(nz) + 0.5) * 0.5
+    psi = np.broadcast_to(np.tanh((z - 6.0) / 1.5), (nx, ny, nz)).copy()
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        (z1, v1, h1, k1), f1 = s.surftrack(0.7)
+        assert f1 == nx * ny
+        s.set_psi(-np.ones((nx, ny, nz)))  # no column arms
+        (z2, v2, h2, k2), f2 = s.surftrack(0.7)
+        assert f2 == 0
+        np.testing.assert_array_equal(z2, z1)  # persistent values, as the reference's surf arrays
