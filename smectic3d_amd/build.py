@@ -47,6 +47,20 @@ def _compile(unit):
     return obj
 
 
+DRIVER_SRC = os.path.join(HERE, "driver", "smx_run.cpp")
+DRIVER = os.path.join(HERE, "driver", "smx_run")
+
+
+def build_driver():
+    """The C++ driver (reference main() loops over the C ABI), linked against libsmectic.so."""
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, DRIVER_SRC,
+           "-L", HERE, "-lsmectic", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stderr)
+    return DRIVER
+
+
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     jobs = min(len(UNITS), int(os.environ.get("MAX_JOBS", "8")))
@@ -58,8 +72,9 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
 
 

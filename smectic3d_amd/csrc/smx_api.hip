@@ -69,6 +69,11 @@ struct smx_handle {
   double* mean_state = nullptr;
   double* red = nullptr;  // reduction partials
   unsigned int* flag = nullptr;
+  void* Ax = nullptr;  // advected step: x-inverse of i kx psi_hat (then d_x psi at (x,y,kz))
+  void* As = nullptr;  // x-inverse of opSH psi_hat; reused for S_x, U_x and N
+  void* By = nullptr;  // d_y psi at (x,y,kz)
+  void* S1 = nullptr;  // S_y then U_y
+  void* S2 = nullptr;  // S_z then U_z
   double* surf = nullptr;    // 4 x nxy persistent surfTrack outputs (zero-initialised)
   double* sfield = nullptr;  // surfTrack real fields, 8 x N (allocated on first use)
   DevTables<float> t32;
@@ -187,6 +192,15 @@ static PParams<T> params(smx_handle* h, bool mixed_mean) {
   return p;
 }
 
+template <typename T>
+static AParams<T> aparams(smx_handle* h, bool mixed_mean) {
+  AParams<T> a;
+  a.p = params<T>(h, mixed_mean);
+  a.cm = (T)(1.0 / (h->cfg.rho * h->cfg.eta));  // CM1 = 1/(rho eta |k|^2), pf3d.cpp:210
+  a.force = h->cfg.force_form;
+  return a;
+}
+
 // ---- pass facade over both boundary conditions ----
 template <typename T>
 static int ypass(smx_handle* h, const void* in, void* out, int dir) {
@@ -238,7 +252,13 @@ static int bootstrap(smx_handle* h) {
   CHK(xpass<T>(h, p, X_FWD_SCALED, h->A, h->P, nullptr, nullptr));
   CHK(zpass<T>(h, p, Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, h->A, h->real_io, nullptr));
   CHK(ypass<T>(h, h->A, h->A, -1));
-  CHK(xpass<T>(h, p, X_BOOT, h->A, h->A, h->R, h->P));
+  if (h->cfg.advect) {
+    hipError_t e = AdvectOps<T>::xpass_adv(X_BOOT, (const cplx<T>*)h->A, (cplx<T>*)h->R, (const cplx<T>*)h->P,
+                                           (cplx<T>*)h->A, (cplx<T>*)h->Ax, (cplx<T>*)h->As, p, h->st);
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("x pass (advected): ") + hipGetErrorString(e));
+  } else {
+    CHK(xpass<T>(h, p, X_BOOT, h->A, h->A, h->R, h->P));
+  }
   CHK(herm<T>(h, p, h->R));
   return SMX_OK;
 }
@@ -246,10 +266,74 @@ static int bootstrap(smx_handle* h) {
 // Kernel slots of one step, in launch order.
 static const char* kSlotNamesP[] = {"k_ypass_inv", "k_zpass_nl", "k_ypass_fwd", "k_xpass_step", "k_herm_project"};
 static const char* kSlotNamesN[] = {"k_dct_ypass_inv", "k_dct_zpass_nl", "k_dct_ypass_fwd", "k_dct_xpass_step"};
-static int num_slots(const smx_handle* h) { return h->neumann ? 4 : 5; }
+static const char* kSlotNamesA[] = {"k_ypass_inv_dy", "k_ypass_inv_psi", "k_ypass_inv_dx", "k_ypass_inv_sh",
+                                    "k_zpass_force", "k_ypass_fwd_s0", "k_ypass_fwd_s1", "k_ypass_fwd_s2",
+                                    "k_xpass_stokes", "k_ypass_inv_u0", "k_ypass_inv_u1", "k_ypass_inv_u2",
+                                    "k_zpass_advnl", "k_ypass_fwd_n", "k_xpass_adv", "k_herm_project"};
+static int num_slots(const smx_handle* h) { return h->neumann ? 4 : (h->cfg.advect ? 16 : 5); }
+static const char* slot_name(const smx_handle* h, int s) {
+  return h->neumann ? kSlotNamesN[s] : (h->cfg.advect ? kSlotNamesA[s] : kSlotNamesP[s]);
+}
+
+// One advected periodic step (pf3d.cpp:395-485): 16 launches, in place per tile.
+template <typename T>
+static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
+  using Ops = PeriodicOps<T>;
+  using Adv = AdvectOps<T>;
+  const PParams<T>& p = ap.p;
+  const DevTables<T>& t = tables<T>(h);
+  cplx<T>*A = (cplx<T>*)h->A, *Ax = (cplx<T>*)h->Ax, *As = (cplx<T>*)h->As, *By = (cplx<T>*)h->By;
+  cplx<T>*S1 = (cplx<T>*)h->S1, *S2 = (cplx<T>*)h->S2, *R = (cplx<T>*)h->R;
+  int slot = 0;
+  hipError_t e = hipSuccess;
+#define SMX_REC()                                             \
+  do {                                                        \
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e)); \
+    if (ev) HIPCHK(hipEventRecord(ev[++slot], h->st));        \
+  } while (0)
+  if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
+  e = Ops::ypass(A, By, p.ny, h->npencils, +1, t.twy, h->st, t.ky);  // d_y psi (pf3d.cpp:405,411)
+  SMX_REC();
+  e = Ops::ypass(A, A, p.ny, h->npencils, +1, t.twy, h->st);
+  SMX_REC();
+  e = Ops::ypass(Ax, Ax, p.ny, h->npencils, +1, t.twy, h->st);
+  SMX_REC();
+  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st);
+  SMX_REC();
+  e = Adv::zforce(A, Ax, By, As, As, S1, S2, ap, h->st);  // S = f grad psi (:417-425)
+  SMX_REC();
+  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st);
+  SMX_REC();
+  e = Ops::ypass(S1, S1, p.ny, h->npencils, -1, t.twy, h->st);
+  SMX_REC();
+  e = Ops::ypass(S2, S2, p.ny, h->npencils, -1, t.twy, h->st);
+  SMX_REC();
+  e = Adv::stokes(As, S1, S2, ap, h->st);  // (:433-447)
+  SMX_REC();
+  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st);
+  SMX_REC();
+  e = Ops::ypass(S1, S1, p.ny, h->npencils, +1, t.twy, h->st);
+  SMX_REC();
+  e = Ops::ypass(S2, S2, p.ny, h->npencils, +1, t.twy, h->st);
+  SMX_REC();
+  e = Adv::zadvnl(As, S1, S2, A, Ax, By, As, ap, h->st);  // N = N(psi) - U.grad psi (:455-465)
+  SMX_REC();
+  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st);
+  SMX_REC();
+  e = Adv::xpass_adv(X_STEP, As, R, nullptr, A, Ax, As, p, h->st);  // CNAB2 (:471-481)
+  SMX_REC();
+  e = Ops::herm_project(R, p, h->st);
+  SMX_REC();
+#undef SMX_REC
+  return SMX_OK;
+}
 
 template <typename T>
 static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
+  if (h->cfg.advect) {
+    AParams<T> ap = aparams<T>(h, h->fp32);
+    return step_advect<T>(h, ap, ev);
+  }
   if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
   CHK(ypass<T>(h, h->A, h->A, +1));
   if (ev) HIPCHK(hipEventRecord(ev[1], h->st));
@@ -478,7 +562,6 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
   if (!(c.lx > 0 && c.ly > 0 && c.lz > 0 && c.dt > 0)) return fail(SMX_ERR_INVALID, "lengths and dt must be > 0");
   if (c.advect && c.bc == SMX_BC_NEUMANN)
     return fail(SMX_ERR_UNSUPPORTED, "advection is periodic only (the reference's Neumann drivers do not advect)");
-  if (c.advect) return fail(SMX_ERR_UNSUPPORTED, "advection not built yet");
   for (int n : {c.nx, c.ny, c.nz})
     if (!length_supported(n))
       return fail(SMX_ERR_UNSUPPORTED, "grid length " + std::to_string(n) +
@@ -521,6 +604,11 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
       (rc = dalloc(h, (void**)&h->red, sizeof(double) * 4 * kRedBlocks)) ||
       (rc = dalloc(h, (void**)&h->flag, sizeof(unsigned int) * 4)) ||
       (rc = dalloc(h, (void**)&h->surf, sizeof(double) * 4 * h->nxy)))
+    return bail(rc);
+  if (c.advect &&
+      ((rc = dalloc(h, &h->Ax, h->wbytes * h->nspec)) || (rc = dalloc(h, &h->As, h->wbytes * h->nspec)) ||
+       (rc = dalloc(h, &h->By, h->wbytes * h->nspec)) || (rc = dalloc(h, &h->S1, h->wbytes * h->nspec)) ||
+       (rc = dalloc(h, &h->S2, h->wbytes * h->nspec))))
     return bail(rc);
   if ((rc = upload_tables<double>(h, h->t64))) return bail(rc);
   if (h->fp32 && (rc = upload_tables<float>(h, h->t32))) return bail(rc);
@@ -644,6 +732,7 @@ int smx_dominant_wavenumber(smx_handle* h, double* q) {
 int smx_dtpsi(smx_handle* h, double* out, int64_t row_stride) {
   if (!h || !out) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  if (h->cfg.advect) return fail(SMX_ERR_UNSUPPORTED, "dpsi/dt of an advected state is not built");
   const smx_config& c = h->cfg;
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
@@ -658,6 +747,7 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
                   int64_t* found) {
   if (!h || !opts || !surfz || !vel || !H || !K) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  if (h->cfg.advect) return fail(SMX_ERR_UNSUPPORTED, "surfTrack of an advected state needs dpsi/dt with advection (not built)");
   HIPCHK(hipSetDevice(h->cfg.device));
   const int64_t N = h->N;
   if (!h->sfield) CHK(dalloc(h, (void**)&h->sfield, sizeof(double) * 8 * N));
@@ -723,13 +813,18 @@ int smx_profile_step(smx_handle* h, int64_t nsteps, int max_slots, int* nslots, 
   CHK(h->fp32 ? profile_T<float>(h, nsteps, acc) : profile_T<double>(h, nsteps, acc));
   const double sc = (double)h->wbytes * (double)h->nspec;  // bytes of one spectral-stage array
   const double plane = (double)h->wbytes * (double)h->nxy;
-  const double bytes[5] = {2 * sc, 2 * sc, 2 * sc, 4 * sc, 2 * 2 * plane};
+  std::vector<double> bytes;
+  if (h->cfg.advect)  // reads + writes of spectral-stage arrays per launch
+    bytes = {2 * sc, 2 * sc, 2 * sc, 2 * sc, 7 * sc, 2 * sc, 2 * sc, 2 * sc,
+             6 * sc, 2 * sc, 2 * sc, 2 * sc, 7 * sc, 2 * sc, 6 * sc, 2 * 2 * plane};
+  else
+    bytes = {2 * sc, 2 * sc, 2 * sc, 4 * sc, 2 * 2 * plane};
   *nslots = ns;
   std::string nm;
   for (int s = 0; s < ns; ++s) {
     avg_ms[s] = acc[s];
     alg_bytes[s] = bytes[s];
-    nm += h->neumann ? kSlotNamesN[s] : kSlotNamesP[s];
+    nm += slot_name(h, s);
     nm += "\n";
   }
   if (names && names_cap > 0) {

@@ -2,3 +2,4 @@
 #include "smx_launch.h"
 template struct smx::PeriodicOps<float>;
 template struct smx::NeumannOps<float>;
+template struct smx::AdvectOps<float>;

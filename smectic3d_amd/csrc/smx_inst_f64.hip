@@ -2,3 +2,4 @@
 #include "smx_launch.h"
 template struct smx::PeriodicOps<double>;
 template struct smx::NeumannOps<double>;
+template struct smx::AdvectOps<double>;

@@ -3,6 +3,7 @@
 #pragma once
 #include <algorithm>
 
+#include "smx_advect.h"
 #include "smx_neumann.h"
 
 namespace smx {
@@ -24,7 +25,7 @@ template <typename T>
 struct PeriodicOps {
   // y pass, in place allowed.  sign -1 forward, +1 inverse.
   static hipError_t ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
-                          const cplx<T>* tw, hipStream_t st);
+                          const cplx<T>* tw, hipStream_t st, const T* kmul = nullptr);
   // x pass (mode: XMode), tiles over (y, kz).
   static hipError_t xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx<T>* rbuf, const cplx<T>* psi0,
                           const PParams<T>& p, hipStream_t st);
@@ -43,13 +44,13 @@ struct PeriodicOps {
 
 template <typename T>
 hipError_t PeriodicOps<T>::ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
-                                 const cplx<T>* tw, hipStream_t st) {
+                                 const cplx<T>* tw, hipStream_t st, const T* kmul) {
   const dim3 grid((unsigned)((npencils + kNC - 1) / kNC));
 #define SMX_CASE(L)                                                                           \
   if (ny == L) {                                                                              \
     const dim3 block(kNC * (L / Plan<L>::E));                                                 \
-    if (sign < 0) hipLaunchKernelGGL((k_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw); \
-    else hipLaunchKernelGGL((k_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw);          \
+    if (sign < 0) hipLaunchKernelGGL((k_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw, kmul); \
+    else hipLaunchKernelGGL((k_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw, kmul);          \
     return hipGetLastError();                                                                 \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -171,5 +172,77 @@ hipError_t NeumannOps<T>::zpass(int in_mode, int out_mode, int op, const T* in, 
 #undef SMX_CASE
   return hipErrorInvalidValue;
 }
+
+
+// Widest tile (16, 8 or 4 columns) whose `ntiles` LDS tiles of L x (NC+1) complex fit in 150 KB.
+template <typename T, int L>
+constexpr int nc_fit(int ntiles) {
+  return (ntiles * L * 17 * (int)sizeof(cplx<T>) <= 150 * 1024) ? 16
+         : (ntiles * L * 9 * (int)sizeof(cplx<T>) <= 150 * 1024) ? 8
+                                                                  : 4;
+}
+
+template <typename T>
+struct AdvectOps {
+  static hipError_t xpass_adv(int mode, const cplx<T>* in, cplx<T>* rbuf, const cplx<T>* psi0, cplx<T>* oa,
+                              cplx<T>* ox, cplx<T>* os, const PParams<T>& p, hipStream_t st) {
+#define SMX_CASE(L)                                                                                       \
+  if (p.nx == L) {                                                                                        \
+    constexpr int NC = nc_fit<T, L>(2);                                                                   \
+    if (p.ny % NC) return hipErrorInvalidValue;                                                           \
+    const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block(NC * (L / Plan<L>::E));                \
+    if (mode == X_STEP)                                                                                   \
+      hipLaunchKernelGGL((k_xpass_adv<T, L, NC, X_STEP>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_xpass_adv<T, L, NC, X_BOOT>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
+    return hipGetLastError();                                                                             \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  static hipError_t stokes(cplx<T>* s0, cplx<T>* s1, cplx<T>* s2, const AParams<T>& ap, hipStream_t st) {
+    const PParams<T>& p = ap.p;
+#define SMX_CASE(L)                                                                                \
+  if (p.nx == L) {                                                                                 \
+    constexpr int NC = nc_fit<T, L>(3);                                                            \
+    if (p.ny % NC) return hipErrorInvalidValue;                                                    \
+    const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block(NC * (L / Plan<L>::E));         \
+    hipLaunchKernelGGL((k_xpass_stokes<T, L, NC>), grid, block, 0, st, s0, s1, s2, ap);            \
+    return hipGetLastError();                                                                      \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  static hipError_t zforce(const cplx<T>* bpsi, const cplx<T>* bx, const cplx<T>* by, const cplx<T>* bs,
+                           cplx<T>* o0, cplx<T>* o1, cplx<T>* o2, const AParams<T>& ap, hipStream_t st) {
+    const PParams<T>& p = ap.p;
+    const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
+#define SMX_CASE(L)                                                                                     \
+  if (p.nz == L) {                                                                                      \
+    hipLaunchKernelGGL((k_zpass_force<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, bpsi, bx, by, \
+                       bs, o0, o1, o2, ap);                                                             \
+    return hipGetLastError();                                                                           \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  static hipError_t zadvnl(const cplx<T>* u0, const cplx<T>* u1, const cplx<T>* u2, const cplx<T>* bpsi,
+                           const cplx<T>* bx, const cplx<T>* by, cplx<T>* nout, const AParams<T>& ap, hipStream_t st) {
+    const PParams<T>& p = ap.p;
+    const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
+#define SMX_CASE(L)                                                                                     \
+  if (p.nz == L) {                                                                                      \
+    hipLaunchKernelGGL((k_zpass_advnl<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, u0, u1, u2, \
+                       bpsi, bx, by, nout, ap);                                                         \
+    return hipGetLastError();                                                                           \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+};
 
 }  // namespace smx

@@ -52,7 +52,8 @@ __device__ __forceinline__ T nonlin(const PParams<T>& p, T s) {
 template <typename T, int L, int SIGN, int NC>
 __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_ypass(const cplx<T>* in, cplx<T>* out,
                                                                int64_t npencils,
-                                                               const cplx<T>* __restrict__ tw) {
+                                                               const cplx<T>* __restrict__ tw,
+                                                               const T* __restrict__ kmul) {
   constexpr int E = Plan<L>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
   __shared__ cplx<T> S[L * NCP];
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
@@ -70,6 +71,13 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_ypass(const cplx<T>* in
   constexpr int RF = (SIGN < 0) ? fwd_first<L>() : inv_first<L>();
   constexpr int RL = (SIGN < 0) ? fwd_last<L>() : inv_last<L>();
   Tile<T, L, E, NCP>::template load<RF>(v, S, col, tc);
+  if (kmul != nullptr) {  // multiply by i k_y before the transform (spectral d/dy, pf3d.cpp:405)
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const T k = kmul[Tile<T, L, E, NCP>::template pos_in<RF>(tc, e)];
+      v[e] = mk(-k * v[e].y, k * v[e].x);
+    }
+  }
   FFT::run(v, S, col, tc, tw);
   __syncthreads();
   Tile<T, L, E, NCP>::template store<RL>(v, S, col, tc);

@@ -1,0 +1,51 @@
+"""The C++ driver (smectic3d_amd/driver/smx_run): initial conditions and the
+reference text-file format, checked on the CPU (--ic-only needs no device)."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from smectic3d_amd import build as B
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(B.DRIVER):
+        B.build_driver()
+    return B.DRIVER
+
+
+def run_ic(driver, tmp_path, *args):
+    subprocess.run([driver, "--ic-only", "--out", str(tmp_path), *args], check=True)
+    return (tmp_path / "psiIC.dat").read_text()
+
+
+def test_dump_format_matches_reference_writer(driver, tmp_path):
+    """`os << v << "\\n "` per value (pf3d.cpp:351): default ostream precision, 6 significant digits."""
+    txt = run_ic(driver, tmp_path, "--preset", "plain", "--n", "16", "--seed", "5", "--amp", "0.01")
+    lines = txt.split("\n ")
+    assert lines[-1] == "" and len(lines) == 16 ** 3 + 1
+    vals = np.array([float(v) for v in lines[:-1]])
+    want = O.ic_noise(16, 16, 16, 5, 0.01).ravel()
+    for s, w in zip(lines[:50], want[:50]):
+        assert s == "%.6g" % w
+    assert np.abs(vals - want).max() <= 5e-6 * np.abs(want).max()
+
+
+def test_hyperboloid_ic_matches_oracle(driver, tmp_path):
+    txt = run_ic(driver, tmp_path, "--preset", "hyper", "--n", "16", "--amp", "1e-3", "--seed", "0")
+    vals = np.array([float(v) for v in txt.split("\n ")[:-1]]).reshape(16, 16, 16)
+    dx = 2 * math.pi / 8
+    want = O.ic_hyperboloid(16, 16, 16, dx, dx, dx, 1.0, 0.4, 1e-3, 0)
+    np.testing.assert_allclose(vals, want, rtol=1e-5, atol=1e-6)
+
+
+def test_file_ic_rejects_short_file(driver, tmp_path):
+    p = tmp_path / "short.dat"
+    p.write_text("0.1\n 0.2\n ")
+    r = subprocess.run([driver, "--ic-only", "--out", str(tmp_path), "--n", "16", "--ic", f"file:{p}"],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "short" in r.stderr
