@@ -144,6 +144,18 @@ typedef struct smx_surf_opts {
 int smx_surftrack(smx_handle *h, const smx_surf_opts *opts, double *surfz, double *vel,
                   double *H, double *K, int64_t *found);
 
+/*
+ * Gradient of J = 1/2 sum_x (psi_T(x) - amp cos(q0 z))^2, z = k dz, after nsteps periodic
+ * CNAB2 steps from psi0 (host, reference layout), with respect to psi0 (host array,
+ * same layout), eps and q0 (the grid spacing does not move with q0).  Hand-written
+ * discrete adjoint of the step (the transpose of pf3d.cpp:471-481 with the bootstrap
+ * :363-374); the forward trajectory is checkpointed every ckpt_every steps and each
+ * segment is recomputed once.  No counterpart in the reference (it has no adjoint);
+ * the periodic, non-advected, penalty-free step only.  Leaves the handle at psi0.
+ */
+int smx_adjoint_grad(smx_handle *h, const double *psi0, int64_t row_stride, int64_t nsteps, int64_t ckpt_every,
+                     double target_amp, double *loss, double *dpsi0, double *deps, double *dq0);
+
 /* Block until the handle's stream is idle. */
 int smx_sync(smx_handle *h);
 

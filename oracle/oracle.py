@@ -58,6 +58,8 @@ def lib():
                                         C.c_uint64, _dp]
         L.or_growth_sweep.argtypes = [C.POINTER(SmxConfig), C.c_double, C.c_double, C.c_int, _dp]
         L.or_growth_sweep.restype = C.c_int
+        L.or_adjoint_grad.argtypes = [C.POINTER(SmxConfig), _dp, C.c_int64, C.c_double, _dp, _dp, _dp, _dp]
+        L.or_adjoint_grad.restype = C.c_int
         _lib = L
     return _lib
 
@@ -140,6 +142,26 @@ def growth_sweep(cfg, mq0, dmq, nq):
     rc = lib().or_growth_sweep(C.byref(cfg), mq0, dmq, nq, _p(out))
     assert rc == 0
     return out
+
+
+def adjoint_grad(cfg, psi0, T, amp=0.5):
+    """FP64 discrete adjoint: returns (J, dJ/dpsi0, dJ/deps, dJ/dq0)."""
+    psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+    d = np.zeros_like(psi0)
+    J, de, dq = C.c_double(), C.c_double(), C.c_double()
+    rc = lib().or_adjoint_grad(C.byref(cfg), _p(psi0), T, amp, C.byref(J), _p(d), C.byref(de), C.byref(dq))
+    assert rc == 0, rc
+    return J.value, d, de.value, dq.value
+
+
+def loss_only(cfg, psi0, T, amp=0.5):
+    """J = 1/2 sum (psi_T - amp cos(q0 z))^2 from a plain forward run (finite-difference checks)."""
+    o = Oracle(cfg)
+    o.set_psi(psi0)
+    o.step(T)
+    p = o.get_psi()
+    z = np.arange(cfg.nz) * cfg.lz / cfg.nz
+    return 0.5 * float(np.sum((p - amp * np.cos(cfg.q0 * z)) ** 2))
 
 
 class Oracle:

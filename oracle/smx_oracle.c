@@ -921,3 +921,107 @@ int or_growth_sweep(const smx_config *cfg, double mq0, double dmq, int nq, doubl
   or_destroy(s);
   return 0;
 }
+
+/* ======================================================================
+ * Discrete adjoint of the periodic CNAB2 step (no advection, no penalty).
+ *
+ * Forward map in real space (pf3d.cpp:471-481 with the bootstrap :363-374):
+ *   psi_{n+1} = A psi_n + B (c_n N(psi_n) - N(psi_{n-1})),  c_0 = 2 (N_{-1} := N_0 drops out), c_n = 3,
+ *   A = F^-1 diag(C1/C2) F,  B = F^-1 diag(dt/2 / C2) F  (symmetric: real even multipliers).
+ * Loss J = 1/2 sum_x (psi_T - g)^2, g = amp cos(q0 z), z = k dz.
+ * Adjoint a_n = dJ/dpsi_n:
+ *   a_T = psi_T - g,  a_n = A a_{n+1} + N'(psi_n) B (c_n a_{n+1} - a_{n+2}),  a_{T+1} = 0.
+ * Parameter gradients (theta = eps or q0, dL/deps = 1, dL/dq0 = 4 alpha q0 (k^2 - q0^2)):
+ *   dJ/dtheta = sum_n < a_{n+1}, F^-1[ dt/2 dL/dtheta (Psi_n + Psi_{n+1}) / C2 ] >  (+ target term for q0).
+ * ==================================================================== */
+static inline double nl_prime(const or_state *s, double p) {
+  if (s->c.nl_model == SMX_NL_CUBIC) return 2.0 * s->c.beta * p - 3.0 * s->c.gamma * p * p;
+  return 3.0 * s->c.beta * p * p - 5.0 * s->c.gamma * p * p * p * p;
+}
+
+/* out = F^-1[m(k) F(a)] with F normalised, m chosen by `which`: 0 -> C1/C2, 1 -> dt/2/C2. */
+static void apply_mult(or_state *s, const double *a, double *out, int which) {
+  cplx *q = (cplx *)malloc(sizeof(cplx) * s->Nh);
+  or_rfft3(s->nx, s->ny, s->nz, a, (double *)q);
+  const double dtd2 = 0.5 * s->c.dt;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nzh; ++k) {
+        double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+        double L = s->c.eps - op_sh(s, k2), C1 = 1.0 + dtd2 * L, C2 = 1.0 - dtd2 * L;
+        q[KIDX(i, j, k)] *= s->scale * (which == 0 ? C1 / C2 : dtd2 / C2);
+      }
+  or_irfft3(s->nx, s->ny, s->nz, (double *)q, out);
+  free(q);
+}
+
+int or_adjoint_grad(const smx_config *cfg, const double *psi0, int64_t T, double amp, double *loss, double *dpsi0,
+                    double *deps, double *dq0) {
+  if (cfg->bc != SMX_BC_PERIODIC || cfg->advect || cfg->w_mid != 0.0 || T < 1) return SMX_ERR_UNSUPPORTED;
+  or_state *s = or_create(cfg);
+  if (!s) return SMX_ERR_INVALID;
+  const int64_t N = s->N;
+  double *traj = (double *)malloc(sizeof(double) * N * (size_t)(T + 1));
+  or_set_psi(s, psi0);
+  memcpy(traj, s->psi, sizeof(double) * N);
+  for (int64_t n = 0; n < T; ++n) {
+    or_step(s, 1);
+    memcpy(traj + (n + 1) * N, s->psi, sizeof(double) * N);
+  }
+  double *a1 = (double *)calloc(N, sizeof(double)), *a2 = (double *)calloc(N, sizeof(double));
+  double *w = (double *)malloc(sizeof(double) * N), *t = (double *)malloc(sizeof(double) * N);
+  /* a_T and the loss */
+  double J = 0.0, tq = 0.0;
+  const double *pT = traj + T * N;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int64_t id = ((int64_t)i * s->ny + j) * s->nz + k;
+        double z = k * s->dz, g = amp * cos(s->c.q0 * z);
+        a1[id] = pT[id] - g;
+        J += 0.5 * a1[id] * a1[id];
+        tq += a1[id] * amp * z * sin(s->c.q0 * z); /* -dg/dq0 */
+      }
+  /* parameter terms: < a_{n+1}, F^-1[dt/2 dL (Psi_n + Psi_{n+1})/C2] > = N sum_k w Re(conj alpha_{n+1} ...) */
+  double ge = 0.0, gq = 0.0;
+  cplx *qa = (cplx *)malloc(sizeof(cplx) * s->Nh), *qp = (cplx *)malloc(sizeof(cplx) * s->Nh);
+  double *sum = (double *)malloc(sizeof(double) * N);
+  const double dtd2 = 0.5 * s->c.dt, q02 = s->c.q0 * s->c.q0;
+  for (int64_t n = T - 1; n >= 0; --n) {
+    /* a1 = a_{n+1}, a2 = a_{n+2} */
+    for (int64_t x = 0; x < N; ++x) sum[x] = traj[n * N + x] + traj[(n + 1) * N + x];
+    or_rfft3(s->nx, s->ny, s->nz, a1, (double *)qa);
+    or_rfft3(s->nx, s->ny, s->nz, sum, (double *)qp);
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nzh; ++k) {
+          int64_t id = KIDX(i, j, k);
+          double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+          double L = s->c.eps - op_sh(s, k2), C2 = 1.0 - dtd2 * L;
+          double wk = (k == 0 || 2 * k == s->nz) ? 1.0 : 2.0;
+          /* normalised spectra: alpha = s qa, Psi = s qp; inner product N sum w Re(conj alpha Psi) */
+          double re = creal(conj(qa[id]) * qp[id]) * s->scale * s->scale * (double)N;
+          double base = wk * re * dtd2 / C2;
+          ge += base;
+          gq += base * 4.0 * s->c.alpha * s->c.q0 * (k2 - q02);
+        }
+    const double c = (n >= 1) ? 3.0 : 2.0;
+    for (int64_t x = 0; x < N; ++x) t[x] = c * a1[x] - a2[x];
+    apply_mult(s, t, w, 1);
+    for (int64_t x = 0; x < N; ++x) w[x] *= nl_prime(s, traj[n * N + x]);
+    apply_mult(s, a1, t, 0);
+    for (int64_t x = 0; x < N; ++x) t[x] += w[x];
+    /* shift: a2 <- a1, a1 <- a_n */
+    double *tmp = a2;
+    a2 = a1;
+    a1 = tmp;
+    memcpy(a1, t, sizeof(double) * N);
+  }
+  memcpy(dpsi0, a1, sizeof(double) * N);
+  *loss = J;
+  *deps = ge;
+  *dq0 = gq + tq;
+  free(traj); free(a1); free(a2); free(w); free(t); free(qa); free(qp); free(sum);
+  or_destroy(s);
+  return 0;
+}

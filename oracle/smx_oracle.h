@@ -59,6 +59,11 @@ void or_ic_hyperboloid(int nx, int ny, int nz, double dx, double dy, double dz, 
  * psi0 = 1e-4 cos(mq*k*dz), for nq values mq = mq0 + i*dmq. */
 int or_growth_sweep(const smx_config *cfg, double mq0, double dmq, int nq, double *growth);
 
+/* Discrete adjoint of the periodic CNAB2 step (no advection/penalty):
+ * J = 1/2 sum (psi_T - amp cos(q0 z))^2 and dJ/dpsi0, dJ/deps, dJ/dq0. */
+int or_adjoint_grad(const smx_config *cfg, const double *psi0, int64_t T, double amp, double *loss, double *dpsi0,
+                    double *deps, double *dq0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,7 @@ EXPORTS = [
     "smx_config_default", "smx_create", "smx_destroy", "smx_last_error", "smx_get_config",
     "smx_set_psi", "smx_get_psi", "smx_step", "smx_l1", "smx_energy", "smx_dominant_wavenumber",
     "smx_dtpsi", "smx_surftrack", "smx_sync", "smx_step_count", "smx_profile_step",
-    "smx_calibration_copy",
+    "smx_calibration_copy", "smx_adjoint_grad",
 ]
 
 _lib = None
@@ -56,6 +56,7 @@ def lib():
     L.smx_profile_step.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.c_char_p,
                                    C.c_int]
     L.smx_calibration_copy.argtypes = [vp, C.c_int64, C.c_int, _dp]
+    L.smx_adjoint_grad.argtypes = [vp, _dp, C.c_int64, C.c_int64, C.c_int64, C.c_double, _dp, _dp, _dp, _dp]
     for n in EXPORTS:
         f = getattr(L, n)
         if n not in ("smx_config_default", "smx_destroy", "smx_last_error"):

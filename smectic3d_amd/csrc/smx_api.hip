@@ -520,6 +520,126 @@ static int profile_T(smx_handle* h, int64_t nsteps, std::vector<double>& acc) {
   return rc;
 }
 
+
+// ---- adjoint (periodic, pure SH) ----
+template <typename T>
+static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* loss, double* deps, double* dq0) {
+  const smx_config& c = h->cfg;
+  const int64_t nseg = (T_ + K - 1) / K;
+  const size_t sbytes = h->wbytes * (size_t)h->nspec;
+  std::vector<void*> tmp;
+  auto talloc = [&](void** pp, size_t bytes) -> int {
+    hipError_t e = hipMalloc(pp, bytes);
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + ") for the adjoint: " +
+                                                      hipGetErrorString(e));
+    tmp.push_back(*pp);
+    return SMX_OK;
+  };
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(h->st);
+    for (void* q : tmp) (void)hipFree(q);
+  };
+  char *ck = nullptr, *recpsi = nullptr, *recnp = nullptr;
+  void *AS = nullptr, *AC = nullptr;
+  double *ckmean = nullptr, *acc = nullptr;
+  const int nxb = c.ny / kNC, nblk = nxb * h->nplanes;
+  int rc = SMX_OK;
+  if ((rc = talloc((void**)&ck, 2 * sbytes * nseg)) || (rc = talloc((void**)&ckmean, sizeof(double) * 2 * nseg)) ||
+      (rc = talloc((void**)&recpsi, sbytes * (K + 1))) ||
+      (rc = talloc((void**)&recnp, sizeof(T) * (size_t)h->N * K)) || (rc = talloc(&AS, sbytes)) ||
+      (rc = talloc(&AC, sbytes)) || (rc = talloc((void**)&acc, sizeof(double) * 2 * nblk))) {
+    cleanup();
+    return rc;
+  }
+  PParams<T> p = params<T>(h, h->fp32);
+  auto save = [&](int64_t seg) -> int {
+    HIPCHK(hipMemcpyAsync(ck + 2 * sbytes * seg, h->A, sbytes, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(ck + 2 * sbytes * seg + sbytes, h->R, sbytes, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(ckmean + 2 * seg, h->mean_state, 2 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    return SMX_OK;
+  };
+  auto restore = [&](int64_t seg) -> int {
+    HIPCHK(hipMemcpyAsync(h->A, ck + 2 * sbytes * seg, sbytes, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->R, ck + 2 * sbytes * seg + sbytes, sbytes, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->mean_state, ckmean + 2 * seg, 2 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    return SMX_OK;
+  };
+  auto run = [&]() -> int {
+    // forward sweep with checkpoints
+    for (int64_t seg = 0; seg < nseg; ++seg) {
+      CHK(save(seg));
+      const int64_t len = std::min(K, T_ - seg * K);
+      for (int64_t j = 0; j < len; ++j) CHK(step_once<T>(h, p, nullptr));
+    }
+    // a_T = psi_T - g, the loss and the explicit q0 term of the target
+    CHK(c2r_state(h, h->real_io));
+    hipLaunchKernelGGL(k_adj_target, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->real_io, h->N, c.nz, h->dz,
+                       c.q0, amp, h->red);
+    HIPCHK(hipGetLastError());
+    std::vector<double> tsum;
+    CHK(host_sum(h, 2, kRedBlocks, tsum));
+    // alpha_T = F a_T; first adjoint stage w_{T-1} = dt/2/C2 c_{T-1} alpha_T
+    PParams<T> pa = p;
+    pa.partial = nullptr;
+    CHK(zpass<T>(h, pa, Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, AS, h->real_io, nullptr));
+    CHK(ypass<T>(h, AS, AS, -1));
+    HIPCHK(hipMemsetAsync(acc, 0, sizeof(double) * 2 * nblk, h->st));
+    HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, nullptr, nullptr, pa, (T)(T_ - 1 >= 1 ? 3 : 2), 1,
+                                    c.q0, c.alpha, acc, h->st));
+    CHK(herm<T>(h, pa, AC));
+    // reverse sweep over the segments, recomputing each one with recording
+    for (int64_t seg = nseg - 1; seg >= 0; --seg) {
+      const int64_t s0 = seg * K, len = std::min(K, T_ - s0);
+      CHK(restore(seg));
+      PParams<T> px = params<T>(h, false);
+      px.scale = (T)(1.0 / (double)c.nx);  // psi_hat_{s0} from the stage buffer
+      CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, recpsi, nullptr, nullptr));
+      for (int64_t j = 0; j < len; ++j) {
+        PParams<T> pr = p;
+        pr.rec_np = (T*)(recnp + sizeof(T) * (size_t)h->N * j);
+        pr.rec_psi = (cplx<T>*)(recpsi + sbytes * (j + 1));
+        CHK(step_once<T>(h, pr, nullptr));
+      }
+      for (int64_t j = len - 1; j >= 0; --j) {
+        const int64_t n = s0 + j;
+        CHK(ypass<T>(h, AS, AS, +1));
+        HIPCHK(AdjointOps<T>::zpass_adj((cplx<T>*)AS, (const T*)(recnp + sizeof(T) * (size_t)h->N * j), pa, h->st));
+        CHK(ypass<T>(h, AS, AS, -1));
+        HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, (const cplx<T>*)(recpsi + sbytes * j),
+                                        (const cplx<T>*)(recpsi + sbytes * (j + 1)), pa, (T)(n - 1 >= 1 ? 3 : 2), 0,
+                                        c.q0, c.alpha, acc, h->st));
+        CHK(herm<T>(h, pa, AC));
+      }
+    }
+    // dJ/dpsi0 = F^-1 alpha_0 (FP64 inverse chain)
+    const int64_t ncomp = 2 * h->nspec;
+    if (h->fp32) {
+      hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)AC, (double*)h->D1,
+                         ncomp);
+      HIPCHK(hipGetLastError());
+    } else {
+      HIPCHK(hipMemcpyAsync(h->D1, AC, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
+    }
+    CHK(c2r_spectrum_d1(h, h->real_io));
+    std::vector<double> ha(2 * (size_t)nblk);
+    HIPCHK(hipMemcpyAsync(ha.data(), acc, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    double se = 0, sq = 0;
+    for (int b = 0; b < nblk; ++b) {
+      se += ha[2 * b];
+      sq += ha[2 * b + 1];
+    }
+    *loss = tsum[0];
+    *deps = (double)h->N * se;  // Parseval: sum_x a b = N sum_k w Re(conj alpha beta)
+    *dq0 = (double)h->N * sq + tsum[1];
+    CHK(restore(0));
+    return SMX_OK;
+  };
+  rc = run();
+  cleanup();
+  return rc;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 extern "C" {
@@ -799,6 +919,26 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   HIPCHK(hipMemcpyAsync(&nf, h->flag, sizeof(nf), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   if (found) *found = nf;
+  return SMX_OK;
+}
+
+int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int64_t nsteps, int64_t ckpt_every,
+                     double target_amp, double* loss, double* dpsi0, double* deps, double* dq0) {
+  if (!h || !psi0 || !loss || !dpsi0 || !deps || !dq0) return fail(SMX_ERR_INVALID, "null argument");
+  const smx_config& c = h->cfg;
+  if (h->neumann || c.advect || c.w_mid != 0.0)
+    return fail(SMX_ERR_UNSUPPORTED, "the adjoint covers the periodic, non-advected, penalty-free step");
+  if (nsteps < 1 || ckpt_every < 1) return fail(SMX_ERR_INVALID, "nsteps and ckpt_every must be >= 1");
+  if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
+  CHK(smx_set_psi(h, psi0, row_stride));
+  const int64_t K = std::min(ckpt_every, nsteps);
+  CHK(h->fp32 ? adjoint_T<float>(h, nsteps, K, target_amp, loss, deps, dq0)
+              : adjoint_T<double>(h, nsteps, K, target_amp, loss, deps, dq0));
+  HIPCHK(hipMemcpy2DAsync(dpsi0, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz, sizeof(double) * c.nz,
+                          (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->steps = 0;
+  h->has_prev = false;
   return SMX_OK;
 }
 

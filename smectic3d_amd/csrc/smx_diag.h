@@ -199,6 +199,26 @@ __global__ void k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, 
     b[i] = a[i];
 }
 
+// a_T = psi_T - amp cos(q0 z) in place (FP64 real, dense [x][y][z]); per-block partials of
+// J = 1/2 sum a_T^2 and of the explicit target term sum a_T (-dg/dq0) = sum a_T amp z sin(q0 z).
+__global__ void k_adj_target(double* __restrict__ a, int64_t n, int nz, double dz, double q0, double amp,
+                             double* __restrict__ out) {
+  __shared__ double red[64];
+  Kahan sj, sq;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double z = (double)(int)(i % nz) * dz;
+    const double r = a[i] - amp * cos(q0 * z);
+    a[i] = r;
+    sj.add(0.5 * r * r);
+    sq.add(r * amp * z * sin(q0 * z));
+  }
+  const double t0 = block_sum(sj.s, red), t1 = block_sum(sq.s, red);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = t0;
+    out[2 * blockIdx.x + 1] = t1;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // surfTrack.  Fields are FP64 real, dense [x][y][z]; one thread per (i,j) column.
 // out: 4 arrays of nx*ny (surfz, vel, H, K) written only for columns that cross.

@@ -3,3 +3,4 @@
 template struct smx::PeriodicOps<float>;
 template struct smx::NeumannOps<float>;
 template struct smx::AdvectOps<float>;
+template struct smx::AdjointOps<float>;

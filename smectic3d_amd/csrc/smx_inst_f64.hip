@@ -3,3 +3,4 @@
 template struct smx::PeriodicOps<double>;
 template struct smx::NeumannOps<double>;
 template struct smx::AdvectOps<double>;
+template struct smx::AdjointOps<double>;

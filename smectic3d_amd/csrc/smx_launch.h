@@ -3,7 +3,7 @@
 #pragma once
 #include <algorithm>
 
-#include "smx_advect.h"
+#include "smx_adjoint.h"
 #include "smx_neumann.h"
 
 namespace smx {
@@ -180,6 +180,35 @@ constexpr int nc_fit(int ntiles) {
   return (ntiles * L * 17 * (int)sizeof(cplx<T>) <= 150 * 1024) ? 16
          : (ntiles * L * 9 * (int)sizeof(cplx<T>) <= 150 * 1024) ? 8
                                                                   : 4;
+}
+
+template <typename T>
+hipError_t AdjointOps<T>::xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T>* psin, const cplx<T>* psin1,
+                                    const PParams<T>& p, T cnext, int init, double q0d, double alphad, double* acc,
+                                    hipStream_t st) {
+  const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
+#define SMX_CASE(L)                                                                                      \
+  if (p.nx == L) {                                                                                       \
+    hipLaunchKernelGGL((k_xpass_adj<T, L, kNC>), grid, dim3(kNC * (L / Plan<L>::E)), 0, st, stage, carry, psin, \
+                       psin1, p, cnext, init, q0d, alphad, acc);                                         \
+    return hipGetLastError();                                                                            \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t AdjointOps<T>::zpass_adj(cplx<T>* stage, const T* nprime, const PParams<T>& p, hipStream_t st) {
+  const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
+#define SMX_CASE(L)                                                                                      \
+  if (p.nz == L) {                                                                                       \
+    hipLaunchKernelGGL((k_zpass_adj<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, stage, nprime, p); \
+    return hipGetLastError();                                                                            \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
 }
 
 template <typename T>

@@ -37,7 +37,19 @@ struct PParams {
   int npartial;
   double* mean_state;  // [0] = R(k=0) in FP64, [1] = psi_hat(0) in FP64
   double m_scale, m_dtd2, m_c1, m_c2;  // FP64 scale, dt/2, C1(0), C2(0) for the mean mode
+  // Recording for the adjoint sweep (nullptr: off): N'(psi_n) by the z pass, layout
+  // [z][x][y]; psi_hat_{n+1} by the x pass, layout of R.
+  T* rec_np = nullptr;
+  cplx<T>* rec_psi = nullptr;
 };
+
+// dN/dpsi: 3 beta psi^2 - 5 gamma psi^4 (quintic) or 2 beta psi - 3 gamma psi^2 (cubic).
+template <typename T>
+__device__ __forceinline__ T nonlin_prime(const PParams<T>& p, T s) {
+  if (p.nl_model == 1) return s * ((T)2 * p.beta - (T)3 * p.gamma * s);
+  const T s2 = s * s;
+  return s2 * ((T)3 * p.beta - (T)5 * p.gamma * s2);
+}
 
 // Local nonlinearity: beta psi^3 - gamma psi^5 (pf3d.cpp:460-461) or the cubic model.
 template <typename T>
@@ -157,6 +169,7 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
             rn = mk((T)p.mean_state[0], (T)0);
           }
           rbuf[idx] = rn;
+          if (p.rec_psi != nullptr) p.rec_psi[idx] = ps;
           v[e] = ps;
         } else {  // X_BOOT: R = C1 psi0 - dt/2 N_past with N_past = N(psi0) (pf3d.cpp:363-374)
           cplx<T> ps = psi0[idx];
@@ -291,6 +304,15 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cpl
       __syncthreads();
       Tile<T, L, E, FP>::template store<RP>(v, F, q, tc);
       __syncthreads();
+    }
+    if (p.rec_np != nullptr) {  // N'(psi_n) for the adjoint sweep
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
+        T* row = p.rec_np + (int64_t)n * p.nxy + f0;
+        if (q < nvalid) row[q] = nonlin_prime(p, v[e].x);
+        if (q + NPR < nvalid) row[q + NPR] = nonlin_prime(p, v[e].y);
+      }
     }
     double acc = 0.0;
 #pragma unroll

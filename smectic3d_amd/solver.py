@@ -110,6 +110,16 @@ class Solver:
         k = n.value
         return list(zip(names.value.decode().split("\n")[:k], ms[:k], by[:k]))
 
+    def adjoint_grad(self, psi0, nsteps, ckpt_every=50, target_amp=0.5):
+        """(J, dJ/dpsi0, dJ/deps, dJ/dq0) for J = 1/2 sum (psi_T - target_amp cos(q0 z))^2.
+        The handle is left at psi0."""
+        psi0 = np.ascontiguousarray(psi0, dtype=np.float64).reshape(self.shape)
+        d = np.empty(self.shape)
+        J, de, dq = C.c_double(), C.c_double(), C.c_double()
+        check(lib().smx_adjoint_grad(self._h, _ptr(psi0), self.cfg.nz, int(nsteps), int(ckpt_every),
+                                     float(target_amp), C.byref(J), _ptr(d), C.byref(de), C.byref(dq)))
+        return J.value, d, de.value, dq.value
+
     def calibration_copy(self, nbytes, reps=20):
         v = C.c_double()
         check(lib().smx_calibration_copy(self._h, int(nbytes), int(reps), C.byref(v)))

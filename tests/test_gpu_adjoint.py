@@ -1,0 +1,118 @@
+"""GPU adjoint of the periodic step (smx_adjoint_grad) against the FP64 oracle's
+discrete adjoint and against central finite differences of the GPU forward."""
+import copy
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from smectic3d_amd import Solver, SmxError
+from smectic3d_amd import _abi as A
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def base_cfg(shape, prec=A.SMX_FP64, **kw):
+    d = dict(nw=8, eps=0.4, q0=1.0, beta=0.3, gamma=1.0, dt=0.05, nl_model=A.SMX_NL_CUBIC)
+    d.update(kw)
+    return A.make_config(*shape, precision=prec, **d)
+
+
+@pytest.mark.parametrize("shape,T,K", [((16, 16, 16), 12, 5), ((32, 16, 24), 20, 20), ((24, 32, 32), 9, 4)])
+def test_adjoint_matches_oracle(shape, T, K):
+    say("adjoint vs oracle", shape, T, K)
+    cfg = base_cfg(shape)
+    psi = O.ic_noise(*shape, 3, 0.3)
+    J0, d0, e0, q0 = O.adjoint_grad(cfg, psi, T)
+    with Solver(cfg) as s:
+        J, d, e, q = s.adjoint_grad(psi, T, K)
+    say("  J", J, J0, "deps", e, e0, "dq0", q, q0)
+    assert abs(J - J0) < 1e-12 * abs(J0)
+    assert abs(e - e0) < 1e-9 * max(1, abs(e0)) and abs(q - q0) < 1e-9 * max(1, abs(q0))
+    assert np.linalg.norm(d - d0) < 1e-9 * np.linalg.norm(d0)
+
+
+def test_adjoint_quintic_matches_oracle():
+    shape, T = (16, 16, 32), 15
+    cfg = base_cfg(shape, eps=-0.2, beta=1.0, dt=0.01, nl_model=A.SMX_NL_QUINTIC)
+    psi = O.ic_noise(*shape, 5, 0.6)
+    J0, d0, e0, q0 = O.adjoint_grad(cfg, psi, T)
+    with Solver(cfg) as s:
+        J, d, e, q = s.adjoint_grad(psi, T, 4)
+    assert abs(e - e0) < 1e-9 * max(1, abs(e0)) and abs(q - q0) < 1e-9 * max(1, abs(q0))
+    assert np.linalg.norm(d - d0) < 1e-9 * np.linalg.norm(d0)
+
+
+def gpu_loss(cfg, psi, T, amp=0.5):
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        s.step(T)
+        p = s.get_psi()
+    z = np.arange(cfg.nz) * cfg.lz / cfg.nz
+    return 0.5 * float(np.sum((p - amp * np.cos(cfg.q0 * z)) ** 2))
+
+
+def test_adjoint_central_differences_64():
+    shape, T, h = (64, 64, 64), 100, 1e-5
+    say("adjoint FD 64^3 x 100")
+    cfg = base_cfg(shape, beta=0.0)
+    psi = O.ic_noise(*shape, 7, 0.05)
+    with Solver(cfg) as s:
+        J, d, de, dq = s.adjoint_grad(psi, T, 25)
+
+    def with_(**kw):
+        c = copy.copy(cfg)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+    fe = (gpu_loss(with_(eps=cfg.eps + h), psi, T) - gpu_loss(with_(eps=cfg.eps - h), psi, T)) / (2 * h)
+    fq = (gpu_loss(with_(q0=cfg.q0 + h), psi, T) - gpu_loss(with_(q0=cfg.q0 - h), psi, T)) / (2 * h)
+    v = O.ic_noise(*shape, 11, 1.0)
+    fv = (gpu_loss(cfg, psi + h * v, T) - gpu_loss(cfg, psi - h * v, T)) / (2 * h)
+    say("  deps", de, fe, "dq0", dq, fq, "dpsi.v", float(np.sum(d * v)), fv)
+    assert abs(de - fe) < 1e-6 * max(1, abs(fe))
+    assert abs(dq - fq) < 1e-6 * max(1, abs(fq))
+    assert abs(np.sum(d * v) - fv) < 1e-6 * max(1, abs(fv))
+
+
+def test_adjoint_fp32_mixed_close_to_fp64():
+    shape, T = (32, 32, 32), 40
+    psi = O.ic_noise(*shape, 3, 0.1)
+    with Solver(base_cfg(shape)) as s:
+        J, d, e, q = s.adjoint_grad(psi, T, 10)
+    with Solver(base_cfg(shape, A.SMX_FP32_MIXED)) as s:
+        J2, d2, e2, q2 = s.adjoint_grad(psi, T, 10)
+    say("  fp32 adjoint rel err", abs(e2 - e) / abs(e), abs(q2 - q) / abs(q), np.linalg.norm(d2 - d) / np.linalg.norm(d))
+    assert abs(e2 - e) < 1e-4 * abs(e) and abs(q2 - q) < 1e-4 * abs(q)
+    assert np.linalg.norm(d2 - d) < 1e-4 * np.linalg.norm(d)
+
+
+def test_autograd_function():
+    torch = pytest.importorskip("torch")
+    from smectic3d_amd.autograd import SHLoss
+    shape, T = (16, 16, 16), 10
+    cfg = base_cfg(shape)
+    psi = torch.tensor(O.ic_noise(*shape, 3, 0.3), requires_grad=True)
+    eps = torch.tensor(0.4, dtype=torch.float64, requires_grad=True)
+    q0 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    J = SHLoss.apply(psi, eps, q0, cfg, T, 4, 0.5)
+    J.backward()
+    J0, d0, e0, q00 = O.adjoint_grad(cfg, psi.detach().numpy(), T)
+    assert abs(J.item() - J0) < 1e-12 * J0
+    assert abs(eps.grad.item() - e0) < 1e-9 * max(1, abs(e0))
+    assert abs(q0.grad.item() - q00) < 1e-9 * max(1, abs(q00))
+    assert np.linalg.norm(psi.grad.numpy() - d0) < 1e-9 * np.linalg.norm(d0)
+
+
+def test_adjoint_rejects_unsupported():
+    cfg = base_cfg((16, 16, 16), advect=1)
+    with Solver(cfg) as s:
+        with pytest.raises(SmxError):
+            s.adjoint_grad(O.ic_noise(16, 16, 16, 1, 0.1), 5, 2)
