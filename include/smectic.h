@@ -156,6 +156,24 @@ int smx_surftrack(smx_handle *h, const smx_surf_opts *opts, double *surfz, doubl
 int smx_adjoint_grad(smx_handle *h, const double *psi0, int64_t row_stride, int64_t nsteps, int64_t ckpt_every,
                      double target_amp, double *loss, double *dpsi0, double *deps, double *dq0);
 
+/*
+ * Ensembles of independent boxes of uneven sizes (parameter sweeps).  Each box is
+ * a full solver (its own config and buffers, on its own stream); smx_batch_run
+ * advances every box by nsteps through one captured hipGraph in which the boxes'
+ * step chains are parallel branches (fork/join on the batch stream).  No
+ * counterpart in the reference (it runs one box per MPI job).
+ */
+typedef struct smx_batch smx_batch;
+int smx_batch_create(int nbox, const smx_config *cfgs, smx_batch **out);
+void smx_batch_destroy(smx_batch *b);
+/* Per-box handle for set/get/diagnostics; owned by the batch. */
+int smx_batch_handle(smx_batch *b, int box, smx_handle **h);
+/* Advance every box by nsteps: graph launches of `graph_steps` steps each
+ * (captured once per batch) plus a direct remainder. */
+int smx_batch_run(smx_batch *b, int64_t nsteps, int64_t graph_steps);
+/* Number of kernel nodes in the captured graph (0 before the first run). */
+int smx_batch_graph_nodes(smx_batch *b, int64_t *nodes);
+
 /* Block until the handle's stream is idle. */
 int smx_sync(smx_handle *h);
 

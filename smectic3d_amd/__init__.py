@@ -7,4 +7,4 @@ loop over that ABI.
 """
 from ._abi import *  # noqa: F401,F403
 from ._lib import EXPORTS, SmxError, lib  # noqa: F401
-from .solver import Solver  # noqa: F401
+from .solver import Batch, Solver  # noqa: F401

@@ -15,7 +15,8 @@ EXPORTS = [
     "smx_config_default", "smx_create", "smx_destroy", "smx_last_error", "smx_get_config",
     "smx_set_psi", "smx_get_psi", "smx_step", "smx_l1", "smx_energy", "smx_dominant_wavenumber",
     "smx_dtpsi", "smx_surftrack", "smx_sync", "smx_step_count", "smx_profile_step",
-    "smx_calibration_copy", "smx_adjoint_grad",
+    "smx_calibration_copy", "smx_adjoint_grad", "smx_batch_create", "smx_batch_destroy",
+    "smx_batch_handle", "smx_batch_run", "smx_batch_graph_nodes",
 ]
 
 _lib = None
@@ -57,9 +58,15 @@ def lib():
                                    C.c_int]
     L.smx_calibration_copy.argtypes = [vp, C.c_int64, C.c_int, _dp]
     L.smx_adjoint_grad.argtypes = [vp, _dp, C.c_int64, C.c_int64, C.c_int64, C.c_double, _dp, _dp, _dp, _dp]
+    L.smx_batch_create.argtypes = [C.c_int, C.POINTER(SmxConfig), C.POINTER(vp)]
+    L.smx_batch_destroy.argtypes = [vp]
+    L.smx_batch_destroy.restype = None
+    L.smx_batch_handle.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.smx_batch_run.argtypes = [vp, C.c_int64, C.c_int64]
+    L.smx_batch_graph_nodes.argtypes = [vp, C.POINTER(C.c_int64)]
     for n in EXPORTS:
         f = getattr(L, n)
-        if n not in ("smx_config_default", "smx_destroy", "smx_last_error"):
+        if n not in ("smx_config_default", "smx_destroy", "smx_last_error", "smx_batch_destroy"):
             f.restype = C.c_int
     _lib = L
     return L

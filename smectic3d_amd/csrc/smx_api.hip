@@ -640,6 +640,64 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
   return rc;
 }
 
+
+// ---- batch of independent boxes in one captured graph ----
+struct smx_batch {
+  std::vector<smx_handle*> boxes;
+  hipStream_t st = nullptr;  // capture / launch stream
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t graph_steps = 0;
+};
+
+static int box_steps(smx_handle* h, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (h->fp32) {
+      PParams<float> p = params<float>(h, true);
+      CHK(step_once<float>(h, p, nullptr));
+    } else {
+      PParams<double> p = params<double>(h, false);
+      CHK(step_once<double>(h, p, nullptr));
+    }
+  }
+  return SMX_OK;
+}
+
+static int batch_capture(smx_batch* b, int64_t gsteps) {
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  int rc = SMX_OK;
+  hipError_t e = hipEventRecord(b->fork, b->st);
+  for (size_t i = 0; i < b->boxes.size() && rc == SMX_OK && e == hipSuccess; ++i) {
+    smx_handle* h = b->boxes[i];
+    e = hipStreamWaitEvent(h->st, b->fork, 0);
+    if (e == hipSuccess) rc = box_steps(h, gsteps);
+    if (rc == SMX_OK && e == hipSuccess) e = hipEventRecord(b->joins[i], h->st);
+    if (rc == SMX_OK && e == hipSuccess) e = hipStreamWaitEvent(b->st, b->joins[i], 0);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->st, &g);
+  if (rc != SMX_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(SMX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_steps = gsteps;
+  return SMX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 extern "C" {
@@ -939,6 +997,78 @@ int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int6
   HIPCHK(hipStreamSynchronize(h->st));
   h->steps = 0;
   h->has_prev = false;
+  return SMX_OK;
+}
+
+
+int smx_batch_create(int nbox, const smx_config* cfgs, smx_batch** out) {
+  if (!cfgs || !out || nbox < 1) return fail(SMX_ERR_INVALID, "bad argument");
+  *out = nullptr;
+  smx_batch* b = new smx_batch();
+  auto bail = [&](int code) {
+    smx_batch_destroy(b);
+    return code;
+  };
+  for (int i = 0; i < nbox; ++i) {
+    if (cfgs[i].device != cfgs[0].device) return bail(fail(SMX_ERR_INVALID, "all boxes of a batch share one device"));
+    smx_handle* h = nullptr;
+    int rc = smx_create(&cfgs[i], &h);
+    if (rc) return bail(rc);
+    b->boxes.push_back(h);
+  }
+  hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  b->joins.assign(nbox, nullptr);
+  for (int i = 0; i < nbox && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming);
+  if (e != hipSuccess) return bail(fail(SMX_ERR_HIP, std::string("batch streams: ") + hipGetErrorString(e)));
+  *out = b;
+  return SMX_OK;
+}
+
+void smx_batch_destroy(smx_batch* b) {
+  if (!b) return;
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto e : b->joins)
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  for (smx_handle* h : b->boxes) smx_destroy(h);
+  delete b;
+}
+
+int smx_batch_handle(smx_batch* b, int box, smx_handle** h) {
+  if (!b || !h || box < 0 || box >= (int)b->boxes.size()) return fail(SMX_ERR_INVALID, "bad box index");
+  *h = b->boxes[box];
+  return SMX_OK;
+}
+
+int smx_batch_run(smx_batch* b, int64_t nsteps, int64_t graph_steps) {
+  if (!b || nsteps < 0 || graph_steps < 1) return fail(SMX_ERR_INVALID, "bad argument");
+  for (smx_handle* h : b->boxes) {
+    if (!h->has_state) return fail(SMX_ERR_STATE, "every box needs smx_set_psi before smx_batch_run");
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  HIPCHK(hipSetDevice(b->boxes[0]->cfg.device));
+  const int64_t ng = nsteps / graph_steps, rem = nsteps - ng * graph_steps;
+  if (ng > 0 && (!b->exec || b->graph_steps != graph_steps)) CHK(batch_capture(b, graph_steps));
+  for (int64_t g = 0; g < ng; ++g) HIPCHK(hipGraphLaunch(b->exec, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (smx_handle* h : b->boxes) {
+    if (rem > 0) CHK(box_steps(h, rem));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->steps += nsteps;
+    h->has_prev = false;  // no L1 snapshot inside a batch run
+  }
+  return SMX_OK;
+}
+
+int smx_batch_graph_nodes(smx_batch* b, int64_t* nodes) {
+  if (!b || !nodes) return fail(SMX_ERR_INVALID, "null argument");
+  size_t n = 0;
+  if (b->graph) HIPCHK(hipGraphGetNodes(b->graph, nullptr, &n));
+  *nodes = (int64_t)n;
   return SMX_OK;
 }
 

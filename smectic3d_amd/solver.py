@@ -124,3 +124,47 @@ class Solver:
         v = C.c_double()
         check(lib().smx_calibration_copy(self._h, int(nbytes), int(reps), C.byref(v)))
         return v.value
+
+
+class Batch:
+    """Independent boxes of uneven sizes advanced together through one captured hipGraph."""
+
+    def __init__(self, cfgs):
+        arr = (_abi.SmxConfig * len(cfgs))(*cfgs)
+        self._b = C.c_void_p()
+        check(lib().smx_batch_create(len(cfgs), arr, C.byref(self._b)))
+        self.cfgs = list(cfgs)
+        self.boxes = []
+        for i, c in enumerate(cfgs):
+            h = C.c_void_p()
+            check(lib().smx_batch_handle(self._b, i, C.byref(h)))
+            s = Solver.__new__(Solver)
+            s.cfg = c
+            s._h = h
+            s.shape = (c.nx, c.ny, c.nz)
+            s.close = lambda: None  # owned by the batch
+            self.boxes.append(s)
+
+    def run(self, nsteps, graph_steps=10):
+        check(lib().smx_batch_run(self._b, int(nsteps), int(graph_steps)))
+
+    def graph_nodes(self):
+        n = C.c_int64()
+        check(lib().smx_batch_graph_nodes(self._b, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            for s in self.boxes:
+                s._h = C.c_void_p()
+            lib().smx_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
