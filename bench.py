@@ -77,6 +77,111 @@ def run_path(precision, psi0, steps, warmup, prof_steps):
     return out
 
 
+def run_neumann(steps, surf_every=20):
+    """cosNoAdvMorph DCT/Neumann + surfTrack: 256x256x384, FP32 bulk with FP64 reductions,
+    eps=0.5, q0=1, gamma=1, dt=0.05, hyperboloid IC; surfTrack every 20 steps."""
+    from smectic3d_amd import Solver
+    from smectic3d_amd import _abi as A
+    nx, ny, nz = 256, 256, 384
+    dx = 2 * math.pi / 8
+    cfg = A.make_config(nx, ny, nz, lx=nx * dx, ly=ny * dx, lz=nz * dx, bc=A.SMX_BC_NEUMANN, eps=0.5, q0=1.0,
+                        beta=0.0, gamma=1.0, dt=0.05, nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED)
+    c = (np.arange(nx) - nx // 2) * dx
+    cz = (np.arange(nz) - nz // 2) * dx
+    r2 = np.maximum(c[:, None, None] ** 2 + c[None, :, None] ** 2 - cz[None, None, :] ** 2, 0.0)
+    psi0 = 0.4 * np.cos(np.sqrt(r2 + 4.0))
+    with Solver(cfg) as s:
+        s.set_psi(psi0)
+        s.step(surf_every)
+        s.sync()
+        t0 = time.perf_counter()
+        found = 0
+        for _ in range(steps // surf_every):
+            s.step(surf_every)
+            _, f = s.surftrack(0.7, -1.0)
+            found += f
+        s.sync()
+        t1 = time.perf_counter()
+        ks = s.profile_step(20)
+    return {"steps": steps, "surftrack_every": surf_every, "steps_per_s_with_surftrack": steps / (t1 - t0),
+            "columns_found_per_call": found / max(1, steps // surf_every),
+            "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
+
+
+def run_advected(steps):
+    """Advection-coupled periodic 256^3: mu grad psi force, spectral Stokes (eta=1), eps=0.5, q0=1,
+    gamma=1, dt=0.02, psi0 ~ U[-0.05, 0.05]; FP32-mixed."""
+    from smectic3d_amd import Solver
+    from smectic3d_amd import _abi as A
+    n = 256
+    L = n / 8 * 2 * math.pi
+    cfg = A.make_config(n, lx=L, ly=L, lz=L, eps=0.5, q0=1.0, beta=0.0, gamma=1.0, dt=0.02,
+                        nl_model=A.SMX_NL_CUBIC, advect=1, eta=1.0, rho=1.0, force_form=A.SMX_FORCE_MU,
+                        precision=A.SMX_FP32_MIXED)
+    rng = np.random.default_rng(4)
+    with Solver(cfg) as s:
+        s.set_psi(rng.uniform(-0.05, 0.05, (n, n, n)))
+        s.step(5)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t1 = time.perf_counter()
+        e = s.energy()
+    return {"steps": steps, "steps_per_s": steps / (t1 - t0), "energy": e}
+
+
+def run_batch(steps):
+    """24 independent periodic boxes, sizes from {64^3, 96^3, 128^3} by seed, eps ~ U[0.2,0.6],
+    q0 ~ U[0.8,1.2], gamma=1, dt=0.05, noise 0.05, advanced in one captured hipGraph (FP32-mixed)."""
+    from smectic3d_amd import Batch
+    from smectic3d_amd import _abi as A
+    rng = np.random.default_rng(2024)
+    cfgs = []
+    for _ in range(24):
+        n = int(rng.choice([64, 96, 128]))
+        L = n / 8 * 2 * math.pi
+        cfgs.append(A.make_config(n, lx=L, ly=L, lz=L, eps=float(rng.uniform(0.2, 0.6)),
+                                  q0=float(rng.uniform(0.8, 1.2)), beta=0.0, gamma=1.0, dt=0.05,
+                                  nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED))
+    pts = sum(c.nx * c.ny * c.nz for c in cfgs)
+    with Batch(cfgs) as b:
+        for i, (s, c) in enumerate(zip(b.boxes, cfgs)):
+            s.set_psi(np.random.default_rng(i).uniform(-0.05, 0.05, (c.nx, c.ny, c.nz)))
+        b.run(10, graph_steps=10)
+        t0 = time.perf_counter()
+        b.run(steps, graph_steps=10)
+        t1 = time.perf_counter()
+        nodes = b.graph_nodes()
+    return {"boxes": 24, "sizes": sorted(c.nx for c in cfgs), "steps": steps,
+            "batch_steps_per_s": steps / (t1 - t0), "box_steps_per_s": 24 * steps / (t1 - t0),
+            "points_per_s": pts * steps / (t1 - t0), "graph_nodes": nodes}
+
+
+def run_adjoint(steps, ckpt):
+    """Gradient of 1/2|psi_T - 0.5 cos(q0 z)|^2 w.r.t. psi0, eps, q0 at 256^3 (checkpoint every 50),
+    against the cost of a forward solve of the same length (FP32-mixed)."""
+    from smectic3d_amd import Solver
+    from smectic3d_amd import _abi as A
+    n = 256
+    L = n / 8 * 2 * math.pi
+    cfg = A.make_config(n, lx=L, ly=L, lz=L, eps=0.3, q0=1.0, beta=0.0, gamma=1.0, dt=0.05,
+                        nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED)
+    psi0 = np.random.default_rng(9).uniform(-0.05, 0.05, (n, n, n))
+    with Solver(cfg) as s:
+        s.set_psi(psi0)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        J, d, de, dq = s.adjoint_grad(psi0, steps, ckpt)
+        tg = time.perf_counter() - t0
+    return {"steps": steps, "ckpt_every": ckpt, "grads_per_s": 1.0 / tg, "forward_s": tf, "gradient_s": tg,
+            "gradient_over_forward": tg / tf, "loss": J, "dJ_deps": de, "dJ_dq0": dq}
+
+
 def cpu_baseline(psi0, budget_s=12.0):
     """The FP64 oracle (a C port of the reference loop, OpenMP threads) timed on the host,
     on a bounded sample of the same workload: whole steps at 256^3 until ~budget_s."""
@@ -104,6 +209,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the Neumann/advected/batch/adjoint legs")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
@@ -164,6 +270,15 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,
     }
+    if not a.no_extra:
+        log("neumann + surftrack ...")
+        line["neumann_surftrack_256x256x384_fp32"] = run_neumann(4000)
+        log("advected ...")
+        line["advected_256_fp32"] = run_advected(1000)
+        log("batch ...")
+        line["batch_24_boxes_fp32"] = run_batch(500)
+        log("adjoint ...")
+        line["adjoint_256_fp32"] = run_adjoint(2000, 50)
     if not a.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(psi0)
