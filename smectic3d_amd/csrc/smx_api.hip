@@ -259,18 +259,18 @@ static int bootstrap(smx_handle* h) {
   } else {
     CHK(xpass<T>(h, p, X_BOOT, h->A, h->A, h->R, h->P));
   }
-  CHK(herm<T>(h, p, h->R));
+  if (h->cfg.advect) CHK(herm<T>(h, p, h->R));  // the plain x pass projects R itself
   return SMX_OK;
 }
 
 // Kernel slots of one step, in launch order.
-static const char* kSlotNamesP[] = {"k_ypass_inv", "k_zpass_nl", "k_ypass_fwd", "k_xpass_step", "k_herm_project"};
+static const char* kSlotNamesP[] = {"k_ypass_inv", "k_zpass_nl", "k_ypass_fwd", "k_xpass_step"};
 static const char* kSlotNamesN[] = {"k_dct_ypass_inv", "k_dct_zpass_nl", "k_dct_ypass_fwd", "k_dct_xpass_step"};
 static const char* kSlotNamesA[] = {"k_ypass_inv_dy", "k_ypass_inv_psi", "k_ypass_inv_dx", "k_ypass_inv_sh",
                                     "k_zpass_force", "k_ypass_fwd_s0", "k_ypass_fwd_s1", "k_ypass_fwd_s2",
                                     "k_xpass_stokes", "k_ypass_inv_u0", "k_ypass_inv_u1", "k_ypass_inv_u2",
                                     "k_zpass_advnl", "k_ypass_fwd_n", "k_xpass_adv", "k_herm_project"};
-static int num_slots(const smx_handle* h) { return h->neumann ? 4 : (h->cfg.advect ? 16 : 5); }
+static int num_slots(const smx_handle* h) { return h->cfg.advect ? 16 : 4; }
 static const char* slot_name(const smx_handle* h, int s) {
   return h->neumann ? kSlotNamesN[s] : (h->cfg.advect ? kSlotNamesA[s] : kSlotNamesP[s]);
 }
@@ -343,11 +343,7 @@ static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
   if (ev) HIPCHK(hipEventRecord(ev[3], h->st));
   CHK(xpass<T>(h, p, X_STEP, h->A, h->A, h->R, nullptr));
   if (ev) HIPCHK(hipEventRecord(ev[4], h->st));
-  if (!h->neumann) {
-    CHK(herm<T>(h, p, h->R));
-    if (ev) HIPCHK(hipEventRecord(ev[5], h->st));
-  }
-  return SMX_OK;
+  return SMX_OK;  // the x pass projects the self-conjugate planes of R itself
 }
 
 // ---- FP64 diagnostics on a copy of the state ----
@@ -1088,7 +1084,7 @@ int smx_profile_step(smx_handle* h, int64_t nsteps, int max_slots, int* nslots, 
     bytes = {2 * sc, 2 * sc, 2 * sc, 2 * sc, 7 * sc, 2 * sc, 2 * sc, 2 * sc,
              6 * sc, 2 * sc, 2 * sc, 2 * sc, 7 * sc, 2 * sc, 6 * sc, 2 * 2 * plane};
   else
-    bytes = {2 * sc, 2 * sc, 2 * sc, 4 * sc, 2 * 2 * plane};
+    bytes = {2 * sc, 2 * sc, 2 * sc, 4 * sc};
   *nslots = ns;
   std::string nm;
   for (int s = 0; s < ns; ++s) {

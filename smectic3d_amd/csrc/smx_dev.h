@@ -141,9 +141,26 @@ SMX_PLAN(512, 16, 16 SMX_COMMA 16 SMX_COMMA 2, 2 SMX_COMMA 16 SMX_COMMA 16)
 #undef SMX_PLAN
 
 // ---------------------------------------------------------------------------
-// Block FFT over an LDS tile [pos][col] with row pitch NCP (>= number of columns).
-template <typename T, int L, int E, int NCP>
-struct Tile {
+// LDS addressing of an exchange tile of NC columns of length L.
+//   RowAddr<NCP>: [pos][col] with row pitch NCP (lanes run across columns).
+//   ColAddr<L>:   [col][pos] with one pad element per 16 positions (lanes run along
+//                 a column; the stride-R writes of the first Stockham pass stay
+//                 conflict-free).
+template <int NCP>
+struct RowAddr {
+  static __device__ __forceinline__ int at(int pos, int col) { return pos * NCP + col; }
+};
+template <int L>
+struct ColAddr {
+  static constexpr int LP = L + L / 16 + 1;
+  static __device__ __forceinline__ int at(int pos, int col) { return col * LP + pos + (pos >> 4); }
+};
+template <typename X> struct AddrOf { using type = X; };
+template <int L, int NCP> struct AddrSel { using type = RowAddr<NCP>; };
+
+// Block FFT over an LDS tile.  AD: addressing policy (RowAddr / ColAddr).
+template <typename T, int L, int E, typename AD>
+struct TileA {
   static constexpr int TPC = L / E;
   // element e of a thread for a pass of radix R: butterfly u = e / R, leg r = e % R
   template <int R> __device__ static __forceinline__ int pos_in(int tc, int e) {
@@ -151,15 +168,16 @@ struct Tile {
   }
   template <int R> __device__ static __forceinline__ void load(cplx<T>* v, const cplx<T>* lds, int col, int tc) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[pos_in<R>(tc, e) * NCP + col];
+    for (int e = 0; e < E; ++e) v[e] = lds[AD::at(pos_in<R>(tc, e), col)];
   }
   template <int R> __device__ static __forceinline__ void store(const cplx<T>* v, cplx<T>* lds, int col, int tc) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) lds[pos_in<R>(tc, e) * NCP + col] = v[e];
+    for (int e = 0; e < E; ++e) lds[AD::at(pos_in<R>(tc, e), col)] = v[e];
   }
 };
+template <typename T, int L, int E, int NCP> using Tile = TileA<T, L, E, RowAddr<NCP>>;
 
-template <typename T, int L, int E, int NCP, int SIGN, int Ns, int R, int... Rest>
+template <typename T, int L, int E, typename AD, int SIGN, int Ns, int R, int... Rest>
 __device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, int tc, const cplx<T>* __restrict__ tw) {
   constexpr int TPC = L / E;
   static_assert(E % R == 0, "E must be a multiple of every radix");
@@ -185,30 +203,33 @@ __device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, in
       const int k = b % Ns;
       const int base = (b - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[(base + r * Ns) * NCP + col] = v[u * R + r];
+      for (int r = 0; r < R; ++r) lds[AD::at(base + r * Ns, col)] = v[u * R + r];
     }
     __syncthreads();
     constexpr int Rn = First<Rest...>::value;
-    Tile<T, L, E, NCP>::template load<Rn>(v, lds, col, tc);
-    fft_passes<T, L, E, NCP, SIGN, Ns * R, Rest...>(v, lds, col, tc, tw);
+    TileA<T, L, E, AD>::template load<Rn>(v, lds, col, tc);
+    fft_passes<T, L, E, AD, SIGN, Ns * R, Rest...>(v, lds, col, tc, tw);
   }
 }
 
-template <typename T, int L, int NCP, int SIGN, typename Rad> struct BlockFFT;
-template <typename T, int L, int NCP, int SIGN, int... Rs>
-struct BlockFFT<T, L, NCP, SIGN, Radices<Rs...>> {
+template <typename T, int L, typename AD, int SIGN, typename Rad> struct BlockFFTA;
+template <typename T, int L, typename AD, int SIGN, int... Rs>
+struct BlockFFTA<T, L, AD, SIGN, Radices<Rs...>> {
   static constexpr int E = Plan<L>::E;
   // v: first-pass input positions on entry; last-pass output positions on exit.
   // The LDS tile is used for the exchanges; all threads must have finished any
   // earlier reads of it (the first exchange starts with a barrier).
   __device__ static __forceinline__ void run(cplx<T>* v, cplx<T>* lds, int col, int tc, const cplx<T>* __restrict__ tw) {
-    fft_passes<T, L, E, NCP, SIGN, 1, Rs...>(v, lds, col, tc, tw);
+    fft_passes<T, L, E, AD, SIGN, 1, Rs...>(v, lds, col, tc, tw);
   }
 };
+template <typename T, int L, int NCP, int SIGN, typename Rad> using BlockFFT = BlockFFTA<T, L, RowAddr<NCP>, SIGN, Rad>;
 
 // forward (SIGN = -1) with Plan::Fwd, inverse (SIGN = +1) with Plan::Inv
-template <typename T, int L, int NCP> using FwdFFT = BlockFFT<T, L, NCP, -1, typename Plan<L>::Fwd>;
-template <typename T, int L, int NCP> using InvFFT = BlockFFT<T, L, NCP, +1, typename Plan<L>::Inv>;
+template <typename T, int L, int NCP> using FwdFFT = BlockFFTA<T, L, RowAddr<NCP>, -1, typename Plan<L>::Fwd>;
+template <typename T, int L, int NCP> using InvFFT = BlockFFTA<T, L, RowAddr<NCP>, +1, typename Plan<L>::Inv>;
+template <typename T, int L, typename AD> using FwdFFTA = BlockFFTA<T, L, AD, -1, typename Plan<L>::Fwd>;
+template <typename T, int L, typename AD> using InvFFTA = BlockFFTA<T, L, AD, +1, typename Plan<L>::Inv>;
 template <int L> constexpr int fwd_first() { return RadInfo<typename Plan<L>::Fwd>::first; }
 template <int L> constexpr int fwd_last() { return RadInfo<typename Plan<L>::Fwd>::last; }
 template <int L> constexpr int inv_first() { return RadInfo<typename Plan<L>::Inv>::first; }

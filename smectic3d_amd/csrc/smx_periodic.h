@@ -66,37 +66,34 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_ypass(const cplx<T>* in
                                                                int64_t npencils,
                                                                const cplx<T>* __restrict__ tw,
                                                                const T* __restrict__ kmul) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
-  __shared__ cplx<T> S[L * NCP];
-  const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
-  const int64_t p0 = (int64_t)blockIdx.x * NC;
-  const int nvalid = (int)((npencils - p0) < NC ? (npencils - p0) : NC);
-  const cplx<T>* src = in + p0 * L;
-  cplx<T>* dst = out + p0 * L;
-  for (int i = tid; i < NC * L; i += NT) {
-    const int c = i / L, pos = i - c * L;
-    S[pos * NCP + c] = (c < nvalid) ? src[i] : mk<T>(0, 0);
-  }
-  __syncthreads();
+  // lanes run along each pencil (tc fastest), so the first-pass loads and last-pass
+  // stores go straight between HBM and registers (16 contiguous complex per lane group)
+  constexpr int E = Plan<L>::E, TPC = L / E;
+  using AD = ColAddr<L>;
+  using TL = TileA<T, L, E, AD>;
+  __shared__ cplx<T> S[NC * AD::LP];
+  const int tid = threadIdx.x, tc = tid % TPC, col = tid / TPC;
+  const int64_t pen = (int64_t)blockIdx.x * NC + col;
+  const bool valid = pen < npencils;
+  const cplx<T>* src = in + pen * L;
   cplx<T> v[E];
-  using FFT = BlockFFT<T, L, NCP, SIGN, typename std::conditional<(SIGN < 0), typename Plan<L>::Fwd, typename Plan<L>::Inv>::type>;
+  using FFT = BlockFFTA<T, L, AD, SIGN, typename std::conditional<(SIGN < 0), typename Plan<L>::Fwd, typename Plan<L>::Inv>::type>;
   constexpr int RF = (SIGN < 0) ? fwd_first<L>() : inv_first<L>();
   constexpr int RL = (SIGN < 0) ? fwd_last<L>() : inv_last<L>();
-  Tile<T, L, E, NCP>::template load<RF>(v, S, col, tc);
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = valid ? src[TL::template pos_in<RF>(tc, e)] : mk<T>(0, 0);
   if (kmul != nullptr) {  // multiply by i k_y before the transform (spectral d/dy, pf3d.cpp:405)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const T k = kmul[Tile<T, L, E, NCP>::template pos_in<RF>(tc, e)];
+      const T k = kmul[TL::template pos_in<RF>(tc, e)];
       v[e] = mk(-k * v[e].y, k * v[e].x);
     }
   }
   FFT::run(v, S, col, tc, tw);
-  __syncthreads();
-  Tile<T, L, E, NCP>::template store<RL>(v, S, col, tc);
-  __syncthreads();
-  for (int i = tid; i < NC * L; i += NT) {
-    const int c = i / L, pos = i - c * L;
-    if (c < nvalid) dst[i] = S[pos * NCP + c];
+  if (valid) {
+    cplx<T>* dst = out + pen * L;
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[TL::template pos_in<RL>(tc, e)] = v[e];
   }
 }
 
@@ -115,9 +112,30 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
-  for (int i = tid; i < NC * L; i += NT) {
-    const int pos = i / NC, c = i - pos * NC;
-    S[pos * NCP + c] = in[plane + (int64_t)pos * p.ny + y0 + c];
+  // On the self-conjugate planes kz = 0 and kz = nz/2 a block takes NC/2 columns ky and
+  // their partners -ky, so that R can be projected onto Hermitian symmetry in place
+  // (the carried state otherwise accumulates an anti-Hermitian part there that the
+  // C2R never sees and that grows in the unstable band; see k_herm_project).
+  constexpr int H = NC / 2;
+  const bool herm = (MODE == X_STEP || MODE == X_BOOT) && (kz == 0 || 2 * kz == p.nz);
+  int ky = y0 + col, pcol = col;
+  if (herm) {
+    const int b = blockIdx.x * H;
+    if (col < H) {
+      ky = b + col;
+      pcol = (b + col == 0) ? col : col + H;
+    } else {
+      ky = (b == 0 && col == H) ? p.ny / 2 : p.ny - (b + col - H);
+      pcol = (b == 0 && col == H) ? col : col - H;
+    }
+  }
+  const cplx<T>* src = in + plane + ky;
+  cplx<T> v[E];
+  // first-pass inputs straight from HBM: lanes run along y (16 contiguous complex per row)
+  {
+    constexpr int R0 = (MODE == X_INV) ? inv_first<L>() : fwd_first<L>();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = src[(int64_t)Tile<T, L, E, NCP>::template pos_in<R0>(tc, e) * p.ny];
   }
   // FP64 mean mode (mixed path): the block owning k = 0 sums the z-pass partials.
   double nsum = 0.0;
@@ -127,18 +145,14 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
     for (int i = tid; i < p.npartial; i += NT) a += p.partial[i];
     nsum = block_sum(a, red);  // valid in thread 0 (= col 0, tc 0)
   }
-  __syncthreads();
-  cplx<T> v[E];
-  if constexpr (MODE == X_INV) {
-    Tile<T, L, E, NCP>::template load<inv_first<L>()>(v, S, col, tc);
-  } else {
-    Tile<T, L, E, NCP>::template load<fwd_first<L>()>(v, S, col, tc);
+  if constexpr (MODE != X_INV) {
     FwdFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
-    const T kyv = p.ky[y0 + col], kzv = p.kz[kz];
+    const T kyv = p.ky[ky], kzv = p.kz[kz];
+    if (herm) __syncthreads();  // S receives R' below; the FFT's last reads must be done
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e);
-      const int64_t idx = plane + (int64_t)kxi * p.ny + y0 + col;
+      const int64_t idx = plane + (int64_t)kxi * p.ny + ky;
       if constexpr (MODE == X_FWD_SCALED) {
         v[e] = p.scale * v[e];
         if (owner0 && tid == 0 && kxi == 0) {
@@ -168,7 +182,8 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
             ps = mk((T)ps0, (T)0);
             rn = mk((T)p.mean_state[0], (T)0);
           }
-          rbuf[idx] = rn;
+          if (herm) S[kxi * NCP + col] = rn;
+          else rbuf[idx] = rn;
           if (p.rec_psi != nullptr) p.rec_psi[idx] = ps;
           v[e] = ps;
         } else {  // X_BOOT: R = C1 psi0 - dt/2 N_past with N_past = N(psi0) (pf3d.cpp:363-374)
@@ -180,24 +195,30 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
             ps = mk((T)ps0, (T)0);
             rn = mk((T)p.mean_state[0], (T)0);
           }
-          rbuf[idx] = rn;
+          if (herm) S[kxi * NCP + col] = rn;
+          else rbuf[idx] = rn;
           v[e] = ps;
         }
       }
     }
+    if (herm) {  // R(k) <- (R(k) + conj R(-k))/2 with -k = (nx - kx, partner column)
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e);
+        const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
+        rbuf[plane + (int64_t)kxi * p.ny + ky] = mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y));
+      }
+    }
   }
+  cplx<T>* dst = out + plane + ky;
   if constexpr (MODE == X_FWD_SCALED) {
-    __syncthreads();
-    Tile<T, L, E, NCP>::template store<fwd_last<L>()>(v, S, col, tc);
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[(int64_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e) * p.ny] = v[e];
   } else {
     InvFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
-    __syncthreads();
-    Tile<T, L, E, NCP>::template store<inv_last<L>()>(v, S, col, tc);
-  }
-  __syncthreads();
-  for (int i = tid; i < NC * L; i += NT) {
-    const int pos = i / NC, c = i - pos * NC;
-    out[plane + (int64_t)pos * p.ny + y0 + c] = S[pos * NCP + c];
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[(int64_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L>()>(tc, e) * p.ny] = v[e];
   }
 }
 
@@ -246,34 +267,29 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cpl
                                                                        double* __restrict__ real_out,
                                                                        int op, PParams<T> p) {
   constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
-  constexpr int NZH = L / 2 + 1, SP = NC + 1, FP = NPR + 1;
-  __shared__ cplx<T> S[NZH * SP];
+  constexpr int NZH = L / 2 + 1, FP = NPR + 1;
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const int nvalid = (int)((p.nxy - f0) < NC ? (p.nxy - f0) : NC);
+  const bool va = q < nvalid, vb = q + NPR < nvalid;
   cplx<T> v[E];
   if constexpr (IN == Z_IN_SPEC) {
-    for (int i = tid; i < NZH * NC; i += NT) {
-      const int k = i / NC, c = i - k * NC;
-      S[k * SP + c] = (c < nvalid) ? in[(int64_t)k * p.nxy + f0 + c] : mk<T>(0, 0);
-    }
-    __syncthreads();
-    // Hermitian extension of the pair; only the real parts of kz = 0 and
-    // kz = nz/2 enter (the half-complex stage of an FFTW c2r).
+    // Hermitian extension of the column pair read straight from HBM; only the real
+    // parts of kz = 0 and kz = nz/2 enter (the half-complex stage of an FFTW c2r).
+    const cplx<T>* col = in + f0 + q;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
-      if (n <= L / 2) {
-        cplx<T> A = S[n * SP + q], B = S[n * SP + q + NPR];
-        if (n == 0 || n == L / 2) { A.y = 0; B.y = 0; }
-        v[e] = mk(A.x - B.y, A.y + B.x);
-      } else {
-        const int m = L - n;
-        const cplx<T> A = S[m * SP + q], B = S[m * SP + q + NPR];
-        v[e] = mk(A.x + B.y, B.x - A.y);
+      const int m = (n <= L / 2) ? n : L - n;
+      cplx<T> A = va ? col[(int64_t)m * p.nxy] : mk<T>(0, 0);
+      cplx<T> B = vb ? col[(int64_t)m * p.nxy + NPR] : mk<T>(0, 0);
+      if (n == 0 || n == L / 2) {
+        A.y = 0;
+        B.y = 0;
       }
+      v[e] = (n <= L / 2) ? mk(A.x - B.y, A.y + B.x) : mk(A.x + B.y, B.x - A.y);
     }
     InvFFT<T, L, FP>::run(v, F, q, tc, p.twz);
   } else {
@@ -310,8 +326,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cpl
       for (int e = 0; e < E; ++e) {
         const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
         T* row = p.rec_np + (int64_t)n * p.nxy + f0;
-        if (q < nvalid) row[q] = nonlin_prime(p, v[e].x);
-        if (q + NPR < nvalid) row[q + NPR] = nonlin_prime(p, v[e].y);
+        if (va) row[q] = nonlin_prime(p, v[e].x);
+        if (vb) row[q + NPR] = nonlin_prime(p, v[e].y);
       }
     }
     double acc = 0.0;
@@ -338,17 +354,13 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cpl
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
     __syncthreads();
-    // split W = X + iY:  X = (W_k + conj W_{-k})/2,  Y = (W_k - conj W_{-k})/(2i)
+    // split W = X + iY:  X = (W_k + conj W_{-k})/2,  Y = (W_k - conj W_{-k})/(2i); straight to HBM
+    cplx<T>* dst = out + f0;
     for (int i = tid; i < NZH * NPR; i += NT) {
       const int k = i / NPR, qq = i - k * NPR;
       const cplx<T> W = F[k * FP + qq], Wm = F[((L - k) % L) * FP + qq];
-      S[k * SP + qq] = mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y));
-      S[k * SP + qq + NPR] = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
-    }
-    __syncthreads();
-    for (int i = tid; i < NZH * NC; i += NT) {
-      const int k = i / NC, c = i - k * NC;
-      if (c < nvalid) out[(int64_t)k * p.nxy + f0 + c] = S[k * SP + c];
+      if (qq < nvalid) dst[(int64_t)k * p.nxy + qq] = mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y));
+      if (qq + NPR < nvalid) dst[(int64_t)k * p.nxy + qq + NPR] = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
     }
   }
 }
