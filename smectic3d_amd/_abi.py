@@ -71,6 +71,22 @@ class SmxSurfOpts(C.Structure):
     _fields_ = [("arm", C.c_double), ("vel_sign", C.c_double)]
 
 
+class SmxSchedule(C.Structure):
+    """Tile schedules of the periodic step kernels: slots (y inverse, z, y forward, x)."""
+    _fields_ = [("nc", C.c_int * 4), ("variant", C.c_int * 4)]
+
+    def as_list(self):
+        return [(int(self.nc[k]), int(self.variant[k])) for k in range(4)]
+
+    @classmethod
+    def from_list(cls, slots):
+        s = cls()
+        for k, (nc, v) in enumerate(slots):
+            s.nc[k] = nc
+            s.variant[k] = v
+        return s
+
+
 def make_config(nx, ny=None, nz=None, *, bc=SMX_BC_PERIODIC, precision=SMX_FP64, lx=None, ly=None,
                 lz=None, nw=None, eps=-0.7, q0=1.0, alpha=1.0, beta=2.0, gamma=1.0, dt=1e-3,
                 nl_model=SMX_NL_QUINTIC, advect=0, eta=1.0, rho=1.0, force_form=SMX_FORCE_SH,

@@ -6,7 +6,7 @@ point raises.  Build it with ``python -m smectic3d_amd.build``.
 import ctypes as C
 import os
 
-from ._abi import SmxConfig, SmxSurfOpts
+from ._abi import SmxConfig, SmxSchedule, SmxSurfOpts
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsmectic.so")
 
@@ -16,7 +16,8 @@ EXPORTS = [
     "smx_set_psi", "smx_get_psi", "smx_step", "smx_l1", "smx_energy", "smx_dominant_wavenumber",
     "smx_dtpsi", "smx_surftrack", "smx_sync", "smx_step_count", "smx_profile_step",
     "smx_calibration_copy", "smx_adjoint_grad", "smx_batch_create", "smx_batch_destroy",
-    "smx_batch_handle", "smx_batch_run", "smx_batch_graph_nodes",
+    "smx_batch_handle", "smx_batch_run", "smx_batch_graph_nodes", "smx_get_schedule", "smx_set_schedule",
+    "smx_autotune", "smx_batch_autotune",
 ]
 
 _lib = None
@@ -64,6 +65,10 @@ def lib():
     L.smx_batch_handle.argtypes = [vp, C.c_int, C.POINTER(vp)]
     L.smx_batch_run.argtypes = [vp, C.c_int64, C.c_int64]
     L.smx_batch_graph_nodes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.smx_get_schedule.argtypes = [vp, C.POINTER(SmxSchedule)]
+    L.smx_set_schedule.argtypes = [vp, C.POINTER(SmxSchedule)]
+    L.smx_autotune.argtypes = [vp, C.c_int, C.POINTER(SmxSchedule)]
+    L.smx_batch_autotune.argtypes = [vp, C.c_int]
     for n in EXPORTS:
         f = getattr(L, n)
         if n not in ("smx_config_default", "smx_destroy", "smx_last_error", "smx_batch_destroy"):

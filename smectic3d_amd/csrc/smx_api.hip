@@ -1,10 +1,14 @@
 // smx_api.hip -- host side of libsmectic: handle, device buffers, step chains,
 // FP64 diagnostics, surfTrack and the C ABI declared in include/smectic.h.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/smectic.h"
@@ -12,6 +16,16 @@
 #include "smx_launch.h"
 
 using namespace smx;
+
+// The pass kernels are instantiated once per precision in smx_inst_f32.hip / smx_inst_f64.hip.
+extern template struct smx::PeriodicOps<float>;
+extern template struct smx::PeriodicOps<double>;
+extern template struct smx::NeumannOps<float>;
+extern template struct smx::NeumannOps<double>;
+extern template struct smx::AdvectOps<float>;
+extern template struct smx::AdvectOps<double>;
+extern template struct smx::AdjointOps<float>;
+extern template struct smx::AdjointOps<double>;
 
 // ---------------------------------------------------------------------------
 // errors
@@ -81,7 +95,12 @@ struct smx_handle {
   std::vector<void*> allocs;
   bool has_state = false, has_prev = false;
   int64_t steps = 0;
+  PassSched sched[NSLOTS];  // tile schedules of the periodic step kernels
 };
+
+// FP64 partial sums of one z pass: one per tile at the narrowest tile (8 columns), plus
+// slack.  The handle holds two such regions: the step's and the autotuner's scratch.
+static int64_t partial_len(const smx_handle* h) { return h->nxy / 8 + 64; }
 
 static int dalloc(smx_handle* h, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
@@ -203,33 +222,34 @@ static AParams<T> aparams(smx_handle* h, bool mixed_mean) {
 
 // ---- pass facade over both boundary conditions ----
 template <typename T>
-static int ypass(smx_handle* h, const void* in, void* out, int dir) {
+static int ypass(smx_handle* h, const void* in, void* out, int dir, PassSched s = PassSched{}) {
   const DevTables<T>& t = tables<T>(h);
   hipError_t e = h->neumann
                      ? NeumannOps<T>::ypass((const T*)in, (T*)out, h->cfg.ny, h->npencils, dir, t.twy, t.qtwy, h->st)
                      : PeriodicOps<T>::ypass((const cplx<T>*)in, (cplx<T>*)out, h->cfg.ny, h->npencils, dir, t.twy,
-                                             h->st);
+                                             h->st, nullptr, s);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("y pass: ") + hipGetErrorString(e));
   return SMX_OK;
 }
 
 template <typename T>
-static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, void* out, void* r, const void* psi0) {
+static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, void* out, void* r, const void* psi0,
+                 PassSched s = PassSched{}) {
   hipError_t e = h->neumann ? NeumannOps<T>::xpass(mode, (const T*)in, (T*)out, (T*)r, (const T*)psi0, p,
                                                    tables<T>(h).qtwx, h->st)
                             : PeriodicOps<T>::xpass(mode, (const cplx<T>*)in, (cplx<T>*)out, (cplx<T>*)r,
-                                                    (const cplx<T>*)psi0, p, h->st);
+                                                    (const cplx<T>*)psi0, p, h->st, s);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("x pass: ") + hipGetErrorString(e));
   return SMX_OK;
 }
 
 template <typename T>
 static int zpass(smx_handle* h, const PParams<T>& p, int in_mode, int out_mode, int op, const void* in, void* out,
-                 const double* real_in, double* real_out) {
+                 const double* real_in, double* real_out, PassSched s = PassSched{}) {
   hipError_t e = h->neumann ? NeumannOps<T>::zpass(in_mode, out_mode, op, (const T*)in, (T*)out, real_in, real_out,
                                                    p, tables<T>(h).qtwz, h->st)
                             : PeriodicOps<T>::zpass(in_mode, out_mode, op, (const cplx<T>*)in, (cplx<T>*)out,
-                                                    real_in, real_out, p, h->st);
+                                                    real_in, real_out, p, h->st, s);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("z pass: ") + hipGetErrorString(e));
   return SMX_OK;
 }
@@ -334,14 +354,30 @@ static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
     AParams<T> ap = aparams<T>(h, h->fp32);
     return step_advect<T>(h, ap, ev);
   }
+  if (h->neumann) {
+    if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
+    CHK(ypass<T>(h, h->A, h->A, +1));
+    if (ev) HIPCHK(hipEventRecord(ev[1], h->st));
+    CHK(zpass<T>(h, p, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr));
+    if (ev) HIPCHK(hipEventRecord(ev[2], h->st));
+    CHK(ypass<T>(h, h->A, h->A, -1));
+    if (ev) HIPCHK(hipEventRecord(ev[3], h->st));
+    CHK(xpass<T>(h, p, X_STEP, h->A, h->A, h->R, nullptr));
+    if (ev) HIPCHK(hipEventRecord(ev[4], h->st));
+    return SMX_OK;
+  }
+  // periodic chain with the handle's tile schedules; the x pass sums the z pass's
+  // FP64 partials, one per z tile
+  PParams<T> q = p;
+  q.npartial = PeriodicOps<T>::zpass_blocks(p, h->sched[SLOT_Z].nc);
   if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
-  CHK(ypass<T>(h, h->A, h->A, +1));
+  CHK(ypass<T>(h, h->A, h->A, +1, h->sched[SLOT_YINV]));
   if (ev) HIPCHK(hipEventRecord(ev[1], h->st));
-  CHK(zpass<T>(h, p, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr));
+  CHK(zpass<T>(h, q, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr, h->sched[SLOT_Z]));
   if (ev) HIPCHK(hipEventRecord(ev[2], h->st));
-  CHK(ypass<T>(h, h->A, h->A, -1));
+  CHK(ypass<T>(h, h->A, h->A, -1, h->sched[SLOT_YFWD]));
   if (ev) HIPCHK(hipEventRecord(ev[3], h->st));
-  CHK(xpass<T>(h, p, X_STEP, h->A, h->A, h->R, nullptr));
+  CHK(xpass<T>(h, q, X_STEP, h->A, h->A, h->R, nullptr, h->sched[SLOT_X]));
   if (ev) HIPCHK(hipEventRecord(ev[4], h->st));
   return SMX_OK;  // the x pass projects the self-conjugate planes of R itself
 }
@@ -695,6 +731,96 @@ static int batch_capture(smx_batch* b, int64_t gsteps) {
 }
 
 // ---------------------------------------------------------------------------
+// Tile-schedule autotuner.  Every valid (columns per tile, plan variant) candidate of
+// each periodic step kernel is timed with HIP events on the handle's stream, on the
+// FP64 diagnostic buffers as scratch (the state A, R and the FP64 mean mode are not
+// touched), and the fastest is kept.  The role FFTW_MEASURE planning plays in the
+// reference (pf3d.cpp:226-291), for the fused pass kernels.  Results are cached per
+// (device, precision, grid) for the life of the process.
+struct TuneKey {
+  int dev, fp32, nx, ny, nz;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dev, fp32, nx, ny, nz) < std::tie(o.dev, o.fp32, o.nx, o.ny, o.nz);
+  }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::array<PassSched, NSLOTS>> g_tune_cache;
+
+template <typename T>
+static hipError_t launch_slot(smx_handle* h, const PParams<T>& p, int slot, PassSched s, bool dry, void* a,
+                              void* r) {
+  const DevTables<T>& t = tables<T>(h);
+  cplx<T>* A = (cplx<T>*)a;
+  switch (slot) {
+    case SLOT_YINV:
+      return PeriodicOps<T>::ypass(A, A, h->cfg.ny, h->npencils, +1, t.twy, h->st, nullptr, s, dry);
+    case SLOT_YFWD:
+      return PeriodicOps<T>::ypass(A, A, h->cfg.ny, h->npencils, -1, t.twy, h->st, nullptr, s, dry);
+    case SLOT_Z:
+      return PeriodicOps<T>::zpass(Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, A, A, nullptr, nullptr, p, h->st, s, dry);
+    case SLOT_X:
+      return PeriodicOps<T>::xpass(X_STEP, A, A, (cplx<T>*)r, nullptr, p, h->st, s, dry);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+static int sched_check(smx_handle* h, const PassSched* s) {
+  PParams<T> p = params<T>(h, false);
+  for (int k = 0; k < NSLOTS; ++k)
+    if (launch_slot<T>(h, p, k, s[k], true, h->D1, h->D2) != hipSuccess)
+      return fail(SMX_ERR_UNSUPPORTED, "schedule slot " + std::to_string(k) + " (nc " + std::to_string(s[k].nc) +
+                                           ", variant " + std::to_string(s[k].v) + ") has no kernel for this grid");
+  return SMX_OK;
+}
+
+template <typename T>
+static int autotune_T(smx_handle* h, int reps, PassSched* best) {
+  PParams<T> p = params<T>(h, h->fp32);
+  if (p.partial) p.partial = h->partial + partial_len(h);  // the tuner's own partial-sum region
+  p.mean_state = h->mean_state + 4;                        // scratch entries 4..7
+  p.npartial = (int)(h->nxy / kNC);
+  const size_t dbytes = (h->neumann ? 8 : 16) * (size_t)h->nspec;
+  HIPCHK(hipMemsetAsync(h->D1, 0, dbytes, h->st));
+  HIPCHK(hipMemsetAsync(h->D2, 0, dbytes, h->st));
+  HIPCHK(hipMemsetAsync(h->partial + partial_len(h), 0, sizeof(double) * partial_len(h), h->st));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  int rc = (e == hipSuccess) ? SMX_OK : fail(SMX_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+  for (int k = 0; k < NSLOTS && rc == SMX_OK; ++k) {
+    double tbest = 1e300;
+    PassSched pick{};
+    for (int v : kSchedV)
+      for (int nc : kSchedNC) {
+        const PassSched c{nc, v};
+        if (rc != SMX_OK || launch_slot<T>(h, p, k, c, true, h->D1, h->D2) != hipSuccess) continue;
+        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_slot<T>(h, p, k, c, false, h->D1, h->D2);
+        if (e == hipSuccess) e = hipEventRecord(e0, h->st);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_slot<T>(h, p, k, c, false, h->D1, h->D2);
+        if (e == hipSuccess) e = hipEventRecord(e1, h->st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = fail(SMX_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+          break;
+        }
+        const double us = 1000.0 * ms / reps;
+        if (us < tbest) {
+          tbest = us;
+          pick = c;
+        }
+      }
+    best[k] = pick;
+  }
+  (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 extern "C" {
 
@@ -773,7 +899,7 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
       (rc = dalloc(h, &h->P, h->wbytes * h->nspec)) || (rc = dalloc(h, &h->D1, dbytes)) ||
       (rc = dalloc(h, &h->D2, dbytes)) || (rc = dalloc(h, (void**)&h->real_io, sizeof(double) * h->N)) ||
       (rc = dalloc(h, (void**)&h->real_prev, sizeof(double) * h->N)) ||
-      (rc = dalloc(h, (void**)&h->partial, sizeof(double) * (h->nxy / kNC + 64))) ||
+      (rc = dalloc(h, (void**)&h->partial, sizeof(double) * 2 * partial_len(h))) ||
       (rc = dalloc(h, (void**)&h->mean_state, sizeof(double) * 8)) ||
       (rc = dalloc(h, (void**)&h->red, sizeof(double) * 4 * kRedBlocks)) ||
       (rc = dalloc(h, (void**)&h->flag, sizeof(unsigned int) * 4)) ||
@@ -997,6 +1123,58 @@ int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int6
 }
 
 
+int smx_get_schedule(const smx_handle* h, smx_schedule* s) {
+  if (!h || !s) return fail(SMX_ERR_INVALID, "null argument");
+  for (int k = 0; k < NSLOTS; ++k) {
+    s->nc[k] = h->sched[k].nc;
+    s->variant[k] = h->sched[k].v;
+  }
+  return SMX_OK;
+}
+
+int smx_set_schedule(smx_handle* h, const smx_schedule* s) {
+  if (!h || !s) return fail(SMX_ERR_INVALID, "null argument");
+  PassSched c[NSLOTS];
+  bool is_default = true;
+  for (int k = 0; k < NSLOTS; ++k) {
+    c[k] = PassSched{s->nc[k], s->variant[k]};
+    is_default = is_default && c[k].nc == kNC && c[k].v == 0;
+  }
+  if (!is_default && (h->neumann || h->cfg.advect))
+    return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic non-advected step only");
+  HIPCHK(hipSetDevice(h->cfg.device));
+  CHK(h->fp32 ? sched_check<float>(h, c) : sched_check<double>(h, c));
+  for (int k = 0; k < NSLOTS; ++k) h->sched[k] = c[k];
+  return SMX_OK;
+}
+
+int smx_autotune(smx_handle* h, int reps, smx_schedule* chosen) {
+  if (!h) return fail(SMX_ERR_INVALID, "null handle");
+  if (reps < 1 || reps > 1000) return fail(SMX_ERR_INVALID, "reps must lie in [1, 1000]");
+  if (h->neumann || h->cfg.advect)
+    return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic non-advected step only");
+  HIPCHK(hipSetDevice(h->cfg.device));
+  const TuneKey key{h->cfg.device, h->fp32 ? 1 : 0, h->cfg.nx, h->cfg.ny, h->cfg.nz};
+  std::array<PassSched, NSLOTS> best;
+  bool hit = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      best = it->second;
+      hit = true;
+    }
+  }
+  if (!hit) {
+    CHK(h->fp32 ? autotune_T<float>(h, reps, best.data()) : autotune_T<double>(h, reps, best.data()));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  for (int k = 0; k < NSLOTS; ++k) h->sched[k] = best[k];
+  if (chosen) CHK(smx_get_schedule(h, chosen));
+  return SMX_OK;
+}
+
 int smx_batch_create(int nbox, const smx_config* cfgs, smx_batch** out) {
   if (!cfgs || !out || nbox < 1) return fail(SMX_ERR_INVALID, "bad argument");
   *out = nullptr;
@@ -1056,6 +1234,21 @@ int smx_batch_run(smx_batch* b, int64_t nsteps, int64_t graph_steps) {
     HIPCHK(hipStreamSynchronize(h->st));
     h->steps += nsteps;
     h->has_prev = false;  // no L1 snapshot inside a batch run
+  }
+  return SMX_OK;
+}
+
+int smx_batch_autotune(smx_batch* b, int reps) {
+  if (!b) return fail(SMX_ERR_INVALID, "null batch");
+  for (smx_handle* h : b->boxes) CHK(smx_autotune(h, reps, nullptr));
+  // the captured graph holds the old schedules: capture again on the next run
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
   }
   return SMX_OK;
 }

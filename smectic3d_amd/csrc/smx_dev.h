@@ -37,6 +37,10 @@ template <typename T> __device__ __forceinline__ cplx<T> conjg(cplx<T> a) { retu
 template <int SIGN, typename T> __device__ __forceinline__ cplx<T> mul_si(cplx<T> a) {
   return SIGN > 0 ? mk(-a.y, a.x) : mk(a.y, -a.x);
 }
+// 1/x: IEEE division in FP64; the hardware reciprocal (1 ulp) in FP32, whose
+// passes already round at that level.
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // cos(2*pi*m/48) and sin(2*pi*m/48), m = 0..47, rounded from extended precision
 __device__ constexpr double kRootCos48[48] = {1.0, 0.9914448613738104, 0.9659258262890683, 0.9238795325112867, 0.8660254037844386, 0.7933533402912352, 0.7071067811865476, 0.6087614290087207, 0.5, 0.3826834323650898, 0.25881904510252074, 0.1305261922200516, 0.0, -0.1305261922200516, -0.25881904510252074, -0.3826834323650898, -0.5, -0.6087614290087207, -0.7071067811865476, -0.7933533402912352, -0.8660254037844386, -0.9238795325112867, -0.9659258262890683, -0.9914448613738104, -1.0, -0.9914448613738104, -0.9659258262890683, -0.9238795325112867, -0.8660254037844386, -0.7933533402912352, -0.7071067811865476, -0.6087614290087207, -0.5, -0.3826834323650898, -0.25881904510252074, -0.1305261922200516, 0.0, 0.1305261922200516, 0.25881904510252074, 0.3826834323650898, 0.5, 0.6087614290087207, 0.7071067811865476, 0.7933533402912352, 0.8660254037844386, 0.9238795325112867, 0.9659258262890683, 0.9914448613738104};
@@ -140,6 +144,27 @@ SMX_PLAN(384, 12, 4 SMX_COMMA 4 SMX_COMMA 4 SMX_COMMA 6, 6 SMX_COMMA 4 SMX_COMMA
 SMX_PLAN(512, 16, 16 SMX_COMMA 16 SMX_COMMA 2, 2 SMX_COMMA 16 SMX_COMMA 16)
 #undef SMX_PLAN
 
+// Alternative schedules of the same length (an autotuner picks per pass kernel).
+// Variant 0 is Plan<L>; variant 1 (power-of-two lengths) changes the elements per
+// thread: E = 8 for L >= 64 (twice the threads per line, half the registers per
+// thread), E = 16 for L = 32.
+template <int L, int V> struct PlanV;
+template <int L> struct PlanV<L, 0> : Plan<L> {};
+#define SMX_PLANV(L_, V_, E_, FWD, INV) \
+  template <> struct PlanV<L_, V_> {    \
+    static constexpr int E = E_;        \
+    using Fwd = Radices<FWD>;           \
+    using Inv = Radices<INV>;           \
+  };
+SMX_PLANV(32, 1, 16, 16 SMX_COMMA 2, 2 SMX_COMMA 16)
+SMX_PLANV(64, 1, 8, 8 SMX_COMMA 8, 8 SMX_COMMA 8)
+SMX_PLANV(128, 1, 8, 8 SMX_COMMA 4 SMX_COMMA 4, 4 SMX_COMMA 4 SMX_COMMA 8)
+SMX_PLANV(256, 1, 8, 8 SMX_COMMA 8 SMX_COMMA 4, 4 SMX_COMMA 8 SMX_COMMA 8)
+SMX_PLANV(512, 1, 8, 8 SMX_COMMA 8 SMX_COMMA 8, 8 SMX_COMMA 8 SMX_COMMA 8)
+#undef SMX_PLANV
+template <int L, int V, typename = void> struct HasPlanV { static constexpr bool value = false; };
+template <int L, int V> struct HasPlanV<L, V, decltype((void)PlanV<L, V>::E)> { static constexpr bool value = true; };
+
 // ---------------------------------------------------------------------------
 // LDS addressing of an exchange tile of NC columns of length L.
 //   RowAddr<NCP>: [pos][col] with row pitch NCP (lanes run across columns).
@@ -212,10 +237,9 @@ __device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, in
   }
 }
 
-template <typename T, int L, typename AD, int SIGN, typename Rad> struct BlockFFTA;
-template <typename T, int L, typename AD, int SIGN, int... Rs>
-struct BlockFFTA<T, L, AD, SIGN, Radices<Rs...>> {
-  static constexpr int E = Plan<L>::E;
+template <typename T, int L, int E, typename AD, int SIGN, typename Rad> struct BlockFFTA;
+template <typename T, int L, int E, typename AD, int SIGN, int... Rs>
+struct BlockFFTA<T, L, E, AD, SIGN, Radices<Rs...>> {
   // v: first-pass input positions on entry; last-pass output positions on exit.
   // The LDS tile is used for the exchanges; all threads must have finished any
   // earlier reads of it (the first exchange starts with a barrier).
@@ -223,17 +247,33 @@ struct BlockFFTA<T, L, AD, SIGN, Radices<Rs...>> {
     fft_passes<T, L, E, AD, SIGN, 1, Rs...>(v, lds, col, tc, tw);
   }
 };
-template <typename T, int L, int NCP, int SIGN, typename Rad> using BlockFFT = BlockFFTA<T, L, RowAddr<NCP>, SIGN, Rad>;
+// forward (SIGN = -1) with the plan's Fwd radices, inverse (SIGN = +1) with its Inv radices
+template <typename T, int L, int NCP, int V = 0>
+using FwdFFT = BlockFFTA<T, L, PlanV<L, V>::E, RowAddr<NCP>, -1, typename PlanV<L, V>::Fwd>;
+template <typename T, int L, int NCP, int V = 0>
+using InvFFT = BlockFFTA<T, L, PlanV<L, V>::E, RowAddr<NCP>, +1, typename PlanV<L, V>::Inv>;
+template <typename T, int L, typename AD, int V = 0>
+using FwdFFTA = BlockFFTA<T, L, PlanV<L, V>::E, AD, -1, typename PlanV<L, V>::Fwd>;
+template <typename T, int L, typename AD, int V = 0>
+using InvFFTA = BlockFFTA<T, L, PlanV<L, V>::E, AD, +1, typename PlanV<L, V>::Inv>;
+template <int L, int V = 0> constexpr int fwd_first() { return RadInfo<typename PlanV<L, V>::Fwd>::first; }
+template <int L, int V = 0> constexpr int fwd_last() { return RadInfo<typename PlanV<L, V>::Fwd>::last; }
+template <int L, int V = 0> constexpr int inv_first() { return RadInfo<typename PlanV<L, V>::Inv>::first; }
+template <int L, int V = 0> constexpr int inv_last() { return RadInfo<typename PlanV<L, V>::Inv>::last; }
 
-// forward (SIGN = -1) with Plan::Fwd, inverse (SIGN = +1) with Plan::Inv
-template <typename T, int L, int NCP> using FwdFFT = BlockFFTA<T, L, RowAddr<NCP>, -1, typename Plan<L>::Fwd>;
-template <typename T, int L, int NCP> using InvFFT = BlockFFTA<T, L, RowAddr<NCP>, +1, typename Plan<L>::Inv>;
-template <typename T, int L, typename AD> using FwdFFTA = BlockFFTA<T, L, AD, -1, typename Plan<L>::Fwd>;
-template <typename T, int L, typename AD> using InvFFTA = BlockFFTA<T, L, AD, +1, typename Plan<L>::Inv>;
-template <int L> constexpr int fwd_first() { return RadInfo<typename Plan<L>::Fwd>::first; }
-template <int L> constexpr int fwd_last() { return RadInfo<typename Plan<L>::Fwd>::last; }
-template <int L> constexpr int inv_first() { return RadInfo<typename Plan<L>::Inv>::first; }
-template <int L> constexpr int inv_last() { return RadInfo<typename Plan<L>::Inv>::last; }
+// ---------------------------------------------------------------------------
+// Global access at a 32-bit byte offset from a wave-uniform base pointer: the base
+// stays in SGPRs and each lane carries one 32-bit offset (the saddr form of
+// global_load/global_store) instead of a 64-bit address pair.  Every array of the
+// solver is below 4 GiB (a 512^3 FP64 half spectrum is 1.08 GB).
+template <typename X>
+__device__ __forceinline__ X ld_off(const X* base, uint32_t off) {
+  return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename X>
+__device__ __forceinline__ void st_off(X* base, uint32_t off, const X& v) {
+  *reinterpret_cast<X*>(reinterpret_cast<char*>(base) + off) = v;
+}
 
 // ---------------------------------------------------------------------------
 // FP64 block reduction helpers (wavefront shuffles, then LDS across waves).

@@ -2,6 +2,7 @@
 // Instantiated once per precision in smx_inst_f32.hip / smx_inst_f64.hip.
 #pragma once
 #include <algorithm>
+#include <type_traits>
 
 #include "smx_adjoint.h"
 #include "smx_neumann.h"
@@ -21,18 +22,58 @@ inline bool length_supported(int n) {
   return false;
 }
 
+// Tile schedule of one pass kernel: columns per tile (NC) and plan variant (V, PlanV).
+// The defaults are the fixed schedule; an autotuner may pick others per pass.
+struct PassSched {
+  int nc = kNC, v = 0;
+};
+// Schedule slots of the periodic step chain, in launch order.
+enum SchedSlot { SLOT_YINV = 0, SLOT_Z = 1, SLOT_YFWD = 2, SLOT_X = 3, NSLOTS = 4 };
+constexpr int kSchedNC[3] = {8, 16, 32};
+constexpr int kSchedV[2] = {0, 1};
+constexpr size_t kMaxLds = 160 * 1024 - 1024;  // LDS of one workgroup on gfx950, less the reduction scratch
+
+template <int N> using IC = std::integral_constant<int, N>;
+
+// Calls f(IC<NC>, IC<V>) for a schedule of length L; invalid schedules give hipErrorInvalidValue.
+template <int L, typename F>
+hipError_t sched_dispatch(const PassSched& s, F&& f) {
+  if (s.v == 0) {
+    switch (s.nc) {
+      case 8: return f(IC<8>{}, IC<0>{});
+      case 16: return f(IC<16>{}, IC<0>{});
+      case 32: return f(IC<32>{}, IC<0>{});
+      default: break;
+    }
+  } else if (s.v == 1) {
+    if constexpr (HasPlanV<L, 1>::value) {
+      switch (s.nc) {
+        case 8: return f(IC<8>{}, IC<1>{});
+        case 16: return f(IC<16>{}, IC<1>{});
+        case 32: return f(IC<32>{}, IC<1>{});
+        default: break;
+      }
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
 template <typename T>
 struct PeriodicOps {
-  // y pass, in place allowed.  sign -1 forward, +1 inverse.
+  // y pass, in place allowed.  sign -1 forward, +1 inverse.  dry: only report whether the
+  // schedule is valid for this length (no launch).
   static hipError_t ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
-                          const cplx<T>* tw, hipStream_t st, const T* kmul = nullptr);
-  // x pass (mode: XMode), tiles over (y, kz).
+                          const cplx<T>* tw, hipStream_t st, const T* kmul = nullptr, PassSched s = PassSched{},
+                          bool dry = false);
+  // x pass (mode: XMode), tiles over (y, kz).  Only X_STEP takes a non-default schedule.
   static hipError_t xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx<T>* rbuf, const cplx<T>* psi0,
-                          const PParams<T>& p, hipStream_t st);
-  // z pass.  in_mode: ZIn, out_mode: ZOut bit set, op: ZOp.
+                          const PParams<T>& p, hipStream_t st, PassSched s = PassSched{}, bool dry = false);
+  // z pass.  in_mode: ZIn, out_mode: ZOut bit set, op: ZOp.  Only the spectral-to-spectral
+  // form takes a non-default schedule; its grid has nx*ny/s.nc blocks (FP64 partials).
   static hipError_t zpass(int in_mode, int out_mode, int op, const cplx<T>* in, cplx<T>* out,
-                          const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st);
-  static int zpass_blocks(const PParams<T>& p) { return (int)((p.nxy + kNC - 1) / kNC); }
+                          const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st,
+                          PassSched s = PassSched{}, bool dry = false);
+  static int zpass_blocks(const PParams<T>& p, int nc = kNC) { return (int)((p.nxy + nc - 1) / nc); }
   // Hermitian projection of the kz = 0 and kz = nz/2 planes of a full spectrum.
   static hipError_t herm_project(cplx<T>* a, const PParams<T>& p, hipStream_t st) {
     const int64_t work = 2 * p.nxy;
@@ -44,15 +85,22 @@ struct PeriodicOps {
 
 template <typename T>
 hipError_t PeriodicOps<T>::ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
-                                 const cplx<T>* tw, hipStream_t st, const T* kmul) {
-  const dim3 grid((unsigned)((npencils + kNC - 1) / kNC));
-#define SMX_CASE(L)                                                                           \
-  if (ny == L) {                                                                              \
-    const dim3 block(kNC * (L / Plan<L>::E));                                                 \
-    if (sign < 0) hipLaunchKernelGGL((k_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw, kmul); \
-    else hipLaunchKernelGGL((k_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw, kmul);          \
-    return hipGetLastError();                                                                 \
-  }
+                                 const cplx<T>* tw, hipStream_t st, const T* kmul, PassSched s, bool dry) {
+#define SMX_CASE(L)                                                                                      \
+  if (ny == L)                                                                                           \
+    return sched_dispatch<L>(s, [&](auto nc, auto v) -> hipError_t {                                     \
+      constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
+      constexpr int NT = NC * (L / PlanV<L, V>::E);                                                      \
+      if constexpr (NT > 1024 || NC * ColAddr<L>::LP * sizeof(cplx<T>) > kMaxLds) {                      \
+        return hipErrorInvalidValue;                                                                     \
+      } else {                                                                                           \
+        if (dry) return hipSuccess;                                                                      \
+        const dim3 grid((unsigned)((npencils + NC - 1) / NC));                                           \
+        if (sign < 0) hipLaunchKernelGGL((k_ypass<T, L, -1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul); \
+        else hipLaunchKernelGGL((k_ypass<T, L, +1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul);          \
+        return hipGetLastError();                                                                        \
+      }                                                                                                  \
+    });
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
   return hipErrorInvalidValue;
@@ -60,13 +108,15 @@ hipError_t PeriodicOps<T>::ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_
 
 template <typename T>
 hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx<T>* rbuf, const cplx<T>* psi0,
-                                 const PParams<T>& p, hipStream_t st) {
-  const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
+                                 const PParams<T>& p, hipStream_t st, PassSched s, bool dry) {
+  if (mode != X_STEP) {
+    if (s.nc != kNC || s.v != 0) return hipErrorInvalidValue;
+    if (dry) return hipSuccess;
+    const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
 #define SMX_CASE(L)                                                                                   \
   if (p.nx == L) {                                                                                    \
     const dim3 block(kNC * (L / Plan<L>::E));                                                         \
     switch (mode) {                                                                                   \
-      case X_STEP: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_STEP>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
       case X_FWD_SCALED: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
       case X_BOOT: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
       case X_INV: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
@@ -74,6 +124,25 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
     }                                                                                                 \
     return hipGetLastError();                                                                         \
   }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  if (p.ny % s.nc != 0) return hipErrorInvalidValue;  // whole tiles along y
+#define SMX_CASE(L)                                                                                      \
+  if (p.nx == L)                                                                                         \
+    return sched_dispatch<L>(s, [&](auto nc, auto v) -> hipError_t {                                     \
+      constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
+      constexpr int NT = NC * (L / PlanV<L, V>::E);                                                      \
+      if constexpr (NT > 1024 || (size_t)L * (NC + 1) * sizeof(cplx<T>) > kMaxLds) {                     \
+        return hipErrorInvalidValue;                                                                     \
+      } else {                                                                                           \
+        if (dry) return hipSuccess;                                                                      \
+        hipLaunchKernelGGL((k_xpass<T, L, NC, X_STEP, V>), dim3((unsigned)(p.ny / NC), (unsigned)p.nzh), dim3(NT), 0, \
+                           st, in, out, rbuf, psi0, p);                                                  \
+        return hipGetLastError();                                                                        \
+      }                                                                                                  \
+    });
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
   return hipErrorInvalidValue;
@@ -81,14 +150,17 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
 
 template <typename T>
 hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T>* in, cplx<T>* out,
-                                 const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st) {
-  const dim3 grid((unsigned)zpass_blocks(p));
+                                 const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st,
+                                 PassSched s, bool dry) {
+  if (p.nxy % s.nc != 0) return hipErrorInvalidValue;  // the kernel assumes whole tiles
+  if (in_mode != Z_IN_SPEC || out_mode != Z_OUT_SPEC) {
+    if (s.nc != kNC || s.v != 0) return hipErrorInvalidValue;
+    if (dry) return hipSuccess;
+    const dim3 grid((unsigned)zpass_blocks(p));
 #define SMX_CASE(L)                                                                                  \
   if (p.nz == L) {                                                                                   \
     const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                  \
-    if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)                                              \
-      hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
-    else if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                         \
+    if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                              \
       hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
     else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
       hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
@@ -96,6 +168,24 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
       return hipErrorInvalidValue;                                                                   \
     return hipGetLastError();                                                                        \
   }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+#define SMX_CASE(L)                                                                                      \
+  if (p.nz == L)                                                                                         \
+    return sched_dispatch<L>(s, [&](auto nc, auto v) -> hipError_t {                                     \
+      constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
+      constexpr int NT = (NC / 2) * (L / PlanV<L, V>::E);                                                \
+      if constexpr (NT > 1024 || (size_t)L * (NC / 2 + 1) * sizeof(cplx<T>) > kMaxLds) {                 \
+        return hipErrorInvalidValue;                                                                     \
+      } else {                                                                                           \
+        if (dry) return hipSuccess;                                                                      \
+        hipLaunchKernelGGL((k_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC, V>), dim3((unsigned)(p.nxy / NC)), dim3(NT), 0, st, \
+                           in, out, real_in, real_out, op, p);                                           \
+        return hipGetLastError();                                                                        \
+      }                                                                                                  \
+    });
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
   return hipErrorInvalidValue;

@@ -61,14 +61,14 @@ __device__ __forceinline__ T nonlin(const PParams<T>& p, T s) {
 
 // ---------------------------------------------------------------------------
 // y pass: contiguous pencils of length L = ny; pencils g = kz*nx + x.
-template <typename T, int L, int SIGN, int NC>
-__global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_ypass(const cplx<T>* in, cplx<T>* out,
+template <typename T, int L, int SIGN, int NC, int V = 0>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>* in, cplx<T>* out,
                                                                int64_t npencils,
                                                                const cplx<T>* __restrict__ tw,
                                                                const T* __restrict__ kmul) {
   // lanes run along each pencil (tc fastest), so the first-pass loads and last-pass
   // stores go straight between HBM and registers (16 contiguous complex per lane group)
-  constexpr int E = Plan<L>::E, TPC = L / E;
+  constexpr int E = PlanV<L, V>::E, TPC = L / E;
   using AD = ColAddr<L>;
   using TL = TileA<T, L, E, AD>;
   __shared__ cplx<T> S[NC * AD::LP];
@@ -77,9 +77,10 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_ypass(const cplx<T>* in
   const bool valid = pen < npencils;
   const cplx<T>* src = in + pen * L;
   cplx<T> v[E];
-  using FFT = BlockFFTA<T, L, AD, SIGN, typename std::conditional<(SIGN < 0), typename Plan<L>::Fwd, typename Plan<L>::Inv>::type>;
-  constexpr int RF = (SIGN < 0) ? fwd_first<L>() : inv_first<L>();
-  constexpr int RL = (SIGN < 0) ? fwd_last<L>() : inv_last<L>();
+  using FFT = BlockFFTA<T, L, E, AD, SIGN,
+                        typename std::conditional<(SIGN < 0), typename PlanV<L, V>::Fwd, typename PlanV<L, V>::Inv>::type>;
+  constexpr int RF = (SIGN < 0) ? fwd_first<L, V>() : inv_first<L, V>();
+  constexpr int RL = (SIGN < 0) ? fwd_last<L, V>() : inv_last<L, V>();
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = valid ? src[TL::template pos_in<RF>(tc, e)] : mk<T>(0, 0);
   if (kmul != nullptr) {  // multiply by i k_y before the transform (spectral d/dy, pf3d.cpp:405)
@@ -101,17 +102,19 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_ypass(const cplx<T>* in
 // x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1, rows x.
 enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
 
-template <typename T, int L, int NC, int MODE>
-__global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
+template <typename T, int L, int NC, int MODE, int V = 0>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
                                                                cplx<T>* __restrict__ rbuf,
                                                                const cplx<T>* __restrict__ psi0,
                                                                PParams<T> p) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
+  constexpr int E = PlanV<L, V>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
+  constexpr uint32_t CB = sizeof(cplx<T>);
   __shared__ cplx<T> S[L * NCP];
   __shared__ double red[64];
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
+  const uint32_t rowb = (uint32_t)p.ny * CB;  // bytes between kx rows
   // On the self-conjugate planes kz = 0 and kz = nz/2 a block takes NC/2 columns ky and
   // their partners -ky, so that R can be projected onto Hermitian symmetry in place
   // (the carried state otherwise accumulates an anti-Hermitian part there that the
@@ -129,13 +132,17 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
       pcol = (b == 0 && col == H) ? col : col - H;
     }
   }
-  const cplx<T>* src = in + plane + ky;
+  // wave-uniform plane bases, per-lane 32-bit byte offsets (row kx, column ky)
+  const cplx<T>* src = in + plane;
+  cplx<T>* rb = rbuf + plane;
+  const uint32_t cofs = (uint32_t)ky * CB;
   cplx<T> v[E];
   // first-pass inputs straight from HBM: lanes run along y (16 contiguous complex per row)
   {
-    constexpr int R0 = (MODE == X_INV) ? inv_first<L>() : fwd_first<L>();
+    constexpr int R0 = (MODE == X_INV) ? inv_first<L, V>() : fwd_first<L, V>();
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = src[(int64_t)Tile<T, L, E, NCP>::template pos_in<R0>(tc, e) * p.ny];
+    for (int e = 0; e < E; ++e)
+      v[e] = ld_off(src, (uint32_t)Tile<T, L, E, NCP>::template pos_in<R0>(tc, e) * rowb + cofs);
   }
   // FP64 mean mode (mixed path): the block owning k = 0 sums the z-pass partials.
   double nsum = 0.0;
@@ -146,13 +153,13 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
     nsum = block_sum(a, red);  // valid in thread 0 (= col 0, tc 0)
   }
   if constexpr (MODE != X_INV) {
-    FwdFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
+    FwdFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
     const T kyv = p.ky[ky], kzv = p.kz[kz];
     if (herm) __syncthreads();  // S receives R' below; the FFT's last reads must be done
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e);
-      const int64_t idx = plane + (int64_t)kxi * p.ny + ky;
+      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
+      const uint32_t o = (uint32_t)kxi * rowb + cofs;
       if constexpr (MODE == X_FWD_SCALED) {
         v[e] = p.scale * v[e];
         if (owner0 && tid == 0 && kxi == 0) {
@@ -169,8 +176,8 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
         if constexpr (MODE == X_STEP) {
           // psi_new = (C1 psi + dt/2 (3 N - N_past))/C2 carried as R = C1 psi - dt/2 N_past
           // (pf3d.cpp:476-478): psi_new = (R + 3 dt/2 N)/C2, R' = C1 psi_new - dt/2 N.
-          const cplx<T> r = rbuf[idx];
-          const T ic2 = (T)1 / C2;
+          const cplx<T> r = ld_off<cplx<T>>(rb, o);
+          const T ic2 = recip(C2);
           cplx<T> ps = mk(ic2 * (r.x + (T)3 * p.dtd2 * nn.x), ic2 * (r.y + (T)3 * p.dtd2 * nn.y));
           cplx<T> rn = mk(C1 * ps.x - p.dtd2 * nn.x, C1 * ps.y - p.dtd2 * nn.y);
           if (owner0 && tid == 0 && kxi == 0) {
@@ -183,11 +190,11 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
             rn = mk((T)p.mean_state[0], (T)0);
           }
           if (herm) S[kxi * NCP + col] = rn;
-          else rbuf[idx] = rn;
-          if (p.rec_psi != nullptr) p.rec_psi[idx] = ps;
+          else st_off(rb, o, rn);
+          if (p.rec_psi != nullptr) st_off(p.rec_psi + plane, o, ps);
           v[e] = ps;
         } else {  // X_BOOT: R = C1 psi0 - dt/2 N_past with N_past = N(psi0) (pf3d.cpp:363-374)
-          cplx<T> ps = psi0[idx];
+          cplx<T> ps = ld_off(psi0 + plane, o);
           cplx<T> rn = mk(C1 * ps.x - p.dtd2 * nn.x, C1 * ps.y - p.dtd2 * nn.y);
           if (owner0 && tid == 0 && kxi == 0) {
             const double ps0 = p.mean_state[1];  // set from the FP64 sum of psi0
@@ -196,7 +203,7 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
             rn = mk((T)p.mean_state[0], (T)0);
           }
           if (herm) S[kxi * NCP + col] = rn;
-          else rbuf[idx] = rn;
+          else st_off(rb, o, rn);
           v[e] = ps;
         }
       }
@@ -205,20 +212,22 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass(const cplx<T>* in
       __syncthreads();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e);
+        const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
         const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
-        rbuf[plane + (int64_t)kxi * p.ny + ky] = mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y));
+        st_off(rb, (uint32_t)kxi * rowb + cofs, mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)));
       }
     }
   }
-  cplx<T>* dst = out + plane + ky;
+  cplx<T>* dst = out + plane;
   if constexpr (MODE == X_FWD_SCALED) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) dst[(int64_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e) * p.ny] = v[e];
+    for (int e = 0; e < E; ++e)
+      st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
   } else {
-    InvFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
+    InvFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
 #pragma unroll
-    for (int e = 0; e < E; ++e) dst[(int64_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L>()>(tc, e) * p.ny] = v[e];
+    for (int e = 0; e < E; ++e)
+      st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
   }
 }
 
@@ -261,57 +270,60 @@ enum ZIn { Z_IN_SPEC = 0, Z_IN_REAL = 1 };
 enum ZOut { Z_OUT_SPEC = 1, Z_OUT_REAL = 2 };
 enum ZOp { Z_OP_ID = 0, Z_OP_NL = 1 };
 
-template <typename T, int L, int NC, int IN, int OUT>
-__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cplx<T>* in, cplx<T>* out,
+template <typename T, int L, int NC, int IN, int OUT, int V = 0>
+__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const cplx<T>* in, cplx<T>* out,
                                                                        const double* __restrict__ real_in,
                                                                        double* __restrict__ real_out,
                                                                        int op, PParams<T> p) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
+  // The grid covers whole tiles: nx*ny is a multiple of NC (every plan length is a
+  // multiple of 8 and the launcher checks it).
+  constexpr int E = PlanV<L, V>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
   constexpr int NZH = L / 2 + 1, FP = NPR + 1;
+  constexpr uint32_t CB = sizeof(cplx<T>);
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   const int64_t f0 = (int64_t)blockIdx.x * NC;
-  const int nvalid = (int)((p.nxy - f0) < NC ? (p.nxy - f0) : NC);
-  const bool va = q < nvalid, vb = q + NPR < nvalid;
+  const uint32_t rowb = (uint32_t)p.nxy * CB;  // bytes between kz rows of one column
   cplx<T> v[E];
   if constexpr (IN == Z_IN_SPEC) {
     // Hermitian extension of the column pair read straight from HBM; only the real
     // parts of kz = 0 and kz = nz/2 enter (the half-complex stage of an FFTW c2r).
-    const cplx<T>* col = in + f0 + q;
+    const cplx<T>* col = in + f0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
+      const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(tc, e);
       const int m = (n <= L / 2) ? n : L - n;
-      cplx<T> A = va ? col[(int64_t)m * p.nxy] : mk<T>(0, 0);
-      cplx<T> B = vb ? col[(int64_t)m * p.nxy + NPR] : mk<T>(0, 0);
+      const uint32_t o = (uint32_t)m * rowb + (uint32_t)q * CB;
+      cplx<T> A = ld_off(col, o);
+      cplx<T> B = ld_off(col, o + NPR * CB);
       if (n == 0 || n == L / 2) {
         A.y = 0;
         B.y = 0;
       }
       v[e] = (n <= L / 2) ? mk(A.x - B.y, A.y + B.x) : mk(A.x + B.y, B.x - A.y);
     }
-    InvFFT<T, L, FP>::run(v, F, q, tc, p.twz);
+    InvFFT<T, L, FP, V>::run(v, F, q, tc, p.twz);
   } else {
     for (int i = tid; i < NC * L; i += NT) {
       const int c = i / L, n = i - c * L;
-      const T val = (c < nvalid) ? (T)real_in[(f0 + c) * L + n] : (T)0;
+      const T val = (T)real_in[(f0 + c) * L + n];
       if (c < NPR) F[n * FP + c].x = val;
       else F[n * FP + c - NPR].y = val;
     }
     __syncthreads();
-    Tile<T, L, E, FP>::template load<fwd_first<L>()>(v, F, q, tc);
+    Tile<T, L, E, FP>::template load<fwd_first<L, V>()>(v, F, q, tc);
   }
   // v now holds (psi_a, psi_b) at positions pos_in<fwd_first>.
-  constexpr int RP = fwd_first<L>();
-  static_assert(fwd_first<L>() == inv_last<L>(), "plan symmetry");
+  constexpr int RP = fwd_first<L, V>();
+  static_assert(fwd_first<L, V>() == inv_last<L, V>(), "plan symmetry");
   if constexpr ((OUT & Z_OUT_REAL) != 0) {
     __syncthreads();
     Tile<T, L, E, FP>::template store<RP>(v, F, q, tc);
     __syncthreads();
     for (int i = tid; i < NC * L; i += NT) {
       const int c = i / L, n = i - c * L;
-      if (c < nvalid) real_out[(f0 + c) * L + n] = (double)(c < NPR ? F[n * FP + c].x : F[n * FP + c - NPR].y);
+      real_out[(f0 + c) * L + n] = (double)(c < NPR ? F[n * FP + c].x : F[n * FP + c - NPR].y);
     }
   }
   if constexpr ((OUT & Z_OUT_SPEC) != 0) {
@@ -326,8 +338,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cpl
       for (int e = 0; e < E; ++e) {
         const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
         T* row = p.rec_np + (int64_t)n * p.nxy + f0;
-        if (va) row[q] = nonlin_prime(p, v[e].x);
-        if (vb) row[q + NPR] = nonlin_prime(p, v[e].y);
+        row[q] = nonlin_prime(p, v[e].x);
+        row[q + NPR] = nonlin_prime(p, v[e].y);
       }
     }
     double acc = 0.0;
@@ -350,17 +362,18 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_zpass(const cpl
       const double s = block_sum(acc, red);
       if (tid == 0) p.partial[blockIdx.x] = s;
     }
-    FwdFFT<T, L, FP>::run(v, F, q, tc, p.twz);
+    FwdFFT<T, L, FP, V>::run(v, F, q, tc, p.twz);
     __syncthreads();
-    Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
+    Tile<T, L, E, FP>::template store<fwd_last<L, V>()>(v, F, q, tc);
     __syncthreads();
     // split W = X + iY:  X = (W_k + conj W_{-k})/2,  Y = (W_k - conj W_{-k})/(2i); straight to HBM
     cplx<T>* dst = out + f0;
     for (int i = tid; i < NZH * NPR; i += NT) {
       const int k = i / NPR, qq = i - k * NPR;
       const cplx<T> W = F[k * FP + qq], Wm = F[((L - k) % L) * FP + qq];
-      if (qq < nvalid) dst[(int64_t)k * p.nxy + qq] = mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y));
-      if (qq + NPR < nvalid) dst[(int64_t)k * p.nxy + qq + NPR] = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
+      const uint32_t o = (uint32_t)k * rowb + (uint32_t)qq * CB;
+      st_off(dst, o, mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y)));
+      st_off(dst, o + NPR * CB, mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x)));
     }
   }
 }

@@ -142,7 +142,7 @@ void preset(Opts& o, const std::string& name, int n) {
     o.save_every = 10;
     return;
   }
-  // BASELINE-style cubic presets with 8 points per wavelength
+  // cubic-model presets (beta = 0, gamma = 1) with 8 grid points per wavelength
   const double L = n / 8.0 * 2 * M_PI;
   o.c.lx = o.c.ly = o.c.lz = L;
   o.c.beta = 0.0;

@@ -125,6 +125,21 @@ class Solver:
         check(lib().smx_calibration_copy(self._h, int(nbytes), int(reps), C.byref(v)))
         return v.value
 
+    def schedule(self):
+        """[(nc, variant)] of the step kernels (y inverse, z, y forward, x)."""
+        s = _abi.SmxSchedule()
+        check(lib().smx_get_schedule(self._h, C.byref(s)))
+        return s.as_list()
+
+    def set_schedule(self, slots):
+        check(lib().smx_set_schedule(self._h, C.byref(_abi.SmxSchedule.from_list(slots))))
+
+    def autotune(self, reps=20):
+        """Time every candidate tile schedule and install the fastest (state untouched)."""
+        s = _abi.SmxSchedule()
+        check(lib().smx_autotune(self._h, int(reps), C.byref(s)))
+        return s.as_list()
+
 
 class Batch:
     """Independent boxes of uneven sizes advanced together through one captured hipGraph."""
@@ -147,6 +162,12 @@ class Batch:
 
     def run(self, nsteps, graph_steps=10):
         check(lib().smx_batch_run(self._b, int(nsteps), int(graph_steps)))
+
+    def autotune(self, reps=20):
+        """Autotune every box's tile schedules (one tuning per box shape); the next run
+        captures the graph again."""
+        check(lib().smx_batch_autotune(self._b, int(reps)))
+        return [b.schedule() for b in self.boxes]
 
     def graph_nodes(self):
         n = C.c_int64()
