@@ -57,10 +57,13 @@ def hyperboloid_ic(n, dx, q0=1.0, a=0.4, noise=1e-3, seed=0):
     return psi + noise * (2.0 * u.reshape(n, n, n) - 1.0)
 
 
-def run_path(precision, psi0, steps, warmup, prof_steps):
+def run_path(precision, psi0, steps, warmup, prof_steps, tune=True):
     from smectic3d_amd import Solver
     cfg = workload_cfg(precision)
     s = Solver(cfg)
+    t0 = time.perf_counter()
+    sched = s.autotune(20) if tune else s.schedule()
+    tune_s = time.perf_counter() - t0
     s.set_psi(psi0)
     s.step(warmup)
     s.sync()
@@ -68,12 +71,18 @@ def run_path(precision, psi0, steps, warmup, prof_steps):
     s.step(steps)
     s.sync()
     t1 = time.perf_counter()
-    out = {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps}
+    out = {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps,
+           "schedule": sched, "tune_s": tune_s}
     out["psi"] = s.get_psi()
     out["energy"] = s.energy()
     out["qdom"] = s.dominant_wavenumber()
     out["l1"] = s.l1()
     out["kernels"] = s.profile_step(prof_steps)  # advances the state; after the final readouts
+    if precision == 1:
+        # plain float4 copy of one stage array: the achievable-bandwidth yardstick
+        n_spec_bytes = cfg.nx * cfg.ny * (cfg.nz // 2 + 1) * 8
+        out["copy_GBs"] = 2 * n_spec_bytes / (s.calibration_copy(n_spec_bytes, 50) * 1e-3) / 1e9
+    s.close()
     return out
 
 
@@ -131,7 +140,7 @@ def run_advected(steps):
     return {"steps": steps, "steps_per_s": steps / (t1 - t0), "energy": e}
 
 
-def run_batch(steps):
+def run_batch(steps, tune=True):
     """24 independent periodic boxes, sizes from {64^3, 96^3, 128^3} by seed, eps ~ U[0.2,0.6],
     q0 ~ U[0.8,1.2], gamma=1, dt=0.05, noise 0.05, advanced in one captured hipGraph (FP32-mixed)."""
     from smectic3d_amd import Batch
@@ -146,6 +155,9 @@ def run_batch(steps):
                                   nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED))
     pts = sum(c.nx * c.ny * c.nz for c in cfgs)
     with Batch(cfgs) as b:
+        t0 = time.perf_counter()
+        scheds = b.autotune(10) if tune else [s.schedule() for s in b.boxes]
+        tune_s = time.perf_counter() - t0
         for i, (s, c) in enumerate(zip(b.boxes, cfgs)):
             s.set_psi(np.random.default_rng(i).uniform(-0.05, 0.05, (c.nx, c.ny, c.nz)))
         b.run(10, graph_steps=10)
@@ -155,7 +167,8 @@ def run_batch(steps):
         nodes = b.graph_nodes()
     return {"boxes": 24, "sizes": sorted(c.nx for c in cfgs), "steps": steps,
             "batch_steps_per_s": steps / (t1 - t0), "box_steps_per_s": 24 * steps / (t1 - t0),
-            "points_per_s": pts * steps / (t1 - t0), "graph_nodes": nodes}
+            "points_per_s": pts * steps / (t1 - t0), "graph_nodes": nodes, "tune_s": tune_s,
+            "schedules": {f"{c.nx}^3": sc for c, sc in zip(cfgs, scheds)}}
 
 
 def run_adjoint(steps, ckpt):
@@ -210,6 +223,7 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the Neumann/advected/batch/adjoint legs")
+    ap.add_argument("--no-tune", action="store_true", help="default tile schedules (no autotuning)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
@@ -221,9 +235,9 @@ def main():
     dx = (n / 8 * 2 * math.pi) / n
     psi0 = hyperboloid_ic(n, dx)
     log(f"workload 256^3, steps={a.steps} warmup={a.warmup}")
-    r32 = run_path(A.SMX_FP32_MIXED, psi0, a.steps, a.warmup, a.prof_steps)
-    log(f"fp32-mixed: {r32['steps_per_s']:.1f} steps/s")
-    r64 = run_path(A.SMX_FP64, psi0, a.steps, a.warmup, a.prof_steps)
+    r32 = run_path(A.SMX_FP32_MIXED, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune)
+    log(f"fp32-mixed: {r32['steps_per_s']:.1f} steps/s schedule {r32['schedule']}")
+    r64 = run_path(A.SMX_FP64, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune)
     log(f"fp64: {r64['steps_per_s']:.1f} steps/s")
 
     err_psi = float(np.linalg.norm(r32["psi"] - r64["psi"]) / np.linalg.norm(r64["psi"]))
@@ -256,7 +270,13 @@ def main():
                         "gamma=1, dt=0.1, cubic NL, CNAB2 (FP32-mixed headline; FP64 beside it)",
             "grid": [n, n, n], "precision": "fp32-mixed",
         },
-        "fp64": {"value": r64["steps_per_s"], "unit": "steps/s", "ms_per_step": r64["ms_per_step"]},
+        "fp64": {"value": r64["steps_per_s"], "unit": "steps/s", "ms_per_step": r64["ms_per_step"],
+                 "schedule": r64["schedule"]},
+        "schedule_fp32": r32["schedule"],
+        "schedule_note": "(columns per tile, plan variant) of the y-inverse, z, y-forward and x kernels, "
+                         "chosen by smx_autotune before the timed region",
+        "autotune_s": {"fp32": r32["tune_s"], "fp64": r64["tune_s"]},
+        "copy_kernel_GBs": r32["copy_GBs"],
         "speedup_fp32_mixed_vs_fp64": r32["steps_per_s"] / r64["steps_per_s"],
         "error_fp32_mixed_vs_fp64": {"psi_rel_l2": err_psi, "energy_rel": err_e, "qdom_rel": err_q,
                                      "steps": a.steps + a.warmup},
@@ -276,7 +296,7 @@ def main():
         log("advected ...")
         line["advected_256_fp32"] = run_advected(1000)
         log("batch ...")
-        line["batch_24_boxes_fp32"] = run_batch(500)
+        line["batch_24_boxes_fp32"] = run_batch(500, tune=not a.no_tune)
         log("adjoint ...")
         line["adjoint_256_fp32"] = run_adjoint(2000, 50)
     if not a.no_cpu:

@@ -155,6 +155,15 @@ int smx_surftrack(smx_handle *h, const smx_surf_opts *opts, double *surfz, doubl
  */
 int smx_adjoint_grad(smx_handle *h, const double *psi0, int64_t row_stride, int64_t nsteps, int64_t ckpt_every,
                      double target_amp, double *loss, double *dpsi0, double *deps, double *dq0);
+/* Device memory the adjoint may spend on keeping recorded segments resident (N'(psi_n) and
+ * psi_hat_n of whole checkpoint segments, recorded during the forward sweep and never
+ * recomputed).  bytes < 0: 60% of the free device memory at the time of the call (default);
+ * 0: checkpoints only (every segment recomputed once).  The gradient does not depend
+ * on it.  No counterpart in the reference. */
+int smx_set_adjoint_budget(smx_handle *h, int64_t bytes);
+/* Bookkeeping of the last smx_adjoint_grad: steps recorded in the forward sweep, steps
+ * recomputed from checkpoints, device bytes allocated for the sweep. */
+int smx_adjoint_info(const smx_handle *h, int64_t *resident_steps, int64_t *recomputed_steps, int64_t *device_bytes);
 
 /*
  * Ensembles of independent boxes of uneven sizes (parameter sweeps).  Each box is

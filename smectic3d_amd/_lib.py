@@ -17,7 +17,7 @@ EXPORTS = [
     "smx_dtpsi", "smx_surftrack", "smx_sync", "smx_step_count", "smx_profile_step",
     "smx_calibration_copy", "smx_adjoint_grad", "smx_batch_create", "smx_batch_destroy",
     "smx_batch_handle", "smx_batch_run", "smx_batch_graph_nodes", "smx_get_schedule", "smx_set_schedule",
-    "smx_autotune", "smx_batch_autotune",
+    "smx_autotune", "smx_batch_autotune", "smx_set_adjoint_budget", "smx_adjoint_info",
 ]
 
 _lib = None
@@ -69,6 +69,8 @@ def lib():
     L.smx_set_schedule.argtypes = [vp, C.POINTER(SmxSchedule)]
     L.smx_autotune.argtypes = [vp, C.c_int, C.POINTER(SmxSchedule)]
     L.smx_batch_autotune.argtypes = [vp, C.c_int]
+    L.smx_set_adjoint_budget.argtypes = [vp, C.c_int64]
+    L.smx_adjoint_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     for n in EXPORTS:
         f = getattr(L, n)
         if n not in ("smx_config_default", "smx_destroy", "smx_last_error", "smx_batch_destroy"):

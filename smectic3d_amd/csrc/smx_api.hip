@@ -96,6 +96,8 @@ struct smx_handle {
   bool has_state = false, has_prev = false;
   int64_t steps = 0;
   PassSched sched[NSLOTS];  // tile schedules of the periodic step kernels
+  int64_t adj_budget = -1;  // device bytes the adjoint may keep resident (-1: 60% of free memory)
+  int64_t adj_info[3] = {0, 0, 0};  // last adjoint: resident steps, recomputed steps, device bytes
 };
 
 // FP64 partial sums of one z pass: one per tile at the narrowest tile (8 columns), plus
@@ -559,11 +561,19 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
   const smx_config& c = h->cfg;
   const int64_t nseg = (T_ + K - 1) / K;
   const size_t sbytes = h->wbytes * (size_t)h->nspec;
+  const size_t npbytes = sizeof(T) * (size_t)h->N;
+  // One record set holds a segment's N'(psi_n) (K real fields, layout [z][x][y]) and its
+  // Psi_n = psi_hat_n (K+1 spectra).  Segments whose sets stay resident in HBM are recorded
+  // during the forward sweep and never recomputed; the others are checkpointed (A, R and
+  // the FP64 mean mode) and recomputed into set 0 during the reverse sweep.
+  const size_t setbytes = npbytes * (size_t)K + sbytes * (size_t)(K + 1);
   std::vector<void*> tmp;
   auto talloc = [&](void** pp, size_t bytes) -> int {
     hipError_t e = hipMalloc(pp, bytes);
-    if (e != hipSuccess) return fail(SMX_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + ") for the adjoint: " +
-                                                      hipGetErrorString(e));
+    if (e != hipSuccess) {
+      *pp = nullptr;
+      return fail(SMX_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + ") for the adjoint: " + hipGetErrorString(e));
+    }
     tmp.push_back(*pp);
     return SMX_OK;
   };
@@ -571,18 +581,53 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     (void)hipStreamSynchronize(h->st);
     for (void* q : tmp) (void)hipFree(q);
   };
-  char *ck = nullptr, *recpsi = nullptr, *recnp = nullptr;
+  int64_t budget = h->adj_budget;
+  if (budget < 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    budget = (int64_t)(0.6 * (double)fr);
+  }
+  const int64_t nres_want = std::min<int64_t>(nseg, budget / (int64_t)setbytes);
   void *AS = nullptr, *AC = nullptr;
   double *ckmean = nullptr, *acc = nullptr;
   const int nxb = c.ny / kNC, nblk = nxb * h->nplanes;
   int rc = SMX_OK;
-  if ((rc = talloc((void**)&ck, 2 * sbytes * nseg)) || (rc = talloc((void**)&ckmean, sizeof(double) * 2 * nseg)) ||
-      (rc = talloc((void**)&recpsi, sbytes * (K + 1))) ||
-      (rc = talloc((void**)&recnp, sizeof(T) * (size_t)h->N * K)) || (rc = talloc(&AS, sbytes)) ||
-      (rc = talloc(&AC, sbytes)) || (rc = talloc((void**)&acc, sizeof(double) * 2 * nblk))) {
+  if ((rc = talloc(&AS, sbytes)) || (rc = talloc(&AC, sbytes)) ||
+      (rc = talloc((void**)&acc, sizeof(double) * 2 * nblk)) ||
+      (rc = talloc((void**)&ckmean, sizeof(double) * 2 * nseg))) {
     cleanup();
     return rc;
   }
+  // record sets, as many as the budget allows (at least one for the recomputed segments)
+  std::vector<char*> setnp, setpsi;
+  for (int64_t i = 0; i < std::max<int64_t>(nres_want, 1); ++i) {
+    void *a = nullptr, *b = nullptr;
+    if (talloc(&a, npbytes * (size_t)K) != SMX_OK || talloc(&b, sbytes * (size_t)(K + 1)) != SMX_OK) {
+      (void)hipGetLastError();
+      if (a) {
+        (void)hipFree(a);
+        tmp.pop_back();
+      }
+      if (i == 0) {
+        cleanup();
+        return fail(SMX_ERR_HIP, "no device memory for one adjoint record set (" + std::to_string(setbytes) + " B)");
+      }
+      break;
+    }
+    setnp.push_back((char*)a);
+    setpsi.push_back((char*)b);
+  }
+  const int64_t nres = std::min<int64_t>(nres_want, (int64_t)setnp.size());
+  const int64_t sres = nseg - nres;  // segments [sres, nseg) are resident
+  char* ck = nullptr;
+  if (sres > 0 && (rc = talloc((void**)&ck, 2 * sbytes * (size_t)sres))) {
+    cleanup();
+    return rc;
+  }
+  h->adj_info[1] = std::min<int64_t>(T_, sres * K);  // recomputed steps
+  h->adj_info[0] = T_ - h->adj_info[1];               // resident steps
+  h->adj_info[2] = (int64_t)(2 * sbytes + sizeof(double) * (2 * nblk + 2 * nseg) + setbytes * setnp.size() +
+                             (ck ? 2 * sbytes * (size_t)sres : 0));
   PParams<T> p = params<T>(h, h->fp32);
   auto save = [&](int64_t seg) -> int {
     HIPCHK(hipMemcpyAsync(ck + 2 * sbytes * seg, h->A, sbytes, hipMemcpyDeviceToDevice, h->st));
@@ -596,12 +641,29 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     HIPCHK(hipMemcpyAsync(h->mean_state, ckmean + 2 * seg, 2 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
     return SMX_OK;
   };
+  // steps [s0, s0+len) from the current state, recording N'(psi_n) and Psi_n into set `set`
+  auto record = [&](int64_t set, int64_t len) -> int {
+    PParams<T> px = params<T>(h, false);
+    px.scale = (T)(1.0 / (double)c.nx);  // psi_hat_{s0} from the stage buffer
+    CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, setpsi[set], nullptr, nullptr));
+    for (int64_t j = 0; j < len; ++j) {
+      PParams<T> pr = p;
+      pr.rec_np = (T*)(setnp[set] + npbytes * j);
+      pr.rec_psi = (cplx<T>*)(setpsi[set] + sbytes * (j + 1));
+      CHK(step_once<T>(h, pr, nullptr));
+    }
+    return SMX_OK;
+  };
   auto run = [&]() -> int {
-    // forward sweep with checkpoints
+    // forward sweep: checkpoints for the recomputed head, records for the resident tail
     for (int64_t seg = 0; seg < nseg; ++seg) {
-      CHK(save(seg));
       const int64_t len = std::min(K, T_ - seg * K);
-      for (int64_t j = 0; j < len; ++j) CHK(step_once<T>(h, p, nullptr));
+      if (seg < sres) {
+        CHK(save(seg));
+        for (int64_t j = 0; j < len; ++j) CHK(step_once<T>(h, p, nullptr));
+      } else {
+        CHK(record(seg - sres, len));
+      }
     }
     // a_T = psi_T - g, the loss and the explicit q0 term of the target
     CHK(c2r_state(h, h->real_io));
@@ -616,31 +678,28 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     CHK(zpass<T>(h, pa, Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, AS, h->real_io, nullptr));
     CHK(ypass<T>(h, AS, AS, -1));
     HIPCHK(hipMemsetAsync(acc, 0, sizeof(double) * 2 * nblk, h->st));
-    HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, nullptr, nullptr, pa, (T)(T_ - 1 >= 1 ? 3 : 2), 1,
+    HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, nullptr, nullptr, pa, (T)(T_ - 1 >= 1 ? 3 : 2), 1, 0,
                                     c.q0, c.alpha, acc, h->st));
-    CHK(herm<T>(h, pa, AC));
-    // reverse sweep over the segments, recomputing each one with recording
+    // reverse sweep over the segments
     for (int64_t seg = nseg - 1; seg >= 0; --seg) {
       const int64_t s0 = seg * K, len = std::min(K, T_ - s0);
-      CHK(restore(seg));
-      PParams<T> px = params<T>(h, false);
-      px.scale = (T)(1.0 / (double)c.nx);  // psi_hat_{s0} from the stage buffer
-      CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, recpsi, nullptr, nullptr));
-      for (int64_t j = 0; j < len; ++j) {
-        PParams<T> pr = p;
-        pr.rec_np = (T*)(recnp + sizeof(T) * (size_t)h->N * j);
-        pr.rec_psi = (cplx<T>*)(recpsi + sbytes * (j + 1));
-        CHK(step_once<T>(h, pr, nullptr));
+      int64_t set = seg - sres;
+      if (seg < sres) {
+        set = 0;
+        CHK(restore(seg));
+        CHK(record(0, len));
       }
       for (int64_t j = len - 1; j >= 0; --j) {
         const int64_t n = s0 + j;
-        CHK(ypass<T>(h, AS, AS, +1));
-        HIPCHK(AdjointOps<T>::zpass_adj((cplx<T>*)AS, (const T*)(recnp + sizeof(T) * (size_t)h->N * j), pa, h->st));
-        CHK(ypass<T>(h, AS, AS, -1));
-        HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, (const cplx<T>*)(recpsi + sbytes * j),
-                                        (const cplx<T>*)(recpsi + sbytes * (j + 1)), pa, (T)(n - 1 >= 1 ? 3 : 2), 0,
-                                        c.q0, c.alpha, acc, h->st));
-        CHK(herm<T>(h, pa, AC));
+        PParams<T> pm = pa;
+        pm.mul_np = (const T*)(setnp[set] + npbytes * j);
+        CHK(ypass<T>(h, AS, AS, +1, h->sched[SLOT_YINV]));
+        CHK(zpass<T>(h, pm, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_MUL, AS, AS, nullptr, nullptr, h->sched[SLOT_Z]));
+        CHK(ypass<T>(h, AS, AS, -1, h->sched[SLOT_YFWD]));
+        const cplx<T>* psin = (const cplx<T>*)(setpsi[set] + sbytes * j);
+        const cplx<T>* psin1 = (n == T_ - 1) ? (const cplx<T>*)(setpsi[set] + sbytes * (j + 1)) : nullptr;
+        HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, psin, psin1, pa, (T)(n - 1 >= 1 ? 3 : 2), 0,
+                                        n >= 1 ? 1 : 0, c.q0, c.alpha, acc, h->st));
       }
     }
     // dJ/dpsi0 = F^-1 alpha_0 (FP64 inverse chain)
@@ -664,7 +723,6 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     *loss = tsum[0];
     *deps = (double)h->N * se;  // Parseval: sum_x a b = N sum_k w Re(conj alpha beta)
     *dq0 = (double)h->N * sq + tsum[1];
-    CHK(restore(0));
     return SMX_OK;
   };
   rc = run();
@@ -1117,11 +1175,23 @@ int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int6
   HIPCHK(hipMemcpy2DAsync(dpsi0, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz, sizeof(double) * c.nz,
                           (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  h->steps = 0;
-  h->has_prev = false;
+  return smx_set_psi(h, psi0, row_stride);  // leave the handle at psi0
+}
+
+
+int smx_set_adjoint_budget(smx_handle* h, int64_t bytes) {
+  if (!h) return fail(SMX_ERR_INVALID, "null handle");
+  h->adj_budget = bytes < 0 ? -1 : bytes;
   return SMX_OK;
 }
 
+int smx_adjoint_info(const smx_handle* h, int64_t* resident_steps, int64_t* recomputed_steps, int64_t* device_bytes) {
+  if (!h || !resident_steps || !recomputed_steps || !device_bytes) return fail(SMX_ERR_INVALID, "null argument");
+  *resident_steps = h->adj_info[0];
+  *recomputed_steps = h->adj_info[1];
+  *device_bytes = h->adj_info[2];
+  return SMX_OK;
+}
 
 int smx_get_schedule(const smx_handle* h, smx_schedule* s) {
   if (!h || !s) return fail(SMX_ERR_INVALID, "null argument");

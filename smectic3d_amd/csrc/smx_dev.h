@@ -25,6 +25,12 @@ struct __attribute__((aligned(2 * sizeof(T)))) cplx {
   T x, y;
 };
 
+// two adjacent complex values moved as one access (16 B in FP32, 32 B in FP64)
+template <typename T>
+struct __attribute__((aligned(4 * sizeof(T)))) cplx2 {
+  cplx<T> a, b;
+};
+
 template <typename T> __device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> r; r.x = a; r.y = b; return r; }
 template <typename T> __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.x + b.x, a.y + b.y); }
 template <typename T> __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.x - b.x, a.y - b.y); }

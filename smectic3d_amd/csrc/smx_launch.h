@@ -274,26 +274,13 @@ constexpr int nc_fit(int ntiles) {
 
 template <typename T>
 hipError_t AdjointOps<T>::xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T>* psin, const cplx<T>* psin1,
-                                    const PParams<T>& p, T cnext, int init, double q0d, double alphad, double* acc,
-                                    hipStream_t st) {
+                                    const PParams<T>& p, T cnext, int init, int tail, double q0d, double alphad,
+                                    double* acc, hipStream_t st) {
   const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
 #define SMX_CASE(L)                                                                                      \
   if (p.nx == L) {                                                                                       \
     hipLaunchKernelGGL((k_xpass_adj<T, L, kNC>), grid, dim3(kNC * (L / Plan<L>::E)), 0, st, stage, carry, psin, \
-                       psin1, p, cnext, init, q0d, alphad, acc);                                         \
-    return hipGetLastError();                                                                            \
-  }
-  SMX_FOR_EACH_LENGTH(SMX_CASE)
-#undef SMX_CASE
-  return hipErrorInvalidValue;
-}
-
-template <typename T>
-hipError_t AdjointOps<T>::zpass_adj(cplx<T>* stage, const T* nprime, const PParams<T>& p, hipStream_t st) {
-  const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
-#define SMX_CASE(L)                                                                                      \
-  if (p.nz == L) {                                                                                       \
-    hipLaunchKernelGGL((k_zpass_adj<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, stage, nprime, p); \
+                       psin1, p, cnext, init, tail, q0d, alphad, acc);                                   \
     return hipGetLastError();                                                                            \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)

@@ -41,6 +41,9 @@ struct PParams {
   // [z][x][y]; psi_hat_{n+1} by the x pass, layout of R.
   T* rec_np = nullptr;
   cplx<T>* rec_psi = nullptr;
+  // Multiplier of the z pass in the Z_OP_MUL form (the adjoint sweep: the recorded N'(psi_n)),
+  // layout [z][x][y].
+  const T* mul_np = nullptr;
 };
 
 // dN/dpsi: 3 beta psi^2 - 5 gamma psi^4 (quintic) or 2 beta psi - 3 gamma psi^2 (cubic).
@@ -264,11 +267,12 @@ __global__ void k_herm_project(cplx<T>* __restrict__ a, int nx, int ny, int nzh,
 }
 
 // ---------------------------------------------------------------------------
-// z pass.  Tile: columns f0 .. f0+NC-1 of the (x,y) plane; column pairs
-// (q, q + NC/2) share one length-nz complex FFT (z = psi_a + i psi_b).
+// z pass.  Tile: columns f0 .. f0+NC-1 of the (x,y) plane; adjacent columns
+// (2q, 2q+1) share one length-nz complex FFT (z = psi_a + i psi_b), so that a lane
+// moves both with one 2-complex access per row (a full 128-byte row per 8 lanes).
 enum ZIn { Z_IN_SPEC = 0, Z_IN_REAL = 1 };
 enum ZOut { Z_OUT_SPEC = 1, Z_OUT_REAL = 2 };
-enum ZOp { Z_OP_ID = 0, Z_OP_NL = 1 };
+enum ZOp { Z_OP_ID = 0, Z_OP_NL = 1, Z_OP_MUL = 2 };
 
 template <typename T, int L, int NC, int IN, int OUT, int V = 0>
 __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const cplx<T>* in, cplx<T>* out,
@@ -285,18 +289,18 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * CB;  // bytes between kz rows of one column
+  const uint32_t qofs = (uint32_t)q * 2 * CB;  // the lane's column pair within a row
   cplx<T> v[E];
   if constexpr (IN == Z_IN_SPEC) {
     // Hermitian extension of the column pair read straight from HBM; only the real
     // parts of kz = 0 and kz = nz/2 enter (the half-complex stage of an FFTW c2r).
-    const cplx<T>* col = in + f0;
+    const cplx2<T>* col = reinterpret_cast<const cplx2<T>*>(in + f0);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(tc, e);
       const int m = (n <= L / 2) ? n : L - n;
-      const uint32_t o = (uint32_t)m * rowb + (uint32_t)q * CB;
-      cplx<T> A = ld_off(col, o);
-      cplx<T> B = ld_off(col, o + NPR * CB);
+      const cplx2<T> ab = ld_off(col, (uint32_t)m * rowb + qofs);
+      cplx<T> A = ab.a, B = ab.b;
       if (n == 0 || n == L / 2) {
         A.y = 0;
         B.y = 0;
@@ -308,8 +312,8 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     for (int i = tid; i < NC * L; i += NT) {
       const int c = i / L, n = i - c * L;
       const T val = (T)real_in[(f0 + c) * L + n];
-      if (c < NPR) F[n * FP + c].x = val;
-      else F[n * FP + c - NPR].y = val;
+      if ((c & 1) == 0) F[n * FP + (c >> 1)].x = val;
+      else F[n * FP + (c >> 1)].y = val;
     }
     __syncthreads();
     Tile<T, L, E, FP>::template load<fwd_first<L, V>()>(v, F, q, tc);
@@ -323,7 +327,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     __syncthreads();
     for (int i = tid; i < NC * L; i += NT) {
       const int c = i / L, n = i - c * L;
-      real_out[(f0 + c) * L + n] = (double)(c < NPR ? F[n * FP + c].x : F[n * FP + c - NPR].y);
+      real_out[(f0 + c) * L + n] = (double)((c & 1) == 0 ? F[n * FP + (c >> 1)].x : F[n * FP + (c >> 1)].y);
     }
   }
   if constexpr ((OUT & Z_OUT_SPEC) != 0) {
@@ -337,15 +341,19 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
-        T* row = p.rec_np + (int64_t)n * p.nxy + f0;
-        row[q] = nonlin_prime(p, v[e].x);
-        row[q + NPR] = nonlin_prime(p, v[e].y);
+        T* row = p.rec_np + (int64_t)n * p.nxy + f0 + 2 * q;
+        row[0] = nonlin_prime(p, v[e].x);
+        row[1] = nonlin_prime(p, v[e].y);
       }
     }
     double acc = 0.0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      if (op == Z_OP_NL) {
+      if (op == Z_OP_MUL) {
+        const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
+        const T* row = p.mul_np + (int64_t)n * p.nxy + f0 + 2 * q;
+        v[e] = mk(row[0] * v[e].x, row[1] * v[e].y);
+      } else if (op == Z_OP_NL) {
         const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
         T a = nonlin(p, v[e].x), b = nonlin(p, v[e].y);
         if (pen && n == p.mid) {
@@ -367,13 +375,14 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     Tile<T, L, E, FP>::template store<fwd_last<L, V>()>(v, F, q, tc);
     __syncthreads();
     // split W = X + iY:  X = (W_k + conj W_{-k})/2,  Y = (W_k - conj W_{-k})/(2i); straight to HBM
-    cplx<T>* dst = out + f0;
+    cplx2<T>* dst = reinterpret_cast<cplx2<T>*>(out + f0);
     for (int i = tid; i < NZH * NPR; i += NT) {
       const int k = i / NPR, qq = i - k * NPR;
       const cplx<T> W = F[k * FP + qq], Wm = F[((L - k) % L) * FP + qq];
-      const uint32_t o = (uint32_t)k * rowb + (uint32_t)qq * CB;
-      st_off(dst, o, mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y)));
-      st_off(dst, o + NPR * CB, mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x)));
+      cplx2<T> xy;
+      xy.a = mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y));
+      xy.b = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
+      st_off(dst, (uint32_t)k * rowb + (uint32_t)qq * 2 * CB, xy);
     }
   }
 }

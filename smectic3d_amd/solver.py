@@ -120,6 +120,16 @@ class Solver:
                                      float(target_amp), C.byref(J), _ptr(d), C.byref(de), C.byref(dq)))
         return J.value, d, de.value, dq.value
 
+    def set_adjoint_budget(self, nbytes):
+        """Device bytes the adjoint may keep resident (-1: 60% of free memory; 0: checkpoints only)."""
+        check(lib().smx_set_adjoint_budget(self._h, int(nbytes)))
+
+    def adjoint_info(self):
+        """(resident steps, recomputed steps, device bytes) of the last adjoint_grad."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().smx_adjoint_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def calibration_copy(self, nbytes, reps=20):
         v = C.c_double()
         check(lib().smx_calibration_copy(self._h, int(nbytes), int(reps), C.byref(v)))

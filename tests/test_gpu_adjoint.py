@@ -116,3 +116,37 @@ def test_adjoint_rejects_unsupported():
     with Solver(cfg) as s:
         with pytest.raises(SmxError):
             s.adjoint_grad(O.ic_noise(16, 16, 16, 1, 0.1), 5, 2)
+
+
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+def test_adjoint_memory_budget_does_not_change_the_gradient(prec):
+    """Resident record sets (forward sweep records, no recompute) against checkpoints only
+    (every segment recomputed once): the same kernels on the same states, so the
+    gradient is identical; and a partial budget in between."""
+    shape, T, K = (32, 32, 48), 23, 5
+    say("adjoint budget", prec)
+    cfg = base_cfg(shape, prec)
+    psi = O.ic_noise(*shape, 13, 0.2)
+    out = {}
+    with Solver(cfg) as s:
+        n = 32 * 32 * 48
+        set_bytes = (4 if prec else 8) * n * K + (8 if prec else 16) * 32 * 32 * 25 * (K + 1)
+        for name, budget in (("auto", -1), ("none", 0), ("two", 2 * set_bytes)):
+            s.set_adjoint_budget(budget)
+            out[name] = s.adjoint_grad(psi, T, K)
+            res, rec, nbytes = s.adjoint_info()
+            say("  ", name, "resident", res, "recomputed", rec, "bytes", nbytes)
+            assert res + rec == T
+            if name == "auto":
+                assert rec == 0
+            elif name == "none":
+                assert res == 0
+            else:  # the last two segments (3 + K steps) resident
+                assert res == 3 + K
+        back = s.get_psi()  # the handle is left at psi0 (up to the transform round trip)
+        assert np.linalg.norm(back - psi) < (1e-14 if prec == A.SMX_FP64 else 2e-6) * np.linalg.norm(psi)
+    for name in ("none", "two"):
+        J, d, e, q = out[name]
+        J0, d0, e0, q0 = out["auto"]
+        assert J == J0 and e == e0 and q == q0
+        np.testing.assert_array_equal(d, d0)
