@@ -159,7 +159,8 @@ int smx_adjoint_grad(smx_handle *h, const double *psi0, int64_t row_stride, int6
  * psi_hat_n of whole checkpoint segments, recorded during the forward sweep and never
  * recomputed).  bytes < 0: 60% of the free device memory at the time of the call (default);
  * 0: checkpoints only (every segment recomputed once).  The gradient does not depend
- * on it.  No counterpart in the reference. */
+ * on it.  The record sets stay allocated in the handle between smx_adjoint_grad calls;
+ * this call and smx_destroy release them.  No counterpart in the reference. */
 int smx_set_adjoint_budget(smx_handle *h, int64_t bytes);
 /* Bookkeeping of the last smx_adjoint_grad: steps recorded in the forward sweep, steps
  * recomputed from checkpoints, device bytes allocated for the sweep. */

@@ -17,20 +17,21 @@ namespace smx {
 // carry holds alpha_{n+1}.
 //   alpha_n = (C1/C2) alpha_{n+1} + s V,  w_{n-1}(k) = dt/2/C2 (cnext alpha_n - alpha_{n+1}).
 // Parameter sums: step n contributes  w_kz Re(conj alpha_{n+1} dt/2 dL/dtheta (Psi_n + Psi_{n+1})/C2)
-// for theta = eps (dL = 1) and q0 (dL = 4 alpha q0 (k^2 - q0^2)).  Only Psi_n (psin) is read: the
-// Psi_{n+1} half of step n was added by the call for n+1 as conj(alpha_{n+1}) Psi_{n+1} (tail != 0
-// there), except for the first call n = T-1, which reads Psi_T (psin1 != nullptr).  Block sums are
-// added to acc[2b], acc[2b+1].  init != 0: alpha_{n+1} = 0 and no sums (sets alpha_T from a_T).
+// for theta = eps (dL = 1) and q0 (dL = 4 alpha q0 (k^2 - q0^2)).  Only one spectrum, psin, is
+// read per call: the call for n adds the Psi_n half of step n (conj alpha_{n+1} Psi_n) and, when
+// tail != 0, the Psi_n half of step n-1 (conj alpha_n Psi_n).  init != 0 sets alpha_T from a_T
+// (alpha_{n+1} = 0); with psin = Psi_T and tail = 1 it adds the Psi_T half of step T-1.  Block
+// sums are added to acc[2b], acc[2b+1] (psin == nullptr: no sums).
 // The self-conjugate planes kz = 0, nz/2 pair columns ky / -ky as in k_xpass, and alpha_n is
 // projected onto Hermitian symmetry there before it is stored.
-template <typename T, int L, int NC>
-__global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass_adj(cplx<T>* stage, cplx<T>* __restrict__ carry,
-                                                                   const cplx<T>* __restrict__ psin,
-                                                                   const cplx<T>* __restrict__ psin1, PParams<T> p,
+template <typename T, int L, int NC, int V = 0>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* stage, cplx<T>* __restrict__ carry,
+                                                                   const cplx<T>* __restrict__ psin, PParams<T> p,
                                                                    T cnext, int init, int tail, double q0d,
                                                                    double alphad, double* __restrict__ acc) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
+  constexpr int E = PlanV<L, V>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
   constexpr uint32_t CB = sizeof(cplx<T>);
+  (void)NT;
   __shared__ cplx<T> S[L * NCP];
   __shared__ double red[64];
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
@@ -56,16 +57,16 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass_adj(cplx<T>* stag
   cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e)
-    v[e] = ld_off<cplx<T>>(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_first<L>()>(tc, e) * rowb + cofs);
-  FwdFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
+    v[e] = ld_off<cplx<T>>(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_first<L, V>()>(tc, e) * rowb + cofs);
+  FwdFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
   const T kyv = p.ky[ky], kzv = p.kz[kz];
   const double wk = herm ? 1.0 : 2.0;
-  const bool sums = !init && acc != nullptr;
+  const bool sums = psin != nullptr && acc != nullptr;
   double ge = 0.0, gq = 0.0;
   if (herm) __syncthreads();  // S receives alpha_n below
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e);
+    const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
     const uint32_t o = (uint32_t)kxi * rowb + cofs;
     const T kxv = p.kx[kxi];
     const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
@@ -76,17 +77,13 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass_adj(cplx<T>* stag
     const cplx<T> a1 = init ? mk<T>(0, 0) : ld_off<cplx<T>>(cb, o);
     const T g = C1 * ic2;
     const cplx<T> a0 = mk(g * a1.x + p.scale * v[e].x, g * a1.y + p.scale * v[e].y);
-    if (sums) {
+    if (sums) {  // per-element terms in the working precision, per-thread sums in FP64
       const cplx<T> s0 = ld_off(psin + plane, o);
-      double re = (double)a1.x * (double)s0.x + (double)a1.y * (double)s0.y;
-      if (psin1 != nullptr) {
-        const cplx<T> s1 = ld_off(psin1 + plane, o);
-        re += (double)a1.x * (double)s1.x + (double)a1.y * (double)s1.y;
-      }
-      if (tail) re += (double)a0.x * (double)s0.x + (double)a0.y * (double)s0.y;
-      const double base = wk * re * (double)p.dtd2 / (double)C2;
-      ge += base;
-      gq += base * 4.0 * alphad * q0d * ((double)k2 - q0d * q0d);
+      T re = a1.x * s0.x + a1.y * s0.y;
+      if (tail) re += a0.x * s0.x + a0.y * s0.y;
+      const T base = re * p.dtd2 * ic2;
+      ge += (double)base;
+      gq += (double)(base * (k2 - p.q02));
     }
     if (herm) S[kxi * NCP + col] = a0;
     else st_off(cb, o, a0);
@@ -97,30 +94,29 @@ __global__ __launch_bounds__(NC*(L / Plan<L>::E)) void k_xpass_adj(cplx<T>* stag
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L>()>(tc, e);
+      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
       const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
       st_off(cb, (uint32_t)kxi * rowb + cofs, mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)));
     }
   }
-  if (sums) {
+  if (sums) {  // plane weight (1 on the self-conjugate planes, 2 elsewhere) and dL/dq0 = 4 alpha q0 (k^2 - q0^2)
     const double be = block_sum(ge, red), bq = block_sum(gq, red);
     if (tid == 0) {
       const int b = blockIdx.y * gridDim.x + blockIdx.x;
-      acc[2 * b] += be;
-      acc[2 * b + 1] += bq;
+      acc[2 * b] += wk * be;
+      acc[2 * b + 1] += wk * 4.0 * alphad * q0d * bq;
     }
   }
-  InvFFT<T, L, NCP>::run(v, S, col, tc, p.twx);
+  InvFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
 #pragma unroll
   for (int e = 0; e < E; ++e)
-    st_off(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L>()>(tc, e) * rowb + cofs, v[e]);
+    st_off(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
 }
 
 template <typename T>
 struct AdjointOps {
-  static hipError_t xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T>* psin, const cplx<T>* psin1,
-                              const PParams<T>& p, T cnext, int init, int tail, double q0d, double alphad,
-                              double* acc, hipStream_t st);
+  static hipError_t xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T>* psin, const PParams<T>& p, T cnext,
+                              int init, int tail, double q0d, double alphad, double* acc, hipStream_t st);
 };
 
 }  // namespace smx

@@ -98,7 +98,18 @@ struct smx_handle {
   PassSched sched[NSLOTS];  // tile schedules of the periodic step kernels
   int64_t adj_budget = -1;  // device bytes the adjoint may keep resident (-1: 60% of free memory)
   int64_t adj_info[3] = {0, 0, 0};  // last adjoint: resident steps, recomputed steps, device bytes
+  // adjoint record sets kept between calls (N'(psi_n) and psi_hat_n of one segment each)
+  std::vector<void*> adj_np, adj_psi;
+  size_t adj_np_bytes = 0, adj_psi_bytes = 0;
 };
+
+static void adj_release(smx_handle* h) {
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  for (void* q : h->adj_np) (void)hipFree(q);
+  for (void* q : h->adj_psi) (void)hipFree(q);
+  h->adj_np.clear();
+  h->adj_psi.clear();
+}
 
 // FP64 partial sums of one z pass: one per tile at the narrowest tile (8 columns), plus
 // slack.  The handle holds two such regions: the step's and the autotuner's scratch.
@@ -298,6 +309,20 @@ static const char* slot_name(const smx_handle* h, int s) {
 }
 
 // One advected periodic step (pf3d.cpp:395-485): 16 launches, in place per tile.
+// Per-kernel timing of a step chain: slot k's kernel is launched with the event pair
+// (ev[2k], ev[2k+1]) (hipExtLaunchKernelGGL stamps them with the kernel's start and end).
+// ev == nullptr: no timing.  Disarms on scope exit.
+struct SlotTimer {
+  hipEvent_t* ev;
+  explicit SlotTimer(hipEvent_t* e) : ev(e) {}
+  void arm(int slot) {
+    if (ev) launch_events() = LaunchEvents{ev[2 * slot], ev[2 * slot + 1]};
+  }
+  ~SlotTimer() {
+    if (ev) launch_events() = LaunchEvents{};
+  }
+};
+
 template <typename T>
 static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
   using Ops = PeriodicOps<T>;
@@ -308,12 +333,13 @@ static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
   cplx<T>*S1 = (cplx<T>*)h->S1, *S2 = (cplx<T>*)h->S2, *R = (cplx<T>*)h->R;
   int slot = 0;
   hipError_t e = hipSuccess;
+  SlotTimer tm(ev);
 #define SMX_REC()                                             \
   do {                                                        \
     if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e)); \
-    if (ev) HIPCHK(hipEventRecord(ev[++slot], h->st));        \
+    if (++slot < 16) tm.arm(slot); /* 16 kernels per step */ \
   } while (0)
-  if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
+  tm.arm(0);
   e = Ops::ypass(A, By, p.ny, h->npencils, +1, t.twy, h->st, t.ky);  // d_y psi (pf3d.cpp:405,411)
   SMX_REC();
   e = Ops::ypass(A, A, p.ny, h->npencils, +1, t.twy, h->st);
@@ -356,31 +382,30 @@ static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
     AParams<T> ap = aparams<T>(h, h->fp32);
     return step_advect<T>(h, ap, ev);
   }
+  SlotTimer tm(ev);
   if (h->neumann) {
-    if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
+    tm.arm(0);
     CHK(ypass<T>(h, h->A, h->A, +1));
-    if (ev) HIPCHK(hipEventRecord(ev[1], h->st));
+    tm.arm(1);
     CHK(zpass<T>(h, p, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr));
-    if (ev) HIPCHK(hipEventRecord(ev[2], h->st));
+    tm.arm(2);
     CHK(ypass<T>(h, h->A, h->A, -1));
-    if (ev) HIPCHK(hipEventRecord(ev[3], h->st));
+    tm.arm(3);
     CHK(xpass<T>(h, p, X_STEP, h->A, h->A, h->R, nullptr));
-    if (ev) HIPCHK(hipEventRecord(ev[4], h->st));
     return SMX_OK;
   }
   // periodic chain with the handle's tile schedules; the x pass sums the z pass's
   // FP64 partials, one per z tile
   PParams<T> q = p;
   q.npartial = PeriodicOps<T>::zpass_blocks(p, h->sched[SLOT_Z].nc);
-  if (ev) HIPCHK(hipEventRecord(ev[0], h->st));
+  tm.arm(0);
   CHK(ypass<T>(h, h->A, h->A, +1, h->sched[SLOT_YINV]));
-  if (ev) HIPCHK(hipEventRecord(ev[1], h->st));
+  tm.arm(1);
   CHK(zpass<T>(h, q, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr, h->sched[SLOT_Z]));
-  if (ev) HIPCHK(hipEventRecord(ev[2], h->st));
+  tm.arm(2);
   CHK(ypass<T>(h, h->A, h->A, -1, h->sched[SLOT_YFWD]));
-  if (ev) HIPCHK(hipEventRecord(ev[3], h->st));
+  tm.arm(3);
   CHK(xpass<T>(h, q, X_STEP, h->A, h->A, h->R, nullptr, h->sched[SLOT_X]));
-  if (ev) HIPCHK(hipEventRecord(ev[4], h->st));
   return SMX_OK;  // the x pass projects the self-conjugate planes of R itself
 }
 
@@ -532,20 +557,20 @@ template <typename T>
 static int profile_T(smx_handle* h, int64_t nsteps, std::vector<double>& acc) {
   PParams<T> p = params<T>(h, h->fp32);
   const int ns = num_slots(h);
-  std::vector<hipEvent_t> ev(ns + 1, nullptr);
+  std::vector<hipEvent_t> ev(2 * ns, nullptr);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   acc.assign(ns, 0.0);
   int rc = SMX_OK;
   for (int64_t n = 0; n < nsteps && rc == SMX_OK; ++n) {
     rc = step_once<T>(h, p, ev.data());
     if (rc) break;
-    if (hipEventSynchronize(ev[ns]) != hipSuccess) {
+    if (hipEventSynchronize(ev[2 * ns - 1]) != hipSuccess) {
       rc = fail(SMX_ERR_HIP, "event sync");
       break;
     }
     for (int s = 0; s < ns; ++s) {
       float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, ev[s], ev[s + 1]);
+      (void)hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
       acc[s] += ms;
     }
   }
@@ -598,24 +623,31 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     cleanup();
     return rc;
   }
-  // record sets, as many as the budget allows (at least one for the recomputed segments)
-  std::vector<char*> setnp, setpsi;
-  for (int64_t i = 0; i < std::max<int64_t>(nres_want, 1); ++i) {
+  // record sets, as many as the budget allows (at least one for the recomputed segments);
+  // kept in the handle for the next call
+  if (h->adj_np_bytes != npbytes * (size_t)K || h->adj_psi_bytes != sbytes * (size_t)(K + 1)) {
+    adj_release(h);
+    h->adj_np_bytes = npbytes * (size_t)K;
+    h->adj_psi_bytes = sbytes * (size_t)(K + 1);
+  }
+  while ((int64_t)h->adj_np.size() < std::max<int64_t>(nres_want, 1)) {
     void *a = nullptr, *b = nullptr;
-    if (talloc(&a, npbytes * (size_t)K) != SMX_OK || talloc(&b, sbytes * (size_t)(K + 1)) != SMX_OK) {
+    if (hipMalloc(&a, h->adj_np_bytes) != hipSuccess || hipMalloc(&b, h->adj_psi_bytes) != hipSuccess) {
       (void)hipGetLastError();
-      if (a) {
-        (void)hipFree(a);
-        tmp.pop_back();
-      }
-      if (i == 0) {
+      if (a) (void)hipFree(a);
+      if (h->adj_np.empty()) {
         cleanup();
         return fail(SMX_ERR_HIP, "no device memory for one adjoint record set (" + std::to_string(setbytes) + " B)");
       }
       break;
     }
-    setnp.push_back((char*)a);
-    setpsi.push_back((char*)b);
+    h->adj_np.push_back(a);
+    h->adj_psi.push_back(b);
+  }
+  std::vector<char*> setnp, setpsi;
+  for (size_t i = 0; i < h->adj_np.size(); ++i) {
+    setnp.push_back((char*)h->adj_np[i]);
+    setpsi.push_back((char*)h->adj_psi[i]);
   }
   const int64_t nres = std::min<int64_t>(nres_want, (int64_t)setnp.size());
   const int64_t sres = nseg - nres;  // segments [sres, nseg) are resident
@@ -678,8 +710,13 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     CHK(zpass<T>(h, pa, Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, AS, h->real_io, nullptr));
     CHK(ypass<T>(h, AS, AS, -1));
     HIPCHK(hipMemsetAsync(acc, 0, sizeof(double) * 2 * nblk, h->st));
-    HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, nullptr, nullptr, pa, (T)(T_ - 1 >= 1 ? 3 : 2), 1, 0,
-                                    c.q0, c.alpha, acc, h->st));
+    {  // Psi_T = psi_hat_T (from the stage buffer) for the Psi_T half of step T-1
+      PParams<T> px = params<T>(h, false);
+      px.scale = (T)(1.0 / (double)c.nx);
+      CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, h->P, nullptr, nullptr));
+    }
+    HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, (const cplx<T>*)h->P, pa, (T)(T_ - 1 >= 1 ? 3 : 2),
+                                    1, 1, c.q0, c.alpha, acc, h->st));
     // reverse sweep over the segments
     for (int64_t seg = nseg - 1; seg >= 0; --seg) {
       const int64_t s0 = seg * K, len = std::min(K, T_ - s0);
@@ -697,8 +734,7 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
         CHK(zpass<T>(h, pm, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_MUL, AS, AS, nullptr, nullptr, h->sched[SLOT_Z]));
         CHK(ypass<T>(h, AS, AS, -1, h->sched[SLOT_YFWD]));
         const cplx<T>* psin = (const cplx<T>*)(setpsi[set] + sbytes * j);
-        const cplx<T>* psin1 = (n == T_ - 1) ? (const cplx<T>*)(setpsi[set] + sbytes * (j + 1)) : nullptr;
-        HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, psin, psin1, pa, (T)(n - 1 >= 1 ? 3 : 2), 0,
+        HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, psin, pa, (T)(n - 1 >= 1 ? 3 : 2), 0,
                                         n >= 1 ? 1 : 0, c.q0, c.alpha, acc, h->st));
       }
     }
@@ -878,6 +914,39 @@ static int autotune_T(smx_handle* h, int reps, PassSched* best) {
   return rc;
 }
 
+// ---- real fields across the boundary ----
+// Host arrays use the reference layout with row stride rs >= nz; device fields are dense.
+// A dense host array moves with one transfer; a padded one is packed / unpacked row by row
+// on the host (the runtime's pitched copies issue one transfer per row), and the host
+// padding is left untouched.
+static int field_to_device(smx_handle* h, const double* psi, int64_t rs) {
+  const int nz = h->cfg.nz;
+  if (rs == nz) {
+    HIPCHK(hipMemcpyAsync(h->real_io, psi, sizeof(double) * (size_t)h->N, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));  // psi is borrowed for the duration of the call only
+    return SMX_OK;
+  }
+  std::vector<double> dense((size_t)h->N);
+  for (int64_t r = 0; r < h->nxy; ++r) std::memcpy(&dense[(size_t)r * nz], psi + r * rs, sizeof(double) * nz);
+  HIPCHK(hipMemcpyAsync(h->real_io, dense.data(), sizeof(double) * dense.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SMX_OK;
+}
+
+static int field_to_host(smx_handle* h, const double* dev, double* out, int64_t rs) {
+  const int nz = h->cfg.nz;
+  if (rs == nz) {
+    HIPCHK(hipMemcpyAsync(out, dev, sizeof(double) * (size_t)h->N, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SMX_OK;
+  }
+  std::vector<double> dense((size_t)h->N);
+  HIPCHK(hipMemcpyAsync(dense.data(), dev, sizeof(double) * dense.size(), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int64_t r = 0; r < h->nxy; ++r) std::memcpy(out + r * rs, &dense[(size_t)r * nz], sizeof(double) * nz);
+  return SMX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 extern "C" {
@@ -979,6 +1048,7 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
 
 void smx_destroy(smx_handle* h) {
   if (!h) return;
+  adj_release(h);
   if (h->st) (void)hipStreamSynchronize(h->st);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -996,8 +1066,7 @@ int smx_set_psi(smx_handle* h, const double* psi, int64_t row_stride) {
   const smx_config& c = h->cfg;
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
-  HIPCHK(hipMemcpy2DAsync(h->real_io, sizeof(double) * c.nz, psi, sizeof(double) * row_stride, sizeof(double) * c.nz,
-                          (size_t)h->nxy, hipMemcpyHostToDevice, h->st));
+  CHK(field_to_device(h, psi, row_stride));
   HIPCHK(hipMemsetAsync(h->mean_state, 0, sizeof(double) * 8, h->st));
   CHK(h->fp32 ? bootstrap<float>(h) : bootstrap<double>(h));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -1014,10 +1083,7 @@ int smx_get_psi(smx_handle* h, double* psi, int64_t row_stride) {
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
   CHK(c2r_state(h, h->real_io));
-  HIPCHK(hipMemcpy2DAsync(psi, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz, sizeof(double) * c.nz,
-                          (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipStreamSynchronize(h->st));
-  return SMX_OK;
+  return field_to_host(h, h->real_io, psi, row_stride);
 }
 
 int smx_step(smx_handle* h, int64_t nsteps) {
@@ -1095,10 +1161,7 @@ int smx_dtpsi(smx_handle* h, double* out, int64_t row_stride) {
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
   CHK(dtpsi_field(h, h->real_io));
-  HIPCHK(hipMemcpy2DAsync(out, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz, sizeof(double) * c.nz,
-                          (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipStreamSynchronize(h->st));
-  return SMX_OK;
+  return field_to_host(h, h->real_io, out, row_stride);
 }
 
 int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, double* vel, double* H, double* K,
@@ -1172,9 +1235,7 @@ int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int6
   const int64_t K = std::min(ckpt_every, nsteps);
   CHK(h->fp32 ? adjoint_T<float>(h, nsteps, K, target_amp, loss, deps, dq0)
               : adjoint_T<double>(h, nsteps, K, target_amp, loss, deps, dq0));
-  HIPCHK(hipMemcpy2DAsync(dpsi0, sizeof(double) * row_stride, h->real_io, sizeof(double) * c.nz, sizeof(double) * c.nz,
-                          (size_t)h->nxy, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipStreamSynchronize(h->st));
+  CHK(field_to_host(h, h->real_io, dpsi0, row_stride));
   return smx_set_psi(h, psi0, row_stride);  // leave the handle at psi0
 }
 
@@ -1182,6 +1243,7 @@ int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int6
 int smx_set_adjoint_budget(smx_handle* h, int64_t bytes) {
   if (!h) return fail(SMX_ERR_INVALID, "null handle");
   h->adj_budget = bytes < 0 ? -1 : bytes;
+  adj_release(h);
   return SMX_OK;
 }
 

@@ -1,6 +1,8 @@
 // smx_launch.h -- runtime-length -> template-instance dispatch for the pass kernels.
 // Instantiated once per precision in smx_inst_f32.hip / smx_inst_f64.hip.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -8,6 +10,23 @@
 #include "smx_neumann.h"
 
 namespace smx {
+
+// Kernel timing: when the calling thread has set a start/stop event pair, launches go
+// through hipExtLaunchKernelGGL, which stamps the events with the kernel's own start and
+// end (no extra packets in the stream).  Used to time each kernel inside a real step chain.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+inline LaunchEvents& launch_events() {
+  static thread_local LaunchEvents ev;
+  return ev;
+}
+#define SMX_LAUNCH(K, G, B, SH, ST, ...)                                                   \
+  do {                                                                                   \
+    const LaunchEvents& ev_ = launch_events();                                           \
+    if (ev_.start) hipExtLaunchKernelGGL(K, G, B, SH, ST, ev_.start, ev_.stop, 0, __VA_ARGS__); \
+    else hipLaunchKernelGGL(K, G, B, SH, ST, __VA_ARGS__);                               \
+  } while (0)
 
 constexpr int kNC = 16;  // columns (pencils) per tile
 
@@ -78,7 +97,7 @@ struct PeriodicOps {
   static hipError_t herm_project(cplx<T>* a, const PParams<T>& p, hipStream_t st) {
     const int64_t work = 2 * p.nxy;
     const unsigned blocks = (unsigned)std::min<int64_t>((work + 255) / 256, 4096);
-    hipLaunchKernelGGL((k_herm_project<T>), dim3(blocks), dim3(256), 0, st, a, p.nx, p.ny, p.nzh, p.nz);
+    SMX_LAUNCH((k_herm_project<T>), dim3(blocks), dim3(256), 0, st, a, p.nx, p.ny, p.nzh, p.nz);
     return hipGetLastError();
   }
 };
@@ -96,8 +115,8 @@ hipError_t PeriodicOps<T>::ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \
         const dim3 grid((unsigned)((npencils + NC - 1) / NC));                                           \
-        if (sign < 0) hipLaunchKernelGGL((k_ypass<T, L, -1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul); \
-        else hipLaunchKernelGGL((k_ypass<T, L, +1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul);          \
+        if (sign < 0) SMX_LAUNCH((k_ypass<T, L, -1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul); \
+        else SMX_LAUNCH((k_ypass<T, L, +1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul);          \
         return hipGetLastError();                                                                        \
       }                                                                                                  \
     });
@@ -117,9 +136,9 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
   if (p.nx == L) {                                                                                    \
     const dim3 block(kNC * (L / Plan<L>::E));                                                         \
     switch (mode) {                                                                                   \
-      case X_FWD_SCALED: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
-      case X_BOOT: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
-      case X_INV: hipLaunchKernelGGL((k_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      case X_FWD_SCALED: SMX_LAUNCH((k_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      case X_BOOT: SMX_LAUNCH((k_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      case X_INV: SMX_LAUNCH((k_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
       default: return hipErrorInvalidValue;                                                           \
     }                                                                                                 \
     return hipGetLastError();                                                                         \
@@ -138,7 +157,7 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
         return hipErrorInvalidValue;                                                                     \
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \
-        hipLaunchKernelGGL((k_xpass<T, L, NC, X_STEP, V>), dim3((unsigned)(p.ny / NC), (unsigned)p.nzh), dim3(NT), 0, \
+        SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V>), dim3((unsigned)(p.ny / NC), (unsigned)p.nzh), dim3(NT), 0, \
                            st, in, out, rbuf, psi0, p);                                                  \
         return hipGetLastError();                                                                        \
       }                                                                                                  \
@@ -161,9 +180,9 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
   if (p.nz == L) {                                                                                   \
     const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                  \
     if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                              \
-      hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
+      SMX_LAUNCH((k_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
     else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
-      hipLaunchKernelGGL((k_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
+      SMX_LAUNCH((k_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p); \
     else                                                                                             \
       return hipErrorInvalidValue;                                                                   \
     return hipGetLastError();                                                                        \
@@ -181,7 +200,7 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
         return hipErrorInvalidValue;                                                                     \
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \
-        hipLaunchKernelGGL((k_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC, V>), dim3((unsigned)(p.nxy / NC)), dim3(NT), 0, st, \
+        SMX_LAUNCH((k_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC, V>), dim3((unsigned)(p.nxy / NC)), dim3(NT), 0, st, \
                            in, out, real_in, real_out, op, p);                                           \
         return hipGetLastError();                                                                        \
       }                                                                                                  \
@@ -211,8 +230,8 @@ hipError_t NeumannOps<T>::ypass(const T* in, T* out, int ny, int64_t npencils, i
 #define SMX_CASE(L)                                                                                 \
   if (ny == L) {                                                                                    \
     const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                 \
-    if (dir < 0) hipLaunchKernelGGL((k_dct_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw); \
-    else hipLaunchKernelGGL((k_dct_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw);          \
+    if (dir < 0) SMX_LAUNCH((k_dct_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw); \
+    else SMX_LAUNCH((k_dct_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw);          \
     return hipGetLastError();                                                                       \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -228,10 +247,10 @@ hipError_t NeumannOps<T>::xpass(int mode, const T* in, T* out, T* rbuf, const T*
   if (p.nx == L) {                                                                                    \
     const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                   \
     switch (mode) {                                                                                   \
-      case X_STEP: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_STEP>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
-      case X_FWD_SCALED: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
-      case X_BOOT: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
-      case X_INV: hipLaunchKernelGGL((k_dct_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      case X_STEP: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_STEP>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      case X_FWD_SCALED: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      case X_BOOT: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
+      case X_INV: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
       default: return hipErrorInvalidValue;                                                           \
     }                                                                                                 \
     return hipGetLastError();                                                                         \
@@ -249,11 +268,11 @@ hipError_t NeumannOps<T>::zpass(int in_mode, int out_mode, int op, const T* in, 
   if (p.nz == L) {                                                                                   \
     const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                  \
     if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)                                              \
-      hipLaunchKernelGGL((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+      SMX_LAUNCH((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
     else if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                         \
-      hipLaunchKernelGGL((k_dct_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+      SMX_LAUNCH((k_dct_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
     else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
-      hipLaunchKernelGGL((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+      SMX_LAUNCH((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
     else                                                                                             \
       return hipErrorInvalidValue;                                                                   \
     return hipGetLastError();                                                                        \
@@ -272,15 +291,24 @@ constexpr int nc_fit(int ntiles) {
                                                                   : 4;
 }
 
+// Plan of the adjoint x pass: the E = 8 variant where there is one and its block stays at
+// <= 512 threads (half the registers per thread of the default plan).
+template <int L>
+constexpr int adj_variant() {
+  if constexpr (HasPlanV<L, 1>::value) return (kNC * (L / PlanV<L, 1>::E) <= 512) ? 1 : 0;
+  else return 0;
+}
+
 template <typename T>
-hipError_t AdjointOps<T>::xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T>* psin, const cplx<T>* psin1,
-                                    const PParams<T>& p, T cnext, int init, int tail, double q0d, double alphad,
-                                    double* acc, hipStream_t st) {
+hipError_t AdjointOps<T>::xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T>* psin, const PParams<T>& p,
+                                    T cnext, int init, int tail, double q0d, double alphad, double* acc,
+                                    hipStream_t st) {
   const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
 #define SMX_CASE(L)                                                                                      \
   if (p.nx == L) {                                                                                       \
-    hipLaunchKernelGGL((k_xpass_adj<T, L, kNC>), grid, dim3(kNC * (L / Plan<L>::E)), 0, st, stage, carry, psin, \
-                       psin1, p, cnext, init, tail, q0d, alphad, acc);                                   \
+    constexpr int V = adj_variant<L>();                                                                  \
+    SMX_LAUNCH((k_xpass_adj<T, L, kNC, V>), grid, dim3(kNC * (L / PlanV<L, V>::E)), 0, st, stage, carry, psin, \
+               p, cnext, init, tail, q0d, alphad, acc);                                                  \
     return hipGetLastError();                                                                            \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -298,9 +326,9 @@ struct AdvectOps {
     if (p.ny % NC) return hipErrorInvalidValue;                                                           \
     const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block(NC * (L / Plan<L>::E));                \
     if (mode == X_STEP)                                                                                   \
-      hipLaunchKernelGGL((k_xpass_adv<T, L, NC, X_STEP>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
+      SMX_LAUNCH((k_xpass_adv<T, L, NC, X_STEP>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_xpass_adv<T, L, NC, X_BOOT>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
+      SMX_LAUNCH((k_xpass_adv<T, L, NC, X_BOOT>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
     return hipGetLastError();                                                                             \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -314,7 +342,7 @@ struct AdvectOps {
     constexpr int NC = nc_fit<T, L>(3);                                                            \
     if (p.ny % NC) return hipErrorInvalidValue;                                                    \
     const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block(NC * (L / Plan<L>::E));         \
-    hipLaunchKernelGGL((k_xpass_stokes<T, L, NC>), grid, block, 0, st, s0, s1, s2, ap);            \
+    SMX_LAUNCH((k_xpass_stokes<T, L, NC>), grid, block, 0, st, s0, s1, s2, ap);            \
     return hipGetLastError();                                                                      \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -327,7 +355,7 @@ struct AdvectOps {
     const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    hipLaunchKernelGGL((k_zpass_force<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, bpsi, bx, by, \
+    SMX_LAUNCH((k_zpass_force<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, bpsi, bx, by, \
                        bs, o0, o1, o2, ap);                                                             \
     return hipGetLastError();                                                                           \
   }
@@ -341,7 +369,7 @@ struct AdvectOps {
     const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    hipLaunchKernelGGL((k_zpass_advnl<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, u0, u1, u2, \
+    SMX_LAUNCH((k_zpass_advnl<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, u0, u1, u2, \
                        bpsi, bx, by, nout, ap);                                                         \
     return hipGetLastError();                                                                           \
   }
