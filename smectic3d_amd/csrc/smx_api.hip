@@ -101,6 +101,8 @@ struct smx_handle {
   // adjoint record sets kept between calls (N'(psi_n) and psi_hat_n of one segment each)
   std::vector<void*> adj_np, adj_psi;
   size_t adj_np_bytes = 0, adj_psi_bytes = 0;
+  void* adj_stage = nullptr;  // adjoint stage and carry alpha, kept with the record sets
+  void* adj_carry = nullptr;
 };
 
 static void adj_release(smx_handle* h) {
@@ -109,6 +111,9 @@ static void adj_release(smx_handle* h) {
   for (void* q : h->adj_psi) (void)hipFree(q);
   h->adj_np.clear();
   h->adj_psi.clear();
+  if (h->adj_stage) (void)hipFree(h->adj_stage);
+  if (h->adj_carry) (void)hipFree(h->adj_carry);
+  h->adj_stage = h->adj_carry = nullptr;
 }
 
 // FP64 partial sums of one z pass: one per tile at the narrowest tile (8 columns), plus
@@ -214,7 +219,7 @@ static PParams<T> params(smx_handle* h, bool mixed_mean) {
   p.twy = t.twy;
   p.twz = t.twz;
   p.partial = mixed_mean ? h->partial : nullptr;
-  p.npartial = (int)((h->nxy + kNC - 1) / kNC);
+  p.npartial = h->neumann ? (int)(h->nxy / neumann_nc<T>()) : (int)((h->nxy + kNC - 1) / kNC);
   p.mean_state = h->mean_state;
   const double q02 = c.q0 * c.q0, aLin0 = c.eps - c.alpha * q02 * q02;
   p.m_scale = 1.0 / (double)h->N;  // k = 0 coefficient of either transform, normalised
@@ -607,18 +612,22 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     for (void* q : tmp) (void)hipFree(q);
   };
   int64_t budget = h->adj_budget;
-  if (budget < 0) {
+  if (budget < 0) {  // 60% of the device memory that is free or already held by the record sets
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
-    budget = (int64_t)(0.6 * (double)fr);
+    const double held = (double)h->adj_np.size() * (double)(h->adj_np_bytes + h->adj_psi_bytes);
+    budget = (int64_t)(0.6 * ((double)fr + held));
   }
   const int64_t nres_want = std::min<int64_t>(nseg, budget / (int64_t)setbytes);
   void *AS = nullptr, *AC = nullptr;
   double *ckmean = nullptr, *acc = nullptr;
   const int nxb = c.ny / kNC, nblk = nxb * h->nplanes;
   int rc = SMX_OK;
-  if ((rc = talloc(&AS, sbytes)) || (rc = talloc(&AC, sbytes)) ||
-      (rc = talloc((void**)&acc, sizeof(double) * 2 * nblk)) ||
+  if (!h->adj_stage) HIPCHK(hipMalloc(&h->adj_stage, sbytes));
+  if (!h->adj_carry) HIPCHK(hipMalloc(&h->adj_carry, sbytes));
+  AS = h->adj_stage;
+  AC = h->adj_carry;
+  if ((rc = talloc((void**)&acc, sizeof(double) * 2 * nblk)) ||
       (rc = talloc((void**)&ckmean, sizeof(double) * 2 * nseg))) {
     cleanup();
     return rc;
@@ -947,6 +956,17 @@ static int field_to_host(smx_handle* h, const double* dev, double* out, int64_t 
   return SMX_OK;
 }
 
+// Bootstrap the state from the real field in real_io (the tail of smx_set_psi).
+static int state_from_real_io(smx_handle* h) {
+  HIPCHK(hipMemsetAsync(h->mean_state, 0, sizeof(double) * 8, h->st));
+  CHK(h->fp32 ? bootstrap<float>(h) : bootstrap<double>(h));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->has_state = true;
+  h->has_prev = false;
+  h->steps = 0;
+  return SMX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 extern "C" {
@@ -1067,13 +1087,7 @@ int smx_set_psi(smx_handle* h, const double* psi, int64_t row_stride) {
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
   CHK(field_to_device(h, psi, row_stride));
-  HIPCHK(hipMemsetAsync(h->mean_state, 0, sizeof(double) * 8, h->st));
-  CHK(h->fp32 ? bootstrap<float>(h) : bootstrap<double>(h));
-  HIPCHK(hipStreamSynchronize(h->st));
-  h->has_state = true;
-  h->has_prev = false;
-  h->steps = 0;
-  return SMX_OK;
+  return state_from_real_io(h);
 }
 
 int smx_get_psi(smx_handle* h, double* psi, int64_t row_stride) {
@@ -1232,11 +1246,15 @@ int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int6
   if (nsteps < 1 || ckpt_every < 1) return fail(SMX_ERR_INVALID, "nsteps and ckpt_every must be >= 1");
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   CHK(smx_set_psi(h, psi0, row_stride));
+  // psi0 stays on the device (the L1 snapshot buffer) to leave the handle at psi0 afterwards
+  const size_t rbytes = sizeof(double) * (size_t)h->N;
+  HIPCHK(hipMemcpyAsync(h->real_prev, h->real_io, rbytes, hipMemcpyDeviceToDevice, h->st));
   const int64_t K = std::min(ckpt_every, nsteps);
   CHK(h->fp32 ? adjoint_T<float>(h, nsteps, K, target_amp, loss, deps, dq0)
               : adjoint_T<double>(h, nsteps, K, target_amp, loss, deps, dq0));
   CHK(field_to_host(h, h->real_io, dpsi0, row_stride));
-  return smx_set_psi(h, psi0, row_stride);  // leave the handle at psi0
+  HIPCHK(hipMemcpyAsync(h->real_io, h->real_prev, rbytes, hipMemcpyDeviceToDevice, h->st));
+  return state_from_real_io(h);
 }
 
 

@@ -211,6 +211,9 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
 }
 
 
+// Tile width (columns, in pairs of adjacent lines) of the Neumann passes: rows of 128 bytes.
+template <typename T> constexpr int neumann_nc() { return sizeof(T) == 4 ? 32 : 16; }
+
 template <typename T>
 struct NeumannOps {
   // y pass on real pencils (npencils = nz*nx), dir -1: DCT-II, +1: DCT-III.
@@ -220,18 +223,20 @@ struct NeumannOps {
                           const cplx<T>* qtw, hipStream_t st);
   static hipError_t zpass(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
                           double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st);
-  static int zpass_blocks(const PParams<T>& p) { return (int)((p.nxy + kNC - 1) / kNC); }
+  static int zpass_blocks(const PParams<T>& p) { return (int)(p.nxy / neumann_nc<T>()); }
 };
 
 template <typename T>
 hipError_t NeumannOps<T>::ypass(const T* in, T* out, int ny, int64_t npencils, int dir, const cplx<T>* tw,
                                 const cplx<T>* qtw, hipStream_t st) {
-  const dim3 grid((unsigned)((npencils + kNC - 1) / kNC));
+  constexpr int NC = neumann_nc<T>();
+  if (npencils % NC != 0) return hipErrorInvalidValue;  // whole tiles of pencil pairs
+  const dim3 grid((unsigned)(npencils / NC));
 #define SMX_CASE(L)                                                                                 \
   if (ny == L) {                                                                                    \
-    const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                 \
-    if (dir < 0) SMX_LAUNCH((k_dct_ypass<T, L, -1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw); \
-    else SMX_LAUNCH((k_dct_ypass<T, L, +1, kNC>), grid, block, 0, st, in, out, npencils, tw, qtw);          \
+    const dim3 block((NC / 2) * (L / Plan<L>::E));                                                  \
+    if (dir < 0) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC>), grid, block, 0, st, in, out, tw, qtw);     \
+    else SMX_LAUNCH((k_dct_ypass<T, L, +1, NC>), grid, block, 0, st, in, out, tw, qtw);             \
     return hipGetLastError();                                                                       \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -239,21 +244,32 @@ hipError_t NeumannOps<T>::ypass(const T* in, T* out, int ny, int64_t npencils, i
   return hipErrorInvalidValue;
 }
 
+template <typename T, int L, int NC>
+static hipError_t dct_xpass_launch(int mode, const T* in, T* out, T* rbuf, const T* psi0, const PParams<T>& p,
+                                   const cplx<T>* qtw, hipStream_t st) {
+  const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block((NC / 2) * (L / Plan<L>::E));
+  switch (mode) {
+    case X_STEP: SMX_LAUNCH((k_dct_xpass<T, L, NC, X_STEP>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break;
+    case X_FWD_SCALED: SMX_LAUNCH((k_dct_xpass<T, L, NC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break;
+    case X_BOOT: SMX_LAUNCH((k_dct_xpass<T, L, NC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break;
+    case X_INV: SMX_LAUNCH((k_dct_xpass<T, L, NC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t NeumannOps<T>::xpass(int mode, const T* in, T* out, T* rbuf, const T* psi0, const PParams<T>& p,
                                 const cplx<T>* qtw, hipStream_t st) {
-  const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh);
-#define SMX_CASE(L)                                                                                   \
-  if (p.nx == L) {                                                                                    \
-    const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                   \
-    switch (mode) {                                                                                   \
-      case X_STEP: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_STEP>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
-      case X_FWD_SCALED: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
-      case X_BOOT: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
-      case X_INV: SMX_LAUNCH((k_dct_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p, qtw); break; \
-      default: return hipErrorInvalidValue;                                                           \
-    }                                                                                                 \
-    return hipGetLastError();                                                                         \
+  // 128-byte rows where ny allows it (ny is a multiple of 16)
+  constexpr int NC = neumann_nc<T>();
+  const bool wide = (p.ny % NC == 0);
+  if (p.ny % 16 != 0) return hipErrorInvalidValue;
+#define SMX_CASE(L)                                                                          \
+  if (p.nx == L) {                                                                           \
+    if (wide) return dct_xpass_launch<T, L, NC>(mode, in, out, rbuf, psi0, p, qtw, st);     \
+    if constexpr (NC != 16) return dct_xpass_launch<T, L, 16>(mode, in, out, rbuf, psi0, p, qtw, st); \
+    return hipErrorInvalidValue;                                                             \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
@@ -263,16 +279,18 @@ hipError_t NeumannOps<T>::xpass(int mode, const T* in, T* out, T* rbuf, const T*
 template <typename T>
 hipError_t NeumannOps<T>::zpass(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
                                 double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st) {
+  constexpr int NC = neumann_nc<T>();
+  if (p.nxy % NC != 0) return hipErrorInvalidValue;  // whole tiles
   const dim3 grid((unsigned)zpass_blocks(p));
 #define SMX_CASE(L)                                                                                  \
   if (p.nz == L) {                                                                                   \
-    const dim3 block((kNC / 2) * (L / Plan<L>::E));                                                  \
+    const dim3 block((NC / 2) * (L / Plan<L>::E));                                                   \
     if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)                                              \
-      SMX_LAUNCH((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
     else if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                         \
-      SMX_LAUNCH((k_dct_zpass<T, L, kNC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
     else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
-      SMX_LAUNCH((k_dct_zpass<T, L, kNC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
+      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
     else                                                                                             \
       return hipErrorInvalidValue;                                                                   \
     return hipGetLastError();                                                                        \

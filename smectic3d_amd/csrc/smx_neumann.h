@@ -15,6 +15,12 @@
 //            y_b[k] = 2 Re(e^{-i pi k/2n} (Z[k] - conj Z[n-k])/2i).
 //   REDFT01: V[k] = e^{i pi k/2n} ((X_k + Y_{n-k}) + i (Y_k - X_{n-k})), X_n = Y_n = 0;
 //            z = IFFT(V) (unnormalised); a[src(m)] = Re z[m], b[src(m)] = Im z[m].
+// The two lines of a pair are adjacent columns, so a lane moves both with one 2-real
+// access (cplx<T> holds the pair: .x column a, .y column b).  The reordering src(m) is
+// applied on the HBM side of the first loads and last stores; the z pass needs none
+// between its DCT-III and DCT-II, whose FFT positions coincide (forward first-pass inputs
+// = inverse last-pass outputs).  In the x pass the DCT-II output at k and n-k, the CNAB2
+// update there and the DCT-III input at k and n-k are one work item.
 #pragma once
 #include "smx_periodic.h"
 
@@ -22,221 +28,262 @@ namespace smx {
 
 __device__ __forceinline__ int mk_src(int m, int n) { return (m < (n + 1) / 2) ? 2 * m : 2 * (n - 1 - m) + 1; }
 
-// DCT-II (REDFT10) of the NC columns of a real LDS tile [L][RP] in place.
-// Threads: pair q in [0, NC/2), tc in [0, TPC).  F: complex exchange tile [L][FP].
-template <typename T, int L, int NC, int RP, int FP>
-__device__ __forceinline__ void dct2_tile(T* tile, cplx<T>* F, int q, int tc, int tid, int nt,
-                                          const cplx<T>* __restrict__ tw, const cplx<T>* __restrict__ qtw) {
-  constexpr int E = Plan<L>::E, NPR = NC / 2;
-  cplx<T> v[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int m = Tile<T, L, E, FP>::template pos_in<fwd_first<L>()>(tc, e);
-    const int s = mk_src(m, L);
-    v[e] = mk(tile[s * RP + q], tile[s * RP + q + NPR]);
-  }
-  FwdFFT<T, L, FP>::run(v, F, q, tc, tw);
-  __syncthreads();
-  Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
-  __syncthreads();
-  for (int i = tid; i < L * NPR; i += nt) {
-    const int k = i / NPR, qq = i - k * NPR;
-    const cplx<T> Z = F[k * FP + qq], Zm = F[((L - k) % L) * FP + qq];
-    const cplx<T> w = qtw[k];  // e^{-i pi k / 2n}
-    tile[k * RP + qq] = w.x * (Z.x + Zm.x) - w.y * (Z.y - Zm.y);
-    tile[k * RP + qq + NPR] = w.x * (Z.y + Zm.y) + w.y * (Z.x - Zm.x);
-  }
-  __syncthreads();
+// REDFT10 outputs of a pair at k from Z[k] and Zm = Z[(n-k) mod n]; w = e^{-i pi k/2n}.
+template <typename T>
+__device__ __forceinline__ cplx<T> dct2_out(cplx<T> Z, cplx<T> Zm, cplx<T> w) {
+  return mk(w.x * (Z.x + Zm.x) - w.y * (Z.y - Zm.y), w.x * (Z.y + Zm.y) + w.y * (Z.x - Zm.x));
 }
-
-// DCT-III (REDFT01) of the NC columns of a real LDS tile [L][RP] in place.
-template <typename T, int L, int NC, int RP, int FP>
-__device__ __forceinline__ void dct3_tile(T* tile, cplx<T>* F, int q, int tc,
-                                          const cplx<T>* __restrict__ tw, const cplx<T>* __restrict__ qtw) {
-  constexpr int E = Plan<L>::E, NPR = NC / 2;
-  cplx<T> v[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
-    const T X = tile[k * RP + q], Y = tile[k * RP + q + NPR];
-    const T Xn = (k == 0) ? (T)0 : tile[(L - k) * RP + q];
-    const T Yn = (k == 0) ? (T)0 : tile[(L - k) * RP + q + NPR];
-    const cplx<T> w = qtw[k];  // conj gives e^{+i pi k / 2n}
-    const T ux = X + Yn, uy = Y - Xn;
-    v[e] = mk(w.x * ux + w.y * uy, w.x * uy - w.y * ux);
-  }
-  InvFFT<T, L, FP>::run(v, F, q, tc, tw);
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int m = Tile<T, L, E, FP>::template pos_in<inv_last<L>()>(tc, e);
-    const int d = mk_src(m, L);
-    tile[d * RP + q] = v[e].x;
-    tile[d * RP + q + NPR] = v[e].y;
-  }
-  __syncthreads();
+// REDFT01 FFT input at k from the pair at k and its partner pn at n-k (zero for k = 0);
+// w = e^{-i pi k/2n} (its conjugate is applied).
+template <typename T>
+__device__ __forceinline__ cplx<T> dct3_in(cplx<T> p, cplx<T> pn, cplx<T> w) {
+  const T ux = p.x + pn.y, uy = p.y - pn.x;
+  return mk(w.x * ux + w.y * uy, w.x * uy - w.y * ux);
 }
 
 // ---------------------------------------------------------------------------
-// y pass: contiguous real pencils of length L = ny; pencils g = kz*nx + x.  DIR -1: DCT-II, +1: DCT-III.
+// y pass: contiguous real pencils of length L = ny; pencils g = kz*nx + x, pairs of adjacent
+// pencils.  Lanes run along the pencils (tc fastest).  DIR -1: DCT-II, +1: DCT-III.
+// The grid covers whole tiles (nx*nz is a multiple of NC; the launcher checks it).
 template <typename T, int L, int DIR, int NC>
-__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const T* in, T* out, int64_t npencils,
+__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const T* in, T* out,
                                                                            const cplx<T>* __restrict__ tw,
                                                                            const cplx<T>* __restrict__ qtw) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, RP = NC + 1, FP = NPR + 1;
-  __shared__ T tile[L * RP];
-  __shared__ cplx<T> F[L * FP];
-  const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
-  const int64_t p0 = (int64_t)blockIdx.x * NC;
-  const int nvalid = (int)((npencils - p0) < NC ? (npencils - p0) : NC);
-  for (int i = tid; i < NC * L; i += NT) {
-    const int c = i / L, pos = i - c * L;
-    tile[pos * RP + c] = (c < nvalid) ? in[p0 * L + i] : (T)0;
-  }
-  __syncthreads();
-  if constexpr (DIR < 0) dct2_tile<T, L, NC, RP, FP>(tile, F, q, tc, tid, NT, tw, qtw);
-  else dct3_tile<T, L, NC, RP, FP>(tile, F, q, tc, tw, qtw);
-  for (int i = tid; i < NC * L; i += NT) {
-    const int c = i / L, pos = i - c * L;
-    if (c < nvalid) out[p0 * L + i] = tile[pos * RP + c];
+  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
+  using AD = ColAddr<L>;
+  using TL = TileA<T, L, E, AD>;
+  __shared__ cplx<T> S[NPR * AD::LP];
+  const int tid = threadIdx.x, tc = tid % TPC, q = tid / TPC;
+  const int64_t pa = (int64_t)blockIdx.x * NC + 2 * q;  // pencils pa (column a), pa + 1 (column b)
+  const T* a = in + pa * L;
+  const T* b = a + L;
+  cplx<T> v[E];
+  if constexpr (DIR < 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int s = mk_src(TL::template pos_in<fwd_first<L>()>(tc, e), L);
+      v[e] = mk(a[s], b[s]);
+    }
+    FwdFFTA<T, L, AD>::run(v, S, q, tc, tw);
+    __syncthreads();
+    TL::template store<fwd_last<L>()>(v, S, q, tc);
+    __syncthreads();
+    T* oa = out + (int64_t)blockIdx.x * NC * L;
+    for (int i = tid; i < NPR * L; i += NT) {
+      const int qq = i / L, k = i - qq * L;
+      const cplx<T> y = dct2_out(S[AD::at(k, qq)], S[AD::at((L - k) % L, qq)], qtw[k]);
+      oa[(2 * qq) * L + k] = y.x;
+      oa[(2 * qq + 1) * L + k] = y.y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = TL::template pos_in<inv_first<L>()>(tc, e);
+      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : mk(a[L - k], b[L - k]);
+      v[e] = dct3_in(mk(a[k], b[k]), pn, qtw[k]);
+    }
+    InvFFTA<T, L, AD>::run(v, S, q, tc, tw);
+    T* oa = out + pa * L;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int d = mk_src(TL::template pos_in<inv_last<L>()>(tc, e), L);
+      oa[d] = v[e].x;
+      oa[L + d] = v[e].y;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1, rows x.
+// x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1 (pairs of adjacent ky), rows x.
 template <typename T, int L, int NC, int MODE>
 __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const T* in, T* out, T* __restrict__ rbuf,
                                                                            const T* __restrict__ psi0, PParams<T> p,
                                                                            const cplx<T>* __restrict__ qtw) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, RP = NC + 1, FP = NPR + 1;
-  __shared__ T tile[L * RP];
+  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
+  constexpr uint32_t CB = sizeof(cplx<T>);
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
-  const int64_t plane = (int64_t)kz * p.nxy;
-  for (int i = tid; i < NC * L; i += NT) {
-    const int pos = i / NC, c = i - pos * NC;
-    tile[pos * RP + c] = in[plane + (int64_t)pos * p.ny + y0 + c];
-  }
+  const int64_t plane = (int64_t)kz * p.nxy + y0;
+  const uint32_t rowb = (uint32_t)p.ny * sizeof(T);
+  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in + plane);
+  cplx<T>* dst = reinterpret_cast<cplx<T>*>(out + plane);
+  cplx<T> v[E];
   double nsum = 0.0;
   const bool owner0 = (p.partial != nullptr) && kz == 0 && y0 == 0;
-  if (owner0) {
-    double a = 0.0;
-    for (int i = tid; i < p.npartial; i += NT) a += p.partial[i];
-    nsum = block_sum(a, red);
+  if (owner0) {  // FP64 mean mode of the mixed path: sum of the z pass's partials
+    double acc = 0.0;
+    for (int i = tid; i < p.npartial; i += NT) acc += p.partial[i];
+    nsum = block_sum(acc, red);
   }
-  __syncthreads();
-  if constexpr (MODE != X_INV) {
-    dct2_tile<T, L, NC, RP, FP>(tile, F, q, tc, tid, NT, p.twx, qtw);
-    const T kzv = p.kz[kz];
-    for (int i = tid; i < NC * L; i += NT) {
-      const int kxi = i / NC, c = i - kxi * NC;
-      const int64_t idx = plane + (int64_t)kxi * p.ny + y0 + c;
-      T v = tile[kxi * RP + c];
-      if constexpr (MODE == X_FWD_SCALED) {
-        v = p.scale * v;
-        if (owner0 && i == 0) {
-          p.mean_state[1] = p.m_scale * nsum;
-          v = (T)p.mean_state[1];
-        }
-      } else {
-        const T kxv = p.kx[kxi], kyv = p.ky[y0 + c];
-        const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-        const T d = k2 - p.q02;
-        const T aLin = p.eps - p.alpha * d * d;  // cosNoAdvMorph.cpp:431-433
-        const T C1 = (T)1 + p.dtd2 * aLin, C2 = (T)1 - p.dtd2 * aLin;  // :434-435
-        const T nn = p.scale * v;
-        T ps, rn;
-        if constexpr (MODE == X_STEP) {  // :524-525 with R = C1 psi - dt/2 N_past
-          ps = (rbuf[idx] + (T)3 * p.dtd2 * nn) / C2;
-          rn = C1 * ps - p.dtd2 * nn;
-          if (owner0 && i == 0) {
-            const double n0 = p.m_scale * nsum;
-            const double ps0 = (p.mean_state[0] + 3.0 * p.m_dtd2 * n0) / p.m_c2;
-            p.mean_state[0] = p.m_c1 * ps0 - p.m_dtd2 * n0;
-            p.mean_state[1] = ps0;
-            ps = (T)ps0;
-            rn = (T)p.mean_state[0];
-          }
-        } else {  // X_BOOT: N_past = N(psi0), cosNoAdvMorph.cpp:436-442,490
-          ps = psi0[idx];
-          rn = C1 * ps - p.dtd2 * nn;
-          if (owner0 && i == 0) {
-            const double ps0 = p.mean_state[1];
-            p.mean_state[0] = p.m_c1 * ps0 - p.m_dtd2 * p.m_scale * nsum;
-            ps = (T)ps0;
-            rn = (T)p.mean_state[0];
-          }
-        }
-        rbuf[idx] = rn;
-        v = ps;
-      }
-      tile[kxi * RP + c] = v;
+  if constexpr (MODE == X_INV) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
+      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : ld_off(src, (uint32_t)(L - k) * rowb + q * CB);
+      v[e] = dct3_in(ld_off(src, (uint32_t)k * rowb + q * CB), pn, qtw[k]);
     }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int s = mk_src(Tile<T, L, E, FP>::template pos_in<fwd_first<L>()>(tc, e), L);
+      v[e] = ld_off(src, (uint32_t)s * rowb + q * CB);
+    }
+    FwdFFT<T, L, FP>::run(v, F, q, tc, p.twx);
     __syncthreads();
+    Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
+    __syncthreads();
+    // one item: the DCT-II outputs at k and L-k of pair qq, the k-space work there, and the
+    // DCT-III inputs at k and L-k
+    cplx<T>* rb = rbuf ? reinterpret_cast<cplx<T>*>(rbuf + plane) : nullptr;
+    const cplx<T>* p0 = psi0 ? reinterpret_cast<const cplx<T>*>(psi0 + plane) : nullptr;
+    const T kzv = p.kz[kz];
+    for (int i = tid; i < (L / 2 + 1) * NPR; i += NT) {
+      const int k = i / NPR, qq = i - k * NPR;
+      const int km = (L - k) % L;
+      const bool two = (k != 0 && 2 * k != L);
+      const cplx<T> Z = F[k * FP + qq], Zm = F[km * FP + qq];
+      cplx<T> y[2];
+      y[0] = dct2_out(Z, Zm, qtw[k]);
+      y[1] = two ? dct2_out(Zm, Z, qtw[km]) : mk<T>(0, 0);
+      const int kk[2] = {k, km};
+      cplx<T> ps[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (j == 1 && !two) {
+          ps[1] = mk<T>(0, 0);
+          continue;
+        }
+        const uint32_t o = (uint32_t)kk[j] * rowb + qq * CB;
+        if constexpr (MODE == X_FWD_SCALED) {
+          cplx<T> r = p.scale * y[j];
+          if (owner0 && kk[j] == 0 && qq == 0) {  // k = 0 entry of column a (item 0: thread 0)
+            p.mean_state[1] = p.m_scale * nsum;
+            r.x = (T)p.mean_state[1];
+          }
+          st_off(dst, o, r);
+        } else {
+          const T kxv = p.kx[kk[j]];
+          const T kya = p.ky[y0 + 2 * qq], kyb = p.ky[y0 + 2 * qq + 1];
+          const T ka = kxv * kxv + kya * kya + kzv * kzv, kb = kxv * kxv + kyb * kyb + kzv * kzv;
+          const T da = ka - p.q02, db = kb - p.q02;
+          const T La = p.eps - p.alpha * da * da, Lb = p.eps - p.alpha * db * db;  // cosNoAdvMorph.cpp:431-433
+          const T C1a = (T)1 + p.dtd2 * La, C2a = (T)1 - p.dtd2 * La;             // :434-435
+          const T C1b = (T)1 + p.dtd2 * Lb, C2b = (T)1 - p.dtd2 * Lb;
+          const cplx<T> nn = p.scale * y[j];
+          cplx<T> s, rn;
+          if constexpr (MODE == X_STEP) {  // :524-525 with R = C1 psi - dt/2 N_past
+            const cplx<T> r = ld_off<cplx<T>>(rb, o);
+            s = mk((r.x + (T)3 * p.dtd2 * nn.x) * recip(C2a), (r.y + (T)3 * p.dtd2 * nn.y) * recip(C2b));
+            rn = mk(C1a * s.x - p.dtd2 * nn.x, C1b * s.y - p.dtd2 * nn.y);
+            if (owner0 && kk[j] == 0 && qq == 0) {
+              const double n0 = p.m_scale * nsum;
+              const double ps0 = (p.mean_state[0] + 3.0 * p.m_dtd2 * n0) / p.m_c2;
+              p.mean_state[0] = p.m_c1 * ps0 - p.m_dtd2 * n0;
+              p.mean_state[1] = ps0;
+              s.x = (T)ps0;
+              rn.x = (T)p.mean_state[0];
+            }
+          } else {  // X_BOOT: N_past = N(psi0), cosNoAdvMorph.cpp:436-442,490
+            s = ld_off(p0, o);
+            rn = mk(C1a * s.x - p.dtd2 * nn.x, C1b * s.y - p.dtd2 * nn.y);
+            if (owner0 && kk[j] == 0 && qq == 0) {
+              const double ps0 = p.mean_state[1];
+              p.mean_state[0] = p.m_c1 * ps0 - p.m_dtd2 * p.m_scale * nsum;
+              s.x = (T)ps0;
+              rn.x = (T)p.mean_state[0];
+            }
+          }
+          st_off(rb, o, rn);
+          ps[j] = s;
+        }
+      }
+      if constexpr (MODE != X_FWD_SCALED) {
+        // DCT-III inputs (the partner of k = 0 is X_L = Y_L = 0; k = L/2 is its own partner)
+        const cplx<T> pk = ps[0], pm = two ? ps[1] : (k == 0 ? mk<T>(0, 0) : ps[0]);
+        F[k * FP + qq] = dct3_in(pk, pm, qtw[k]);
+        if (two) F[km * FP + qq] = dct3_in(ps[1], pk, qtw[km]);
+      }
+    }
+    if constexpr (MODE == X_FWD_SCALED) return;
+    __syncthreads();
+    Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F, q, tc);
   }
-  if constexpr (MODE != X_FWD_SCALED) dct3_tile<T, L, NC, RP, FP>(tile, F, q, tc, p.twx, qtw);
-  for (int i = tid; i < NC * L; i += NT) {
-    const int pos = i / NC, c = i - pos * NC;
-    out[plane + (int64_t)pos * p.ny + y0 + c] = tile[pos * RP + c];
+  InvFFT<T, L, FP>::run(v, F, q, tc, p.twx);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int d = mk_src(Tile<T, L, E, FP>::template pos_in<inv_last<L>()>(tc, e), L);
+    st_off(dst, (uint32_t)d * rowb + q * CB, v[e]);
   }
 }
 
 // ---------------------------------------------------------------------------
-// z pass.  Tile: columns f0 .. f0+NC-1 of the (x,y) plane, rows z.
-// ZIn / ZOut as in the periodic pass; Z_OUT_REAL writes [x][y][z] (dense).
+// z pass.  Tile: columns f0 .. f0+NC-1 of the (x,y) plane (pairs of adjacent columns), rows z.
+// ZIn / ZOut as in the periodic pass; Z_OUT_REAL writes [x][y][z] (dense).  Whole tiles.
 template <typename T, int L, int NC, int IN, int OUT>
 __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const T* in, T* out,
                                                                            const double* __restrict__ real_in,
                                                                            double* __restrict__ real_out, int op,
                                                                            PParams<T> p,
                                                                            const cplx<T>* __restrict__ qtw) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, RP = NC + 1, FP = NPR + 1;
-  __shared__ T tile[L * RP];
+  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
+  constexpr uint32_t CB = sizeof(cplx<T>);
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   const int64_t f0 = (int64_t)blockIdx.x * NC;
-  const int nvalid = (int)((p.nxy - f0) < NC ? (p.nxy - f0) : NC);
+  const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
+  static_assert(fwd_first<L>() == inv_last<L>(), "plan symmetry");
+  constexpr int RP = fwd_first<L>();
+  cplx<T> v[E];  // (psi_a, psi_b) at z = src(m), m = pos_in<RP>
   if constexpr (IN == Z_IN_SPEC) {
-    for (int i = tid; i < NC * L; i += NT) {
-      const int k = i / NC, c = i - k * NC;
-      tile[k * RP + c] = (c < nvalid) ? in[(int64_t)k * p.nxy + f0 + c] : (T)0;
+    const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in + f0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
+      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : ld_off(src, (uint32_t)(L - k) * rowb + q * CB);
+      v[e] = dct3_in(ld_off(src, (uint32_t)k * rowb + q * CB), pn, qtw[k]);
     }
-    __syncthreads();
-    dct3_tile<T, L, NC, RP, FP>(tile, F, q, tc, p.twz, qtw);
+    InvFFT<T, L, FP>::run(v, F, q, tc, p.twz);
   } else {
     for (int i = tid; i < NC * L; i += NT) {
-      const int c = i / L, n = i - c * L;
-      tile[n * RP + c] = (c < nvalid) ? (T)real_in[(f0 + c) * L + n] : (T)0;
+      const int c = i / L, z = i - c * L;
+      const T val = (T)real_in[(f0 + c) * L + z];
+      if ((c & 1) == 0) F[z * FP + (c >> 1)].x = val;
+      else F[z * FP + (c >> 1)].y = val;
     }
     __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = F[mk_src(Tile<T, L, E, FP>::template pos_in<RP>(tc, e), L) * FP + q];
   }
   if constexpr ((OUT & Z_OUT_REAL) != 0) {
-    for (int i = tid; i < NC * L; i += NT) {
-      const int c = i / L, n = i - c * L;
-      if (c < nvalid) real_out[(f0 + c) * L + n] = (double)tile[n * RP + c];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int z = mk_src(Tile<T, L, E, FP>::template pos_in<RP>(tc, e), L);
+      real_out[(f0 + 2 * q) * L + z] = (double)v[e].x;
+      real_out[(f0 + 2 * q + 1) * L + z] = (double)v[e].y;
     }
   }
   if constexpr ((OUT & Z_OUT_SPEC) != 0) {
     double acc = 0.0;
-    for (int i = tid; i < NC * L; i += NT) {
-      const int n = i / NC, c = i - n * NC;
-      T v = tile[n * RP + c];
-      if (op == Z_OP_NL) v = nonlin(p, v);  // cosNoAdvMorph.cpp:502-503
-      tile[n * RP + c] = v;
-      acc += (double)v;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (op == Z_OP_NL) v[e] = mk(nonlin(p, v[e].x), nonlin(p, v[e].y));  // cosNoAdvMorph.cpp:502-503
+      acc += (double)v[e].x + (double)v[e].y;
     }
     if (p.partial != nullptr) {
       const double s = block_sum(acc, red);
       if (tid == 0) p.partial[blockIdx.x] = s;
     }
+    FwdFFT<T, L, FP>::run(v, F, q, tc, p.twz);
     __syncthreads();
-    dct2_tile<T, L, NC, RP, FP>(tile, F, q, tc, tid, NT, p.twz, qtw);
-    for (int i = tid; i < NC * L; i += NT) {
-      const int k = i / NC, c = i - k * NC;
-      if (c < nvalid) out[(int64_t)k * p.nxy + f0 + c] = tile[k * RP + c];
+    Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
+    __syncthreads();
+    cplx<T>* dst = reinterpret_cast<cplx<T>*>(out + f0);
+    for (int i = tid; i < L * NPR; i += NT) {
+      const int k = i / NPR, qq = i - k * NPR;
+      st_off(dst, (uint32_t)k * rowb + qq * CB, dct2_out(F[k * FP + qq], F[((L - k) % L) * FP + qq], qtw[k]));
     }
   }
 }
