@@ -623,10 +623,6 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
   double *ckmean = nullptr, *acc = nullptr;
   const int nxb = c.ny / kNC, nblk = nxb * h->nplanes;
   int rc = SMX_OK;
-  if (!h->adj_stage) HIPCHK(hipMalloc(&h->adj_stage, sbytes));
-  if (!h->adj_carry) HIPCHK(hipMalloc(&h->adj_carry, sbytes));
-  AS = h->adj_stage;
-  AC = h->adj_carry;
   if ((rc = talloc((void**)&acc, sizeof(double) * 2 * nblk)) ||
       (rc = talloc((void**)&ckmean, sizeof(double) * 2 * nseg))) {
     cleanup();
@@ -653,6 +649,15 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     h->adj_np.push_back(a);
     h->adj_psi.push_back(b);
   }
+  // stage and carry after the record sets: a change of segment length releases the
+  // whole workspace above
+  if ((!h->adj_stage && hipMalloc(&h->adj_stage, sbytes) != hipSuccess) ||
+      (!h->adj_carry && hipMalloc(&h->adj_carry, sbytes) != hipSuccess)) {
+    cleanup();
+    return fail(SMX_ERR_HIP, "hipMalloc of the adjoint stage and carry failed");
+  }
+  AS = h->adj_stage;
+  AC = h->adj_carry;
   std::vector<char*> setnp, setpsi;
   for (size_t i = 0; i < h->adj_np.size(); ++i) {
     setnp.push_back((char*)h->adj_np[i]);
