@@ -204,6 +204,32 @@ def run_adjoint(steps, ckpt):
             "loss": J, "dJ_deps": de, "dJ_dq0": dq}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "periodic_256_fp32_traffic.json")
+
+
+def measured_traffic(slot, schedule, n):
+    """HBM bytes per launch of the kernel behind step slot `slot` under `schedule`, from the
+    committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (each counter calibrated against the
+    copy kernel of known size); None if that instantiation was not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            kernels = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    slots = ["k_ypass_inv", "k_zpass_nl", "k_ypass_fwd", "k_xpass_step"]
+    if slot not in slots:
+        return None
+    nc, v = schedule[slots.index(slot)]
+    inst = {"k_ypass_inv": f"smx::k_ypass<float, {n}, 1, {nc}, {v}>",
+            "k_zpass_nl": f"smx::k_zpass<float, {n}, {nc}, 0, 1, {v}>",
+            "k_ypass_fwd": f"smx::k_ypass<float, {n}, -1, {nc}, {v}>",
+            "k_xpass_step": f"smx::k_xpass<float, {n}, {nc}, 0, {v}>"}[slot]
+    k = kernels.get(inst)
+    if not k or "FETCH_SIZE_bytes" not in k or "WRITE_SIZE_bytes" not in k:
+        return None
+    return k["FETCH_SIZE_bytes"] + k["WRITE_SIZE_bytes"]
+
+
 def cpu_baseline(psi0, budget_s=12.0):
     """The FP64 oracle (a C port of the reference loop, OpenMP threads) timed on the host,
     on a bounded sample of the same workload: whole steps at 256^3 until ~budget_s."""
@@ -233,12 +259,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the Neumann/advected/batch/adjoint legs")
     ap.add_argument("--no-tune", action="store_true", help="default tile schedules (no autotuning)")
+    ap.add_argument("--leg", choices=["neumann", "advected", "batch", "adjoint"],
+                    help="run only this secondary workload (with --leg-steps steps) and print its result")
+    ap.add_argument("--leg-steps", type=int, default=None)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
               "(multi-GPU sharding is a design note, not an implemented path).", file=sys.stderr)
         sys.exit(2)
     from smectic3d_amd import _abi as A
+
+    if a.leg:  # one secondary workload alone (profiling runs)
+        runs = {"neumann": (run_neumann, 4000), "advected": (run_advected, 1000),
+                "batch": (run_batch, 500), "adjoint": (lambda k: run_adjoint(k, 50), 2000)}
+        fn, default_steps = runs[a.leg]
+        print(json.dumps({"leg": a.leg, "result": fn(a.leg_steps or default_steps)}, default=str))
+        return
 
     n = 256
     dx = (n / 8 * 2 * math.pi) / n
@@ -296,7 +332,8 @@ def main():
         "kernels_fp64": [{"name": k, "avg_ms": m, "alg_bytes": b, "GBs": b / (m * 1e-3) / 1e9}
                          for k, m, b in r64["kernels"]],
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(name, r32["schedule"], n)},
         "cpu_baseline": None,
     }
     if not a.no_extra:

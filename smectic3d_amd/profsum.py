@@ -8,8 +8,8 @@
 table (calls, total and average duration in microseconds).  `pmc` averages each
 counter per kernel over its dispatches; FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
 are converted to bytes, and when --copy-bytes gives the bytes one launch of the plain
-copy kernel k_copy_f4 reads (and writes), the counters of that kernel calibrate a
-factor bytes_counted / bytes_moved that the other kernels' values are divided by.
+copy kernel k_copy_f4 reads (and writes), that kernel's two counters calibrate one
+factor each (counted bytes / moved bytes) that the other kernels' values are divided by.
 """
 import argparse
 import csv
@@ -69,26 +69,29 @@ def pmc(paths, cmd, out, json_out=None, copy_bytes=None):
     per = defaultdict(dict)
     for (kname, cname), (s, n) in acc.items():
         per[kname][cname] = (s / n, n)
-    factor = None
+    factor = {}
     if copy_bytes:
         for kname, cs in per.items():
-            if "k_copy_f4" in kname and "FETCH_SIZE" in cs:
-                factor = cs["FETCH_SIZE"][0] * 1024.0 / float(copy_bytes)
+            if "k_copy_f4" in kname:
+                for cname in ("FETCH_SIZE", "WRITE_SIZE"):
+                    if cname in cs:
+                        factor[cname] = cs[cname][0] * 1024.0 / float(copy_bytes)
     res = {}
     with open(out, "w") as f:
         f.write("# rocprofv3 --pmc summary (MI355X, one GPU): counter averages per dispatch\n")
         f.write(f"# command: {cmd}\n")
-        if factor:
-            f.write(f"# calibration: k_copy_f4 moves {copy_bytes} B read + {copy_bytes} B write per launch; "
-                    f"FETCH_SIZE reads {factor:.4f} x the bytes read (factor applied below)\n")
+        for cname, fac in sorted(factor.items()):
+            f.write(f"# calibration: k_copy_f4 moves {copy_bytes} B read and {copy_bytes} B written per launch; "
+                    f"{cname} counts {fac:.4f} x the bytes moved (divided out below)\n")
         f.write("# columns: dispatches, counter, average (KiB), calibrated MB, kernel\n")
         for kname in sorted(per):
             for cname, (avg, n) in sorted(per[kname].items()):
-                mb = avg * 1024.0 / (factor or 1.0) / 1e6 if cname in ("FETCH_SIZE", "WRITE_SIZE") else float("nan")
+                b = avg * 1024.0 / factor.get(cname, 1.0)
+                mb = b / 1e6 if cname in ("FETCH_SIZE", "WRITE_SIZE") else float("nan")
                 f.write(f"{n:8d} {cname:12s} {avg:14.1f} {mb:10.2f}  {kname}\n")
-                res.setdefault(kname, {})[cname + "_bytes"] = avg * 1024.0 / (factor or 1.0)
+                res.setdefault(kname, {})[cname + "_bytes"] = b
     if json_out:
-        json.dump({"command": cmd, "copy_factor": factor, "kernels": res}, open(json_out, "w"), indent=1)
+        json.dump({"command": cmd, "copy_factors": factor, "kernels": res}, open(json_out, "w"), indent=1)
 
 
 def main(argv=None):
