@@ -137,7 +137,9 @@ def run_advected(steps):
         s.sync()
         t1 = time.perf_counter()
         e = s.energy()
-    return {"steps": steps, "steps_per_s": steps / (t1 - t0), "energy": e}
+        ks = s.profile_step(10)
+    return {"steps": steps, "steps_per_s": steps / (t1 - t0), "energy": e,
+            "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
 
 
 def run_batch(steps, tune=True):

@@ -272,14 +272,6 @@ static int zpass(smx_handle* h, const PParams<T>& p, int in_mode, int out_mode, 
   return SMX_OK;
 }
 
-template <typename T>
-static int herm(smx_handle* h, const PParams<T>& p, void* a) {
-  if (h->neumann) return SMX_OK;  // real-to-real transforms carry no redundant modes
-  hipError_t e = PeriodicOps<T>::herm_project((cplx<T>*)a, p, h->st);
-  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("herm: ") + hipGetErrorString(e));
-  return SMX_OK;
-}
-
 // Bootstrap from real psi in h->real_io: P = psi_hat (normalised), R = C1 psi_hat - dt/2 N_hat(psi),
 // A = x-inverse(psi_hat).  pf3d.cpp:363-374, pf3dMorph.cpp:451-472, cosNoAdvMorph.cpp:423-442.
 template <typename T>
@@ -297,7 +289,6 @@ static int bootstrap(smx_handle* h) {
   } else {
     CHK(xpass<T>(h, p, X_BOOT, h->A, h->A, h->R, h->P));
   }
-  if (h->cfg.advect) CHK(herm<T>(h, p, h->R));  // the plain x pass projects R itself
   return SMX_OK;
 }
 
@@ -307,8 +298,8 @@ static const char* kSlotNamesN[] = {"k_dct_ypass_inv", "k_dct_zpass_nl", "k_dct_
 static const char* kSlotNamesA[] = {"k_ypass_inv_dy", "k_ypass_inv_psi", "k_ypass_inv_dx", "k_ypass_inv_sh",
                                     "k_zpass_force", "k_ypass_fwd_s0", "k_ypass_fwd_s1", "k_ypass_fwd_s2",
                                     "k_xpass_stokes", "k_ypass_inv_u0", "k_ypass_inv_u1", "k_ypass_inv_u2",
-                                    "k_zpass_advnl", "k_ypass_fwd_n", "k_xpass_adv", "k_herm_project"};
-static int num_slots(const smx_handle* h) { return h->cfg.advect ? 16 : 4; }
+                                    "k_zpass_advnl", "k_ypass_fwd_n", "k_xpass_adv"};
+static int num_slots(const smx_handle* h) { return h->cfg.advect ? 15 : 4; }
 static const char* slot_name(const smx_handle* h, int s) {
   return h->neumann ? kSlotNamesN[s] : (h->cfg.advect ? kSlotNamesA[s] : kSlotNamesP[s]);
 }
@@ -342,7 +333,7 @@ static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
 #define SMX_REC()                                             \
   do {                                                        \
     if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e)); \
-    if (++slot < 16) tm.arm(slot); /* 16 kernels per step */ \
+    if (++slot < 15) tm.arm(slot); /* 15 kernels per step */ \
   } while (0)
   tm.arm(0);
   e = Ops::ypass(A, By, p.ny, h->npencils, +1, t.twy, h->st, t.ky);  // d_y psi (pf3d.cpp:405,411)
@@ -373,9 +364,7 @@ static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
   SMX_REC();
   e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st);
   SMX_REC();
-  e = Adv::xpass_adv(X_STEP, As, R, nullptr, A, Ax, As, p, h->st);  // CNAB2 (:471-481)
-  SMX_REC();
-  e = Ops::herm_project(R, p, h->st);
+  e = Adv::xpass_adv(X_STEP, As, R, nullptr, A, Ax, As, p, h->st);  // CNAB2 (:471-481); projects R
   SMX_REC();
 #undef SMX_REC
   return SMX_OK;
@@ -385,6 +374,7 @@ template <typename T>
 static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
   if (h->cfg.advect) {
     AParams<T> ap = aparams<T>(h, h->fp32);
+    ap.p.npartial = AdvectOps<T>::zpass_blocks(ap.p);  // FP64 partials of the advected z pass
     return step_advect<T>(h, ap, ev);
   }
   SlotTimer tm(ev);
@@ -1426,11 +1416,10 @@ int smx_profile_step(smx_handle* h, int64_t nsteps, int max_slots, int* nslots, 
   std::vector<double> acc;
   CHK(h->fp32 ? profile_T<float>(h, nsteps, acc) : profile_T<double>(h, nsteps, acc));
   const double sc = (double)h->wbytes * (double)h->nspec;  // bytes of one spectral-stage array
-  const double plane = (double)h->wbytes * (double)h->nxy;
   std::vector<double> bytes;
   if (h->cfg.advect)  // reads + writes of spectral-stage arrays per launch
     bytes = {2 * sc, 2 * sc, 2 * sc, 2 * sc, 7 * sc, 2 * sc, 2 * sc, 2 * sc,
-             6 * sc, 2 * sc, 2 * sc, 2 * sc, 7 * sc, 2 * sc, 6 * sc, 2 * 2 * plane};
+             6 * sc, 2 * sc, 2 * sc, 2 * sc, 7 * sc, 2 * sc, 6 * sc};
   else
     bytes = {2 * sc, 2 * sc, 2 * sc, 4 * sc};
   *nslots = ns;

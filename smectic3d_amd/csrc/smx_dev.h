@@ -115,6 +115,20 @@ __device__ __forceinline__ void dft(cplx<T>* a) {
 }
 
 // ---------------------------------------------------------------------------
+// Global access at a 32-bit byte offset from a wave-uniform base pointer: the base
+// stays in SGPRs and each lane carries one 32-bit offset (the saddr form of
+// global_load/global_store) instead of a 64-bit address pair.  Every array of the
+// solver is below 4 GiB (a 512^3 FP64 half spectrum is 1.08 GB).
+template <typename X>
+__device__ __forceinline__ X ld_off(const X* base, uint32_t off) {
+  return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename X>
+__device__ __forceinline__ void st_off(X* base, uint32_t off, const X& v) {
+  *reinterpret_cast<X*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// ---------------------------------------------------------------------------
 // Radix plans.  Fwd and Inv list the same radices in opposite orders.
 template <int... Rs> struct Radices {};
 template <int R0, int... Rs> struct First { static constexpr int value = R0; };
@@ -219,7 +233,7 @@ __device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, in
       const int k = b % Ns;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        cplx<T> w = tw[r * k * (L / (Ns * R))];
+        cplx<T> w = ld_off(tw, (uint32_t)(r * k * (L / (Ns * R))) * (uint32_t)sizeof(cplx<T>));
         if (SIGN > 0) w.y = -w.y;
         v[u * R + r] = cmul(v[u * R + r], w);
       }
@@ -266,20 +280,6 @@ template <int L, int V = 0> constexpr int fwd_first() { return RadInfo<typename 
 template <int L, int V = 0> constexpr int fwd_last() { return RadInfo<typename PlanV<L, V>::Fwd>::last; }
 template <int L, int V = 0> constexpr int inv_first() { return RadInfo<typename PlanV<L, V>::Inv>::first; }
 template <int L, int V = 0> constexpr int inv_last() { return RadInfo<typename PlanV<L, V>::Inv>::last; }
-
-// ---------------------------------------------------------------------------
-// Global access at a 32-bit byte offset from a wave-uniform base pointer: the base
-// stays in SGPRs and each lane carries one 32-bit offset (the saddr form of
-// global_load/global_store) instead of a 64-bit address pair.  Every array of the
-// solver is below 4 GiB (a 512^3 FP64 half spectrum is 1.08 GB).
-template <typename X>
-__device__ __forceinline__ X ld_off(const X* base, uint32_t off) {
-  return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + off);
-}
-template <typename X>
-__device__ __forceinline__ void st_off(X* base, uint32_t off, const X& v) {
-  *reinterpret_cast<X*>(reinterpret_cast<char*>(base) + off) = v;
-}
 
 // ---------------------------------------------------------------------------
 // FP64 block reduction helpers (wavefront shuffles, then LDS across waves).

@@ -93,13 +93,6 @@ struct PeriodicOps {
                           const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st,
                           PassSched s = PassSched{}, bool dry = false);
   static int zpass_blocks(const PParams<T>& p, int nc = kNC) { return (int)((p.nxy + nc - 1) / nc); }
-  // Hermitian projection of the kz = 0 and kz = nz/2 planes of a full spectrum.
-  static hipError_t herm_project(cplx<T>* a, const PParams<T>& p, hipStream_t st) {
-    const int64_t work = 2 * p.nxy;
-    const unsigned blocks = (unsigned)std::min<int64_t>((work + 255) / 256, 4096);
-    SMX_LAUNCH((k_herm_project<T>), dim3(blocks), dim3(256), 0, st, a, p.nx, p.ny, p.nzh, p.nz);
-    return hipGetLastError();
-  }
 };
 
 template <typename T>
@@ -301,13 +294,6 @@ hipError_t NeumannOps<T>::zpass(int in_mode, int out_mode, int op, const T* in, 
 }
 
 
-// Widest tile (16, 8 or 4 columns) whose `ntiles` LDS tiles of L x (NC+1) complex fit in 150 KB.
-template <typename T, int L>
-constexpr int nc_fit(int ntiles) {
-  return (ntiles * L * 17 * (int)sizeof(cplx<T>) <= 150 * 1024) ? 16
-         : (ntiles * L * 9 * (int)sizeof(cplx<T>) <= 150 * 1024) ? 8
-                                                                  : 4;
-}
 
 // Plan of the adjoint x pass: the E = 8 variant where there is one and its block stays at
 // <= 512 threads (half the registers per thread of the default plan).
@@ -334,19 +320,30 @@ hipError_t AdjointOps<T>::xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T
   return hipErrorInvalidValue;
 }
 
+// Plan of a pass kernel that keeps several fields in registers: the E = 8 variant where
+// there is one and the block stays at <= 512 threads.
+template <int L, int LANES>
+constexpr int light_variant() {
+  if constexpr (HasPlanV<L, 1>::value) return (LANES * (L / PlanV<L, 1>::E) <= 512) ? 1 : 0;
+  else return 0;
+}
+
 template <typename T>
 struct AdvectOps {
+  static constexpr int NCX = 16;                         // x kernels: 16 columns per tile
+  static constexpr int NCZ = sizeof(T) == 4 ? 32 : 16;   // z kernels: 128-byte rows
   static hipError_t xpass_adv(int mode, const cplx<T>* in, cplx<T>* rbuf, const cplx<T>* psi0, cplx<T>* oa,
                               cplx<T>* ox, cplx<T>* os, const PParams<T>& p, hipStream_t st) {
+    if (p.ny % NCX) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.ny / NCX), (unsigned)p.nzh);
 #define SMX_CASE(L)                                                                                       \
   if (p.nx == L) {                                                                                        \
-    constexpr int NC = nc_fit<T, L>(2);                                                                   \
-    if (p.ny % NC) return hipErrorInvalidValue;                                                           \
-    const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block(NC * (L / Plan<L>::E));                \
+    constexpr int V = light_variant<L, NCX>();                                                            \
+    const dim3 block(NCX * (L / PlanV<L, V>::E));                                                         \
     if (mode == X_STEP)                                                                                   \
-      SMX_LAUNCH((k_xpass_adv<T, L, NC, X_STEP>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
+      SMX_LAUNCH((k_xpass_adv<T, L, NCX, X_STEP, V>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p);  \
     else                                                                                                  \
-      SMX_LAUNCH((k_xpass_adv<T, L, NC, X_BOOT>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p); \
+      SMX_LAUNCH((k_xpass_adv<T, L, NCX, X_BOOT, V>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p);  \
     return hipGetLastError();                                                                             \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -355,12 +352,12 @@ struct AdvectOps {
   }
   static hipError_t stokes(cplx<T>* s0, cplx<T>* s1, cplx<T>* s2, const AParams<T>& ap, hipStream_t st) {
     const PParams<T>& p = ap.p;
+    if (p.ny % NCX) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.ny / NCX), (unsigned)p.nzh);
 #define SMX_CASE(L)                                                                                \
   if (p.nx == L) {                                                                                 \
-    constexpr int NC = nc_fit<T, L>(3);                                                            \
-    if (p.ny % NC) return hipErrorInvalidValue;                                                    \
-    const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block(NC * (L / Plan<L>::E));         \
-    SMX_LAUNCH((k_xpass_stokes<T, L, NC>), grid, block, 0, st, s0, s1, s2, ap);            \
+    constexpr int V = light_variant<L, NCX>();                                                     \
+    SMX_LAUNCH((k_xpass_stokes<T, L, NCX, V>), grid, dim3(NCX * (L / PlanV<L, V>::E)), 0, st, s0, s1, s2, ap); \
     return hipGetLastError();                                                                      \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -370,11 +367,13 @@ struct AdvectOps {
   static hipError_t zforce(const cplx<T>* bpsi, const cplx<T>* bx, const cplx<T>* by, const cplx<T>* bs,
                            cplx<T>* o0, cplx<T>* o1, cplx<T>* o2, const AParams<T>& ap, hipStream_t st) {
     const PParams<T>& p = ap.p;
-    const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
+    if (p.nxy % NCZ) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.nxy / NCZ));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    SMX_LAUNCH((k_zpass_force<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, bpsi, bx, by, \
-                       bs, o0, o1, o2, ap);                                                             \
+    constexpr int V = light_variant<L, NCZ / 2>();                                                      \
+    SMX_LAUNCH((k_zpass_force<T, L, NCZ, V>), grid, dim3((NCZ / 2) * (L / PlanV<L, V>::E)), 0, st, bpsi, bx, by, \
+               bs, o0, o1, o2, ap);                                                                     \
     return hipGetLastError();                                                                           \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -384,17 +383,21 @@ struct AdvectOps {
   static hipError_t zadvnl(const cplx<T>* u0, const cplx<T>* u1, const cplx<T>* u2, const cplx<T>* bpsi,
                            const cplx<T>* bx, const cplx<T>* by, cplx<T>* nout, const AParams<T>& ap, hipStream_t st) {
     const PParams<T>& p = ap.p;
-    const dim3 grid((unsigned)((p.nxy + kNC - 1) / kNC));
+    if (p.nxy % NCZ) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.nxy / NCZ));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    SMX_LAUNCH((k_zpass_advnl<T, L, kNC>), grid, dim3((kNC / 2) * (L / Plan<L>::E)), 0, st, u0, u1, u2, \
-                       bpsi, bx, by, nout, ap);                                                         \
+    constexpr int V = light_variant<L, NCZ / 2>();                                                      \
+    SMX_LAUNCH((k_zpass_advnl<T, L, NCZ, V>), grid, dim3((NCZ / 2) * (L / PlanV<L, V>::E)), 0, st, u0, u1, u2, \
+               bpsi, bx, by, nout, ap);                                                                 \
     return hipGetLastError();                                                                           \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
     return hipErrorInvalidValue;
   }
+  // partial sums written by zadvnl: one per z tile
+  static int zpass_blocks(const PParams<T>& p) { return (int)(p.nxy / NCZ); }
 };
 
 }  // namespace smx

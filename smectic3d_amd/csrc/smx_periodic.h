@@ -121,7 +121,9 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
   // On the self-conjugate planes kz = 0 and kz = nz/2 a block takes NC/2 columns ky and
   // their partners -ky, so that R can be projected onto Hermitian symmetry in place
   // (the carried state otherwise accumulates an anti-Hermitian part there that the
-  // C2R never sees and that grows in the unstable band; see k_herm_project).
+  // C2R never sees -- it reads only real parts on those planes -- and that grows by
+  // (1 + dt L/2)/(1 - dt L/2) per step in the unstable band L(k) > 0; the reference
+  // re-derives psi_hat from real psi each step, pf3d.cpp:395, and never carries it).
   constexpr int H = NC / 2;
   const bool herm = (MODE == X_STEP || MODE == X_BOOT) && (kz == 0 || 2 * kz == p.nz);
   int ky = y0 + col, pcol = col;
@@ -231,38 +233,6 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
 #pragma unroll
     for (int e = 0; e < E; ++e)
       st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Hermitian projection of the self-conjugate planes kz = 0 and kz = nz/2 of a
-// full spectrum [kz][kx][ky]: X(k) <- (X(k) + conj X(-k))/2.  The reference
-// re-derives psi_hat from a real field every step (R2C of psi, pf3d.cpp:395),
-// which keeps these planes Hermitian; the carried CNAB2 state has to be
-// projected explicitly, or the rounding-level anti-Hermitian part -- invisible
-// to the C2R, which reads only real parts there -- grows by
-// (1 + dt L/2)/(1 - dt L/2) per step in the unstable band L(k) > 0.
-// One thread per conjugate pair; self-conjugate entries lose their imaginary part.
-template <typename T>
-__global__ void k_herm_project(cplx<T>* __restrict__ a, int nx, int ny, int nzh, int nz) {
-  const int64_t nxy = (int64_t)nx * ny;
-  const int nplanes = (2 * (nzh - 1) == nz) ? 2 : 1;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nplanes * nxy;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int pl = (int)(t / nxy);
-    const int64_t i = t - pl * nxy;
-    const int ix = (int)(i / ny), iy = (int)(i - (int64_t)ix * ny);
-    const int jx = (nx - ix) % nx, jy = (ny - iy) % ny;
-    const int64_t j = (int64_t)jx * ny + jy;
-    cplx<T>* p = a + (int64_t)(pl == 0 ? 0 : nzh - 1) * nxy;
-    if (i == j) {
-      p[i].y = (T)0;
-    } else if (i < j) {
-      const cplx<T> u = p[i], w = p[j];
-      const cplx<T> m = mk((T)0.5 * (u.x + w.x), (T)0.5 * (u.y - w.y));
-      p[i] = m;
-      p[j] = conjg(m);
-    }
   }
 }
 
