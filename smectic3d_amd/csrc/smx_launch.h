@@ -328,17 +328,24 @@ constexpr int light_variant() {
   else return 0;
 }
 
+// Tiles of the advected-step kernels (measured at 256^3): the z kernels take 128-byte rows and,
+// in FP32, the E = 16 plan (fewer exchanges; FP64 keeps the E = 8 plan for registers); the
+// Stokes x pass, which holds three fields, takes 8-column tiles and the E = 8 plan; the
+// CNAB2 x pass takes 16 columns and, in FP32, the E = 16 plan.
 template <typename T>
 struct AdvectOps {
-  static constexpr int NCX = 16;                         // x kernels: 16 columns per tile
-  static constexpr int NCZ = sizeof(T) == 4 ? 32 : 16;   // z kernels: 128-byte rows
+  static constexpr int NCX = 16;                         // x pass with the CNAB2 update
+  static constexpr int NCS = 8;                          // Stokes x pass
+  static constexpr int NCZ = sizeof(T) == 4 ? 32 : 16;   // z kernels
+  template <int L> static constexpr int vx() { return sizeof(T) == 4 ? 0 : light_variant<L, NCX>(); }
+  template <int L> static constexpr int vz() { return sizeof(T) == 4 ? 0 : light_variant<L, NCZ / 2>(); }
   static hipError_t xpass_adv(int mode, const cplx<T>* in, cplx<T>* rbuf, const cplx<T>* psi0, cplx<T>* oa,
                               cplx<T>* ox, cplx<T>* os, const PParams<T>& p, hipStream_t st) {
     if (p.ny % NCX) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(p.ny / NCX), (unsigned)p.nzh);
 #define SMX_CASE(L)                                                                                       \
   if (p.nx == L) {                                                                                        \
-    constexpr int V = light_variant<L, NCX>();                                                            \
+    constexpr int V = vx<L>();                                                                            \
     const dim3 block(NCX * (L / PlanV<L, V>::E));                                                         \
     if (mode == X_STEP)                                                                                   \
       SMX_LAUNCH((k_xpass_adv<T, L, NCX, X_STEP, V>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p);  \
@@ -352,12 +359,12 @@ struct AdvectOps {
   }
   static hipError_t stokes(cplx<T>* s0, cplx<T>* s1, cplx<T>* s2, const AParams<T>& ap, hipStream_t st) {
     const PParams<T>& p = ap.p;
-    if (p.ny % NCX) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(p.ny / NCX), (unsigned)p.nzh);
+    if (p.ny % NCS) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.ny / NCS), (unsigned)p.nzh);
 #define SMX_CASE(L)                                                                                \
   if (p.nx == L) {                                                                                 \
-    constexpr int V = light_variant<L, NCX>();                                                     \
-    SMX_LAUNCH((k_xpass_stokes<T, L, NCX, V>), grid, dim3(NCX * (L / PlanV<L, V>::E)), 0, st, s0, s1, s2, ap); \
+    constexpr int V = light_variant<L, NCS>();                                                     \
+    SMX_LAUNCH((k_xpass_stokes<T, L, NCS, V>), grid, dim3(NCS * (L / PlanV<L, V>::E)), 0, st, s0, s1, s2, ap); \
     return hipGetLastError();                                                                      \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -371,7 +378,7 @@ struct AdvectOps {
     const dim3 grid((unsigned)(p.nxy / NCZ));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    constexpr int V = light_variant<L, NCZ / 2>();                                                      \
+    constexpr int V = vz<L>();                                                                          \
     SMX_LAUNCH((k_zpass_force<T, L, NCZ, V>), grid, dim3((NCZ / 2) * (L / PlanV<L, V>::E)), 0, st, bpsi, bx, by, \
                bs, o0, o1, o2, ap);                                                                     \
     return hipGetLastError();                                                                           \
@@ -387,7 +394,7 @@ struct AdvectOps {
     const dim3 grid((unsigned)(p.nxy / NCZ));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    constexpr int V = light_variant<L, NCZ / 2>();                                                      \
+    constexpr int V = vz<L>();                                                                          \
     SMX_LAUNCH((k_zpass_advnl<T, L, NCZ, V>), grid, dim3((NCZ / 2) * (L / PlanV<L, V>::E)), 0, st, u0, u1, u2, \
                bpsi, bx, by, nout, ap);                                                                 \
     return hipGetLastError();                                                                           \
