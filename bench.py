@@ -129,6 +129,7 @@ def run_advected(steps):
                         precision=A.SMX_FP32_MIXED)
     rng = np.random.default_rng(4)
     with Solver(cfg) as s:
+        s.autotune(10)
         s.set_psi(rng.uniform(-0.05, 0.05, (n, n, n)))
         s.step(5)
         s.sync()

@@ -189,7 +189,8 @@ int smx_batch_graph_nodes(smx_batch *b, int64_t *nodes);
  * y pass, 1: z pass with N(psi), 2: forward y pass, 3: x pass with the CNAB2 update): the
  * columns per tile (8, 16 or 32) and the radix-plan variant (0: default; 1: the alternative
  * plan of the power-of-two lengths).  The step's result depends on the schedule only
- * through rounding.  The default is nc = 16, variant = 0 in every slot.
+ * through rounding.  The default is nc = 16, variant = 0 in every slot.  The advected step
+ * uses the schedules of slots 0 and 2 for its eleven y passes.
  */
 typedef struct smx_schedule {
   int nc[4];
@@ -198,14 +199,14 @@ typedef struct smx_schedule {
 int smx_get_schedule(const smx_handle *h, smx_schedule *s);
 /* Install a schedule; SMX_ERR_UNSUPPORTED if some slot has no kernel for this grid (too
  * many threads or too much LDS per tile, or ny / nx*ny not a multiple of nc), or if a
- * non-default schedule is given to a Neumann or advected handle. */
+ * non-default schedule is given to a Neumann handle. */
 int smx_set_schedule(smx_handle *h, const smx_schedule *s);
 /* Autotuner: time every valid candidate of every slot (reps launches each, HIP events on
  * the handle's stream) on scratch buffers -- the state is not touched -- install the
  * fastest and return it in *chosen (may be NULL).  Results are cached per (device,
  * precision, grid) for the life of the process.  Replaces the FFTW_MEASURE planning of
- * pf3d.cpp:226-291 (which times candidate FFT algorithms at set-up).  Periodic,
- * non-advected handles only; not callable during stream capture. */
+ * pf3d.cpp:226-291 (which times candidate FFT algorithms at set-up).  Periodic handles
+ * only; not callable during stream capture. */
 int smx_autotune(smx_handle *h, int reps, smx_schedule *chosen);
 /* smx_autotune on every box (boxes of one shape are tuned once); the next smx_batch_run
  * captures its graph again with the new schedules. */

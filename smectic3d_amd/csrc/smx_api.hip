@@ -330,39 +330,40 @@ static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
   int slot = 0;
   hipError_t e = hipSuccess;
   SlotTimer tm(ev);
+  const PassSched yi = h->sched[SLOT_YINV], yf = h->sched[SLOT_YFWD];  // tuned y passes
 #define SMX_REC()                                             \
   do {                                                        \
     if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e)); \
     if (++slot < 15) tm.arm(slot); /* 15 kernels per step */ \
   } while (0)
   tm.arm(0);
-  e = Ops::ypass(A, By, p.ny, h->npencils, +1, t.twy, h->st, t.ky);  // d_y psi (pf3d.cpp:405,411)
+  e = Ops::ypass(A, By, p.ny, h->npencils, +1, t.twy, h->st, t.ky, yi);  // d_y psi (pf3d.cpp:405,411)
   SMX_REC();
-  e = Ops::ypass(A, A, p.ny, h->npencils, +1, t.twy, h->st);
+  e = Ops::ypass(A, A, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
   SMX_REC();
-  e = Ops::ypass(Ax, Ax, p.ny, h->npencils, +1, t.twy, h->st);
+  e = Ops::ypass(Ax, Ax, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
   SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st);
+  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
   SMX_REC();
   e = Adv::zforce(A, Ax, By, As, As, S1, S2, ap, h->st);  // S = f grad psi (:417-425)
   SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st);
+  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
   SMX_REC();
-  e = Ops::ypass(S1, S1, p.ny, h->npencils, -1, t.twy, h->st);
+  e = Ops::ypass(S1, S1, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
   SMX_REC();
-  e = Ops::ypass(S2, S2, p.ny, h->npencils, -1, t.twy, h->st);
+  e = Ops::ypass(S2, S2, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
   SMX_REC();
   e = Adv::stokes(As, S1, S2, ap, h->st);  // (:433-447)
   SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st);
+  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
   SMX_REC();
-  e = Ops::ypass(S1, S1, p.ny, h->npencils, +1, t.twy, h->st);
+  e = Ops::ypass(S1, S1, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
   SMX_REC();
-  e = Ops::ypass(S2, S2, p.ny, h->npencils, +1, t.twy, h->st);
+  e = Ops::ypass(S2, S2, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
   SMX_REC();
   e = Adv::zadvnl(As, S1, S2, A, Ax, By, As, ap, h->st);  // N = N(psi) - U.grad psi (:455-465)
   SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st);
+  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
   SMX_REC();
   e = Adv::xpass_adv(X_STEP, As, R, nullptr, A, Ax, As, p, h->st);  // CNAB2 (:471-481); projects R
   SMX_REC();
@@ -1285,8 +1286,7 @@ int smx_set_schedule(smx_handle* h, const smx_schedule* s) {
     c[k] = PassSched{s->nc[k], s->variant[k]};
     is_default = is_default && c[k].nc == kNC && c[k].v == 0;
   }
-  if (!is_default && (h->neumann || h->cfg.advect))
-    return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic non-advected step only");
+  if (!is_default && h->neumann) return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic step only");
   HIPCHK(hipSetDevice(h->cfg.device));
   CHK(h->fp32 ? sched_check<float>(h, c) : sched_check<double>(h, c));
   for (int k = 0; k < NSLOTS; ++k) h->sched[k] = c[k];
@@ -1296,8 +1296,7 @@ int smx_set_schedule(smx_handle* h, const smx_schedule* s) {
 int smx_autotune(smx_handle* h, int reps, smx_schedule* chosen) {
   if (!h) return fail(SMX_ERR_INVALID, "null handle");
   if (reps < 1 || reps > 1000) return fail(SMX_ERR_INVALID, "reps must lie in [1, 1000]");
-  if (h->neumann || h->cfg.advect)
-    return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic non-advected step only");
+  if (h->neumann) return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic step only");
   HIPCHK(hipSetDevice(h->cfg.device));
   const TuneKey key{h->cfg.device, h->fp32 ? 1 : 0, h->cfg.nx, h->cfg.ny, h->cfg.nz};
   std::array<PassSched, NSLOTS> best;

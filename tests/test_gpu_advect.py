@@ -104,3 +104,27 @@ def test_advect_rejects_dtpsi_until_built():
         s.set_psi(structured_ic((16, 16, 16)))
         with pytest.raises(SmxError):
             s.dtpsi()
+
+
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+def test_advect_tuned_y_passes(prec):
+    """The advected chain runs its eleven y passes with the handle's tuned schedules."""
+    shape = (64, 64, 32)
+    say("advect autotuned", prec)
+    cfg = cfg_reference(*shape, dt=2e-3, advect=1, precision=prec)
+    psi = structured_ic(shape)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(5)
+    with Solver(cfg) as s:
+        s.set_schedule([(8, 1), (16, 0), (32, 1), (16, 0)])
+        s.set_psi(psi)
+        s.step(5)
+        e = rel_l2(s.get_psi(), orc.get_psi())
+        chosen = s.autotune(3)
+        say("  fixed schedule err", e, "autotuned", chosen)
+        s.set_psi(psi)
+        s.step(5)
+        e2 = rel_l2(s.get_psi(), orc.get_psi())
+    tol = 1e-10 if prec == A.SMX_FP64 else 2e-5
+    assert e < tol and e2 < tol
