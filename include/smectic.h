@@ -215,6 +215,20 @@ int smx_batch_autotune(smx_batch *b, int reps);
 /* Block until the handle's stream is idle. */
 int smx_sync(smx_handle *h);
 
+/*
+ * Exact checkpoint of the solver state -- the carried CNAB2 arrays (stage buffer and
+ * R = C1 psi_hat - dt/2 N_hat_past; with advection also the two derivative stages), the FP64
+ * mean mode and the step count -- so that a run resumed from it continues bit for bit.
+ * Replaces the reference's restart from a 6-digit text dump of psi, which re-bootstraps
+ * N_hat_past from N(psi) without advection (pf3dMorph.cpp:416-472) and is therefore not
+ * continuation-exact.  Binary file: header (magic "SMXCKPT1", ABI version, smx_config, step
+ * count, array sizes), the arrays, the mean mode and an FNV-1a 64-bit checksum.  Loading
+ * fails with SMX_ERR_INVALID for a file of another grid, precision or operator (the configs
+ * must agree in every field but device) or with a bad checksum.
+ */
+int smx_save_checkpoint(smx_handle *h, const char *path);
+int smx_load_checkpoint(smx_handle *h, const char *path);
+
 /* Number of steps taken since the last smx_set_psi. */
 int smx_step_count(const smx_handle *h, int64_t *n);
 

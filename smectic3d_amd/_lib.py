@@ -18,6 +18,7 @@ EXPORTS = [
     "smx_calibration_copy", "smx_adjoint_grad", "smx_batch_create", "smx_batch_destroy",
     "smx_batch_handle", "smx_batch_run", "smx_batch_graph_nodes", "smx_get_schedule", "smx_set_schedule",
     "smx_autotune", "smx_batch_autotune", "smx_set_adjoint_budget", "smx_adjoint_info",
+    "smx_save_checkpoint", "smx_load_checkpoint",
 ]
 
 _lib = None
@@ -70,6 +71,8 @@ def lib():
     L.smx_autotune.argtypes = [vp, C.c_int, C.POINTER(SmxSchedule)]
     L.smx_batch_autotune.argtypes = [vp, C.c_int]
     L.smx_set_adjoint_budget.argtypes = [vp, C.c_int64]
+    L.smx_save_checkpoint.argtypes = [vp, C.c_char_p]
+    L.smx_load_checkpoint.argtypes = [vp, C.c_char_p]
     L.smx_adjoint_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     for n in EXPORTS:
         f = getattr(L, n)

@@ -963,6 +963,44 @@ static int state_from_real_io(smx_handle* h) {
   return SMX_OK;
 }
 
+// ---- exact checkpoints ----
+// File: CkptHeader, then the carried arrays (A, R; with advection also Ax, As), each
+// wbytes * nspec bytes, the FP64 mean state (2 doubles), and an FNV-1a 64-bit checksum of
+// everything before it.
+struct CkptHeader {
+  char magic[8];  // "SMXCKPT1"
+  int32_t abi_version;
+  int32_t narrays;
+  int64_t wbytes, nspec, steps;
+  smx_config cfg;
+};
+
+static uint64_t fnv1a(uint64_t hsh, const void* data, size_t n) {
+  const unsigned char* b = (const unsigned char*)data;
+  for (size_t i = 0; i < n; ++i) {
+    hsh ^= b[i];
+    hsh *= 1099511628211ull;
+  }
+  return hsh;
+}
+
+static bool same_operator(const smx_config& a, const smx_config& b) {
+  return a.abi_version == b.abi_version && a.bc == b.bc && a.precision == b.precision && a.nx == b.nx &&
+         a.ny == b.ny && a.nz == b.nz && a.lx == b.lx && a.ly == b.ly && a.lz == b.lz && a.eps == b.eps &&
+         a.q0 == b.q0 && a.alpha == b.alpha && a.beta == b.beta && a.gamma == b.gamma && a.dt == b.dt &&
+         a.nl_model == b.nl_model && a.advect == b.advect && a.eta == b.eta && a.rho == b.rho &&
+         a.force_form == b.force_form && a.w_mid == b.w_mid && a.mid_plane == b.mid_plane;
+}
+
+static std::vector<void*> carried_arrays(smx_handle* h) {
+  std::vector<void*> v = {h->A, h->R};
+  if (h->cfg.advect) {
+    v.push_back(h->Ax);
+    v.push_back(h->As);
+  }
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 extern "C" {
@@ -1111,6 +1149,85 @@ int smx_step(smx_handle* h, int64_t nsteps) {
 int smx_sync(smx_handle* h) {
   if (!h) return fail(SMX_ERR_INVALID, "null handle");
   HIPCHK(hipStreamSynchronize(h->st));
+  return SMX_OK;
+}
+
+int smx_save_checkpoint(smx_handle* h, const char* path) {
+  if (!h || !path) return fail(SMX_ERR_INVALID, "null argument");
+  if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  HIPCHK(hipSetDevice(h->cfg.device));
+  const std::vector<void*> arrs = carried_arrays(h);
+  const size_t abytes = h->wbytes * (size_t)h->nspec;
+  CkptHeader hd;
+  std::memset(&hd, 0, sizeof(hd));
+  std::memcpy(hd.magic, "SMXCKPT1", 8);
+  hd.abi_version = SMX_ABI_VERSION;
+  hd.narrays = (int32_t)arrs.size();
+  hd.wbytes = (int64_t)h->wbytes;
+  hd.nspec = h->nspec;
+  hd.steps = h->steps;
+  hd.cfg = h->cfg;
+  std::vector<char> buf(abytes);
+  double mean[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(mean, h->mean_state, sizeof(mean), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(SMX_ERR_INVALID, std::string("cannot open ") + path + " for writing");
+  uint64_t hsh = 1469598103934665603ull;
+  bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1;
+  hsh = fnv1a(hsh, &hd, sizeof(hd));
+  for (size_t i = 0; i < arrs.size() && ok; ++i) {
+    if (hipMemcpy(buf.data(), arrs[i], abytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      std::fclose(f);
+      return fail(SMX_ERR_HIP, "checkpoint read-back failed");
+    }
+    ok = std::fwrite(buf.data(), 1, abytes, f) == abytes;
+    hsh = fnv1a(hsh, buf.data(), abytes);
+  }
+  ok = ok && std::fwrite(mean, sizeof(mean), 1, f) == 1;
+  hsh = fnv1a(hsh, mean, sizeof(mean));
+  ok = ok && std::fwrite(&hsh, sizeof(hsh), 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(SMX_ERR_INVALID, std::string("write to ") + path + " failed");
+  return SMX_OK;
+}
+
+int smx_load_checkpoint(smx_handle* h, const char* path) {
+  if (!h || !path) return fail(SMX_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(h->cfg.device));
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(SMX_ERR_INVALID, std::string("cannot open ") + path);
+  CkptHeader hd;
+  const std::vector<void*> arrs = carried_arrays(h);
+  const size_t abytes = h->wbytes * (size_t)h->nspec;
+  auto bad = [&](const std::string& why) {
+    std::fclose(f);
+    return fail(SMX_ERR_INVALID, std::string(path) + ": " + why);
+  };
+  if (std::fread(&hd, sizeof(hd), 1, f) != 1 || std::memcmp(hd.magic, "SMXCKPT1", 8) != 0)
+    return bad("not a checkpoint");
+  if (hd.abi_version != SMX_ABI_VERSION || hd.narrays != (int32_t)arrs.size() || hd.wbytes != (int64_t)h->wbytes ||
+      hd.nspec != h->nspec || !same_operator(hd.cfg, h->cfg))
+    return bad("checkpoint of a different grid, precision or operator");
+  uint64_t hsh = fnv1a(1469598103934665603ull, &hd, sizeof(hd));
+  std::vector<std::vector<char>> data(arrs.size(), std::vector<char>(abytes));
+  for (auto& d : data) {
+    if (std::fread(d.data(), 1, abytes, f) != abytes) return bad("truncated");
+    hsh = fnv1a(hsh, d.data(), abytes);
+  }
+  double mean[2];
+  uint64_t stored = 0;
+  if (std::fread(mean, sizeof(mean), 1, f) != 1 || std::fread(&stored, sizeof(stored), 1, f) != 1)
+    return bad("truncated");
+  hsh = fnv1a(hsh, mean, sizeof(mean));
+  std::fclose(f);
+  if (hsh != stored) return fail(SMX_ERR_INVALID, std::string(path) + ": checksum mismatch");
+  for (size_t i = 0; i < arrs.size(); ++i)
+    HIPCHK(hipMemcpy(arrs[i], data[i].data(), abytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->mean_state, mean, sizeof(mean), hipMemcpyHostToDevice));
+  h->steps = hd.steps;
+  h->has_state = true;
+  h->has_prev = false;
   return SMX_OK;
 }
 

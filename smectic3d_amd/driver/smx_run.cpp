@@ -8,6 +8,9 @@
 //           [--fp32] [--advect 0|1] [--force sh|mu] [--wmid W] [--ic noise|hyperboloid|pf3d|morph|dct|file:PATH]
 //           [--amp A] [--seed S] [--steps N] [--l1-limit L] [--l1-every K] [--save-every K]
 //           [--surf-every K] [--arm A] [--vsign S] [--out DIR] [--max-steps N] [--ic-only]
+//           [--resume CKPT] [--checkpoint CKPT]
+// --resume continues bit for bit from an exact binary checkpoint written by --checkpoint
+// (at the end of the run) instead of re-bootstrapping from a text dump.
 //
 // Files (one value per line, `os << v << "\n "`, default 6 significant digits, loop
 // order i, j, k as pf3d.cpp:346-352): psiIC.dat, L1.dat (appended every l1-every
@@ -40,6 +43,7 @@ struct Opts {
   long surf_every = 0;     // in L1 checks (0: off)
   long max_steps = 1000000;
   bool ic_only = false;  // write psiIC.dat and stop (no device needed)
+  std::string resume, checkpoint;  // exact binary checkpoint to continue from / to write at the end
   double arm = -1, vsign = 1.0;
   int nw = 16;
 };
@@ -204,6 +208,8 @@ int run(int argc, char** argv) {
     else if (k == "--vsign" && has) o.vsign = num(a);
     else if (k == "--out" && has) o.out = argv[++a];
     else if (k == "--ic-only") o.ic_only = true;
+    else if (k == "--resume" && has) o.resume = argv[++a];
+    else if (k == "--checkpoint" && has) o.checkpoint = argv[++a];
     else throw std::runtime_error("unknown or incomplete option " + k);
   }
   if (nx > 0 || ny > 0 || nz > 0) {
@@ -225,7 +231,11 @@ int run(int argc, char** argv) {
   smx_handle* h = nullptr;
   if (smx_create(&o.c, &h) != SMX_OK) throw std::runtime_error(std::string("smx_create: ") + smx_last_error());
   std::ofstream(dir + "L1.dat").close();
-  if (smx_set_psi(h, psi.data(), o.c.nz) != SMX_OK) throw std::runtime_error(smx_last_error());
+  if (!o.resume.empty()) {
+    if (smx_load_checkpoint(h, o.resume.c_str()) != SMX_OK) throw std::runtime_error(smx_last_error());
+  } else if (smx_set_psi(h, psi.data(), o.c.nz) != SMX_OK) {
+    throw std::runtime_error(smx_last_error());
+  }
 
   const int64_t ncol = (int64_t)o.c.nx * o.c.ny;
   std::vector<double> sz(ncol), sv(ncol), sh(ncol), sk(ncol);
@@ -266,6 +276,8 @@ int run(int argc, char** argv) {
   }
   smx_sync(h);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (!o.checkpoint.empty() && smx_save_checkpoint(h, o.checkpoint.c_str()) != SMX_OK)
+    throw std::runtime_error(smx_last_error());
   smx_get_psi(h, psi.data(), o.c.nz);
   write_field(dir + "psi.dat", psi);
   double F = 0;

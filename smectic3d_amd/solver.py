@@ -130,6 +130,13 @@ class Solver:
         check(lib().smx_adjoint_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def save_checkpoint(self, path):
+        """Exact binary checkpoint (carried state, FP64 mean mode, step count)."""
+        check(lib().smx_save_checkpoint(self._h, str(path).encode()))
+
+    def load_checkpoint(self, path):
+        check(lib().smx_load_checkpoint(self._h, str(path).encode()))
+
     def calibration_copy(self, nbytes, reps=20):
         v = C.c_double()
         check(lib().smx_calibration_copy(self._h, int(nbytes), int(reps), C.byref(v)))

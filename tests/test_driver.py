@@ -49,3 +49,18 @@ def test_file_ic_rejects_short_file(driver, tmp_path):
     r = subprocess.run([driver, "--ic-only", "--out", str(tmp_path), "--n", "16", "--ic", f"file:{p}"],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "short" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_resume_from_checkpoint_is_exact(driver, tmp_path):
+    """100 steps, --checkpoint, then --resume for 100 more: the final psi.dat equals that of
+    one 200-step run (the reference's text restart re-bootstraps and is not exact)."""
+    common = ["--preset", "plain", "--n", "32", "--l1-every", "50", "--save-every", "1000"]
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    ck = tmp_path / "state.ckpt"
+    subprocess.run([driver, *common, "--steps", "200", "--out", str(a)], check=True)
+    subprocess.run([driver, *common, "--steps", "100", "--out", str(b), "--checkpoint", str(ck)], check=True)
+    subprocess.run([driver, *common, "--steps", "100", "--out", str(b), "--resume", str(ck)], check=True)
+    assert (a / "psi.dat").read_text() == (b / "psi.dat").read_text()

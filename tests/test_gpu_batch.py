@@ -61,3 +61,18 @@ def test_batch_matches_oracle_one_box():
     o.step(40)
     ref = o.get_psi()
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_growth_sweep_batch_matches_oracle():
+    """pf3dVal harness as one GPU batch (one box per wavenumber) against the oracle's sweep
+    and the growth law sigma/(1 - dt sigma/2) at lattice wavenumbers."""
+    import math
+    from smectic3d_amd.sweeps import growth_sweep
+    cfg = A.make_config(16, 16, 32, nw=16, eps=-0.7, beta=2.0, gamma=1.0, dt=1e-3)
+    dmq = 2 * math.pi / cfg.lz
+    g = growth_sweep(cfg, dmq, dmq, 10)
+    g0 = O.growth_sweep(cfg, dmq, dmq, 10)
+    np.testing.assert_allclose(g, g0, rtol=1e-9)
+    mq = dmq * np.arange(1, 11)
+    sigma = -0.7 - (mq ** 2 - 1.0) ** 2
+    np.testing.assert_allclose(g, sigma / (1 - 1e-3 * sigma / 2), rtol=1e-6)
