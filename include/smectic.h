@@ -188,7 +188,9 @@ int smx_batch_graph_nodes(smx_batch *b, int64_t *nodes);
  * Tile schedules of the four kernels of the periodic step, in launch order (slot 0: inverse
  * y pass, 1: z pass with N(psi), 2: forward y pass, 3: x pass with the CNAB2 update): the
  * columns per tile (8, 16 or 32) and the radix-plan variant (0: default; 1: the alternative
- * plan of the power-of-two lengths).  The step's result depends on the schedule only
+ * plan of the power-of-two lengths).  Slot 3 also takes variants 2 and 3: plans 0 and 1
+ * with the x pass's R stores issued after its inverse transform instead of between the
+ * transforms (bit-identical results).  The step's result depends on the schedule only
  * through rounding.  The default is nc = 16, variant = 0 in every slot.  The advected step
  * uses the schedules of slots 0 and 2 for its eleven y passes.
  */
@@ -201,9 +203,10 @@ int smx_get_schedule(const smx_handle *h, smx_schedule *s);
  * many threads or too much LDS per tile, or ny / nx*ny not a multiple of nc), or if a
  * non-default schedule is given to a Neumann handle. */
 int smx_set_schedule(smx_handle *h, const smx_schedule *s);
-/* Autotuner: time every valid candidate of every slot (reps launches each, HIP events on
- * the handle's stream) on scratch buffers -- the state is not touched -- install the
- * fastest and return it in *chosen (may be NULL).  Results are cached per (device,
+/* Autotuner: slot by slot, time every valid candidate (reps whole step chains each, HIP
+ * events on the handle's stream) on a scratch copy of the state -- hashed noise when none
+ * is set; the state itself is not touched -- install the fastest and return it in *chosen
+ * (may be NULL).  Results are cached per (device,
  * precision, grid) for the life of the process.  Replaces the FFTW_MEASURE planning of
  * pf3d.cpp:226-291 (which times candidate FFT algorithms at set-up).  Periodic handles
  * only; not callable during stream capture. */
@@ -243,8 +246,9 @@ int smx_step_count(const smx_handle *h, int64_t *n);
 int smx_profile_step(smx_handle *h, int64_t nsteps, int max_slots, int *nslots, double *avg_ms,
                      double *alg_bytes, char *names, int names_cap);
 
-/* Plain device copy kernel over nbytes (grid-stride float4 copy), used to
- * calibrate hardware byte counters against a kernel of known traffic. */
+/* Plain device copy kernel over nbytes (grid-stride float4 copy of hashed,
+ * non-zero words), used to calibrate hardware byte counters against a kernel of
+ * known traffic and as the achievable-bandwidth yardstick. */
 int smx_calibration_copy(smx_handle *h, int64_t nbytes, int reps, double *avg_ms);
 
 #ifdef __cplusplus

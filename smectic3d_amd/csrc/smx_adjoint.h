@@ -31,10 +31,11 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
                                                                    double alphad, double* __restrict__ acc) {
   constexpr int E = PlanV<L, V>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
   constexpr uint32_t CB = sizeof(cplx<T>);
-  (void)NT;
   __shared__ cplx<T> S[L * NCP];
   __shared__ double red[64];
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
 #pragma unroll
   for (int e = 0; e < E; ++e)
     v[e] = ld_off<cplx<T>>(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_first<L, V>()>(tc, e) * rowb + cofs);
-  FwdFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
+  FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
   const T kyv = p.ky[ky], kzv = p.kz[kz];
   const double wk = herm ? 1.0 : 2.0;
   const bool sums = psin != nullptr && acc != nullptr;
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
       acc[2 * b + 1] += wk * 4.0 * alphad * q0d * bq;
     }
   }
-  InvFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
+  InvFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
 #pragma unroll
   for (int e = 0; e < E; ++e)
     st_off(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);

@@ -47,6 +47,8 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
   __shared__ cplx<T> S[L * NCP];
   __shared__ double red[64];
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
     for (int i = tid; i < p.npartial; i += NT) a += p.partial[i];
     nsum = block_sum(a, red);
   }
-  FwdFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
+  FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
   const T kyv = p.ky[ky], kzv = p.kz[kz];
   if (herm) __syncthreads();  // S receives R' below
 #pragma unroll
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
       else if (o == 1) v[e] = mk(-kxv * ps[e].y, kxv * ps[e].x);  // i kx psi_hat (pf3d.cpp:404)
       else v[e] = op_sh(p, kxv * kxv + kyv * kyv + kzv * kzv) * ps[e];  // opSH psi_hat (pf3d.cpp:407)
     }
-    InvFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
+    InvFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
 #pragma unroll
     for (int e = 0; e < E; ++e)
       st_off(outs[o], (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
@@ -155,6 +157,8 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_stokes(cplx<T
   __shared__ cplx<T> Sx[L * NCP];
   const PParams<T>& p = ap.p;
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NC * (L / E)>(TW, p.twx, tid);
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB, cofs = (uint32_t)(y0 + col) * CB;
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_stokes(cplx<T
 #pragma unroll
     for (int e = 0; e < E; ++e)
       v[a][e] = ld_off<cplx<T>>(arr[a], (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_first<L, V>()>(tc, e) * rowb + cofs);
-    FwdFFT<T, L, NCP, V>::run(v[a], Sx, col, tc, p.twx);
+    FwdFFT<T, L, NCP, V>::run(v[a], Sx, col, tc, twl);
   }
   const T kyv = p.ky[y0 + col], kzv = p.kz[kz];
 #pragma unroll
@@ -190,7 +194,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_stokes(cplx<T
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    InvFFT<T, L, NCP, V>::run(v[a], Sx, col, tc, p.twx);
+    InvFFT<T, L, NCP, V>::run(v[a], Sx, col, tc, twl);
 #pragma unroll
     for (int e = 0; e < E; ++e)
       st_off(arr[a], (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[a][e]);
@@ -264,12 +268,14 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_force
   __shared__ cplx<T> F[L * Z::FP];
   const PParams<T>& p = ap.p;
   const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, Z::NT>(TW, p.twz, tid);
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * Z::CB;
   cplx<T> v[E], f[E];
-  Z::c2r(bs, f0, rowb, q, tc, nullptr, F, p.twz, f);  // psiSH
+  Z::c2r(bs, f0, rowb, q, tc, nullptr, F, twl, f);  // psiSH
   if (ap.force == F_MU) {
-    Z::c2r(bpsi, f0, rowb, q, tc, nullptr, F, p.twz, v);  // psi
+    Z::c2r(bpsi, f0, rowb, q, tc, nullptr, F, twl, v);  // psi
 #pragma unroll
     for (int e = 0; e < E; ++e)
       f[e] = mk(f[e].x - p.eps * v[e].x - nonlin(p, v[e].x), f[e].y - p.eps * v[e].y - nonlin(p, v[e].y));
@@ -278,10 +284,10 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_force
   cplx<T>* dst[3] = {o0, o1, o2};
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    Z::c2r(src[a], f0, rowb, q, tc, a == 2 ? p.kz : nullptr, F, p.twz, v);
+    Z::c2r(src[a], f0, rowb, q, tc, a == 2 ? p.kz : nullptr, F, twl, v);
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = mk(f[e].x * v[e].x, f[e].y * v[e].y);
-    Z::r2c_store(v, dst[a], f0, rowb, q, tc, tid, F, p.twz);
+    Z::r2c_store(v, dst[a], f0, rowb, q, tc, tid, F, twl);
   }
 }
 
@@ -300,11 +306,13 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_advnl
   __shared__ double red[64];
   const PParams<T>& p = ap.p;
   const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, Z::NT>(TW, p.twz, tid);
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * Z::CB;
   constexpr int RP = fwd_first<L, V>();
   cplx<T> v[E], nacc[E], u[E];
-  Z::c2r(bpsi, f0, rowb, q, tc, nullptr, F, p.twz, v);
+  Z::c2r(bpsi, f0, rowb, q, tc, nullptr, F, twl, v);
   const bool pen = p.mid > 0 && p.mid < L - 1;
   if (pen) {
     __syncthreads();
@@ -326,8 +334,8 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_advnl
   const cplx<T>* gs[3] = {bx, by, bpsi};
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    Z::c2r(us[a], f0, rowb, q, tc, nullptr, F, p.twz, u);
-    Z::c2r(gs[a], f0, rowb, q, tc, a == 2 ? p.kz : nullptr, F, p.twz, v);
+    Z::c2r(us[a], f0, rowb, q, tc, nullptr, F, twl, u);
+    Z::c2r(gs[a], f0, rowb, q, tc, a == 2 ? p.kz : nullptr, F, twl, v);
 #pragma unroll
     for (int e = 0; e < E; ++e) nacc[e] = mk(nacc[e].x - u[e].x * v[e].x, nacc[e].y - u[e].y * v[e].y);
   }
@@ -338,7 +346,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_advnl
     const double s = block_sum(acc, red);
     if (tid == 0) p.partial[blockIdx.x] = s;
   }
-  Z::r2c_store(nacc, nout, f0, rowb, q, tc, tid, F, p.twz);
+  Z::r2c_store(nacc, nout, f0, rowb, q, tc, tid, F, twl);
 }
 
 }  // namespace smx

@@ -874,48 +874,92 @@ static int sched_check(smx_handle* h, const PassSched* s) {
   return SMX_OK;
 }
 
+// Autotuning runs whole periodic step chains on a scratch copy of the state (D1 <- A,
+// D2 <- R; hashed noise when no state is set), so that every candidate moves field-like
+// data in its real place in the chain: HBM throughput depends on the words moved (a
+// zero-filled buffer would flatter every kernel), and a kernel's speed on what the
+// previous one left in the caches.  A candidate's score is the median, over reps chains,
+// of its own kernel's duration (start/end events stamped on that launch alone).
+// Coordinate descent: the slots are tuned in launch order, each over all its candidates
+// with the others at their picks so far.  The handle's own state and mean mode are not
+// touched.
 template <typename T>
 static int autotune_T(smx_handle* h, int reps, PassSched* best) {
   PParams<T> p = params<T>(h, h->fp32);
   if (p.partial) p.partial = h->partial + partial_len(h);  // the tuner's own partial-sum region
   p.mean_state = h->mean_state + 4;                        // scratch entries 4..7
-  p.npartial = (int)(h->nxy / kNC);
-  const size_t dbytes = (h->neumann ? 8 : 16) * (size_t)h->nspec;
-  HIPCHK(hipMemsetAsync(h->D1, 0, dbytes, h->st));
-  HIPCHK(hipMemsetAsync(h->D2, 0, dbytes, h->st));
-  HIPCHK(hipMemsetAsync(h->partial + partial_len(h), 0, sizeof(double) * partial_len(h), h->st));
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIPCHK(hipEventCreate(&e0));
-  hipError_t e = hipEventCreate(&e1);
+  const int64_t ncomp = 2 * h->nspec;                      // real words of one spectrum
+  T* wa = (T*)h->D1;
+  T* wr = (T*)h->D2;
+  auto reset = [&]() -> int {
+    if (h->has_state) {
+      HIPCHK(hipMemcpyAsync(wa, h->A, sizeof(T) * ncomp, hipMemcpyDeviceToDevice, h->st));
+      HIPCHK(hipMemcpyAsync(wr, h->R, sizeof(T) * ncomp, hipMemcpyDeviceToDevice, h->st));
+      HIPCHK(hipMemcpyAsync(h->mean_state + 4, h->mean_state, 2 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    } else {
+      // stage entries of size a give psi ~ a sqrt(ny nz); the carry is x-normalised
+      const double a = 0.1 / std::sqrt((double)h->cfg.ny * h->cfg.nz);
+      hipLaunchKernelGGL(k_fill_hash<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, wa, ncomp, (T)a, 1u);
+      hipLaunchKernelGGL(k_fill_hash<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, wr, ncomp,
+                         (T)(a / std::sqrt((double)h->cfg.nx)), 2u);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemsetAsync(h->mean_state + 4, 0, 2 * sizeof(double), h->st));
+    }
+    return SMX_OK;
+  };
+  // one chain; slot ks's launch is stamped with (ev0, ev1) when ev0 is set
+  auto chain = [&](const PassSched* s, int ks, hipEvent_t ev0, hipEvent_t ev1) -> hipError_t {
+    PParams<T> q = p;
+    q.npartial = PeriodicOps<T>::zpass_blocks(p, s[SLOT_Z].nc);
+    for (int k = 0; k < NSLOTS; ++k) {
+      if (k == ks && ev0) launch_events() = LaunchEvents{ev0, ev1};
+      const hipError_t e = launch_slot<T>(h, q, k, s[k], false, wa, wr);
+      launch_events() = LaunchEvents{};
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  PassSched cur[NSLOTS];
+  for (int k = 0; k < NSLOTS; ++k) cur[k] = PassSched{};
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < ev.size() && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
   int rc = (e == hipSuccess) ? SMX_OK : fail(SMX_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+  std::vector<float> dur((size_t)reps);
   for (int k = 0; k < NSLOTS && rc == SMX_OK; ++k) {
     double tbest = 1e300;
-    PassSched pick{};
-    for (int v : kSchedV)
+    PassSched pick = cur[k];
+    const int* vs = (k == SLOT_X) ? kSchedVX : kSchedV;
+    const int nv = (k == SLOT_X) ? 4 : 2;
+    for (int iv = 0; iv < nv && rc == SMX_OK; ++iv)
       for (int nc : kSchedNC) {
-        const PassSched c{nc, v};
-        if (rc != SMX_OK || launch_slot<T>(h, p, k, c, true, h->D1, h->D2) != hipSuccess) continue;
-        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_slot<T>(h, p, k, c, false, h->D1, h->D2);
-        if (e == hipSuccess) e = hipEventRecord(e0, h->st);
-        for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_slot<T>(h, p, k, c, false, h->D1, h->D2);
-        if (e == hipSuccess) e = hipEventRecord(e1, h->st);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        const PassSched c{nc, vs[iv]};
+        if (launch_slot<T>(h, p, k, c, true, wa, wr) != hipSuccess) continue;
+        PassSched s[NSLOTS];
+        for (int j = 0; j < NSLOTS; ++j) s[j] = cur[j];
+        s[k] = c;
+        rc = reset();
+        if (rc != SMX_OK) break;
+        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = chain(s, -1, nullptr, nullptr);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = chain(s, k, ev[2 * r], ev[2 * r + 1]);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[2 * (size_t)reps - 1]);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipEventElapsedTime(&dur[r], ev[2 * r], ev[2 * r + 1]);
         if (e != hipSuccess) {
           rc = fail(SMX_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
           break;
         }
-        const double us = 1000.0 * ms / reps;
+        std::nth_element(dur.begin(), dur.begin() + reps / 2, dur.end());
+        const double us = 1000.0 * dur[reps / 2];
         if (us < tbest) {
           tbest = us;
           pick = c;
         }
       }
-    best[k] = pick;
+    cur[k] = pick;
   }
-  (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
+  for (int k = 0; k < NSLOTS; ++k) best[k] = cur[k];
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
   return rc;
 }
 
@@ -1566,7 +1610,7 @@ int smx_calibration_copy(smx_handle* h, int64_t nbytes, int reps, double* avg_ms
     (void)hipFree(a);
     return fail(SMX_ERR_HIP, "hipMalloc");
   }
-  (void)hipMemsetAsync(a, 0, n4 * 16, h->st);
+  hipLaunchKernelGGL(k_fill_hash_f4, dim3(2048), dim3(256), 0, h->st, (float4*)a, n4);
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);

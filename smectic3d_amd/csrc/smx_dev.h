@@ -281,6 +281,19 @@ template <int L, int V = 0> constexpr int fwd_last() { return RadInfo<typename P
 template <int L, int V = 0> constexpr int inv_first() { return RadInfo<typename PlanV<L, V>::Inv>::first; }
 template <int L, int V = 0> constexpr int inv_last() { return RadInfo<typename PlanV<L, V>::Inv>::last; }
 
+// LDS copy of a length-L twiddle table, filled by the NT threads of the block.  Twiddle
+// reads are per-lane gathers repeated in every pass; from LDS they take the short LDS
+// path instead of the vector-memory path, where each would also wait behind the tile's
+// HBM loads (one in-order counter per wave).  The first pass of every plan reads no
+// twiddles and the first exchange starts with a barrier, so the copy is complete before
+// its first read without a barrier of its own.
+template <int L, int NT, typename T>
+__device__ __forceinline__ const cplx<T>* tw_lds(cplx<T>* lds, const cplx<T>* __restrict__ tw, int tid) {
+#pragma unroll
+  for (int i = tid; i < L; i += NT) lds[i] = tw[i];
+  return lds;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 block reduction helpers (wavefront shuffles, then LDS across waves).
 __device__ __forceinline__ double wave_sum(double v) {

@@ -199,6 +199,42 @@ __global__ void k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, 
     b[i] = a[i];
 }
 
+// Fills a with floats in [-1, 1) from a 32-bit integer hash of the word index.  HBM
+// throughput depends on the data moved (an all-zero buffer copies about 13% faster
+// than one of random words on this hardware), so the copy yardstick moves field-like
+// words, not zeros.
+__global__ void k_fill_hash_f4(float4* __restrict__ a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint32_t h = (uint32_t)(4 * i + c) * 0x9E3779B9u;
+      h ^= h >> 16;
+      h *= 0x85EBCA6Bu;
+      h ^= h >> 13;
+      h *= 0xC2B2AE35u;
+      h ^= h >> 16;
+      w[c] = (float)(int32_t)h * 4.656612873077393e-10f;
+    }
+    a[i] = make_float4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// a[i] = amp * u_i with u_i in [-1, 1) from a 32-bit integer hash of (seed, i): scratch
+// fields of realistic bit patterns (tuning runs without a state).
+template <typename T>
+__global__ void k_fill_hash(T* __restrict__ a, int64_t n, T amp, uint32_t seed) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t h = ((uint32_t)i ^ (uint32_t)(i >> 32) * 0x27D4EB2Fu ^ seed) * 0x9E3779B9u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    a[i] = amp * (T)((double)(int32_t)h * 4.656612873077393e-10);
+  }
+}
+
 // a_T = psi_T - amp cos(q0 z) in place (FP64 real, dense [x][y][z]); per-block partials of
 // J = 1/2 sum a_T^2 and of the explicit target term sum a_T (-dg/dq0) = sum a_T amp z sin(q0 z).
 __global__ void k_adj_target(double* __restrict__ a, int64_t n, int nz, double dz, double q0, double amp,

@@ -41,7 +41,8 @@ inline bool length_supported(int n) {
   return false;
 }
 
-// Tile schedule of one pass kernel: columns per tile (NC) and plan variant (V, PlanV).
+// Tile schedule of one pass kernel: columns per tile (NC) and plan variant (V, PlanV;
+// in the x slot bit 1 of v also selects the carry-store order of k_xpass).
 // The defaults are the fixed schedule; an autotuner may pick others per pass.
 struct PassSched {
   int nc = kNC, v = 0;
@@ -50,7 +51,9 @@ struct PassSched {
 enum SchedSlot { SLOT_YINV = 0, SLOT_Z = 1, SLOT_YFWD = 2, SLOT_X = 3, NSLOTS = 4 };
 constexpr int kSchedNC[3] = {8, 16, 32};
 constexpr int kSchedV[2] = {0, 1};
+constexpr int kSchedVX[4] = {0, 1, 2, 3};  // the x pass adds the carry-store order (bit 1)
 constexpr size_t kMaxLds = 160 * 1024 - 1024;  // LDS of one workgroup on gfx950, less the reduction scratch
+// (each tile check adds the L-entry twiddle copy every pass kernel keeps in LDS)
 
 template <int N> using IC = std::integral_constant<int, N>;
 
@@ -103,7 +106,7 @@ hipError_t PeriodicOps<T>::ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_
     return sched_dispatch<L>(s, [&](auto nc, auto v) -> hipError_t {                                     \
       constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
       constexpr int NT = NC * (L / PlanV<L, V>::E);                                                      \
-      if constexpr (NT > 1024 || NC * ColAddr<L>::LP * sizeof(cplx<T>) > kMaxLds) {                      \
+      if constexpr (NT > 1024 || (NC * ColAddr<L>::LP + L) * sizeof(cplx<T>) > kMaxLds) {                      \
         return hipErrorInvalidValue;                                                                     \
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \
@@ -141,17 +144,22 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
     return hipErrorInvalidValue;
   }
   if (p.ny % s.nc != 0) return hipErrorInvalidValue;  // whole tiles along y
+  // variant bit 0: radix plan; bit 1: carry-store order (k_xpass ORD)
+  if (s.v < 0 || s.v > 3) return hipErrorInvalidValue;
+  const int ord = s.v >> 1;
+  const PassSched sp{s.nc, s.v & 1};
 #define SMX_CASE(L)                                                                                      \
   if (p.nx == L)                                                                                         \
-    return sched_dispatch<L>(s, [&](auto nc, auto v) -> hipError_t {                                     \
+    return sched_dispatch<L>(sp, [&](auto nc, auto v) -> hipError_t {                                    \
       constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
       constexpr int NT = NC * (L / PlanV<L, V>::E);                                                      \
-      if constexpr (NT > 1024 || (size_t)L * (NC + 1) * sizeof(cplx<T>) > kMaxLds) {                     \
+      if constexpr (NT > 1024 || (size_t)L * (NC + 2) * sizeof(cplx<T>) > kMaxLds) {                     \
         return hipErrorInvalidValue;                                                                     \
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \
-        SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V>), dim3((unsigned)(p.ny / NC), (unsigned)p.nzh), dim3(NT), 0, \
-                           st, in, out, rbuf, psi0, p);                                                  \
+        const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh);                                         \
+        if (ord == 0) SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V, 0>), grid, dim3(NT), 0, st, in, out, rbuf, psi0, p); \
+        else SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V, 1>), grid, dim3(NT), 0, st, in, out, rbuf, psi0, p);          \
         return hipGetLastError();                                                                        \
       }                                                                                                  \
     });
@@ -189,7 +197,7 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
     return sched_dispatch<L>(s, [&](auto nc, auto v) -> hipError_t {                                     \
       constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
       constexpr int NT = (NC / 2) * (L / PlanV<L, V>::E);                                                \
-      if constexpr (NT > 1024 || (size_t)L * (NC / 2 + 1) * sizeof(cplx<T>) > kMaxLds) {                 \
+      if constexpr (NT > 1024 || (size_t)L * (NC / 2 + 2) * sizeof(cplx<T>) > kMaxLds) {                 \
         return hipErrorInvalidValue;                                                                     \
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \

@@ -54,6 +54,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
   using TL = TileA<T, L, E, AD>;
   __shared__ cplx<T> S[NPR * AD::LP];
   const int tid = threadIdx.x, tc = tid % TPC, q = tid / TPC;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NT>(TW, tw, tid);
   const int64_t pa = (int64_t)blockIdx.x * NC + 2 * q;  // pencils pa (column a), pa + 1 (column b)
   const T* a = in + pa * L;
   const T* b = a + L;
@@ -64,7 +66,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
       const int s = mk_src(TL::template pos_in<fwd_first<L>()>(tc, e), L);
       v[e] = mk(a[s], b[s]);
     }
-    FwdFFTA<T, L, AD>::run(v, S, q, tc, tw);
+    FwdFFTA<T, L, AD>::run(v, S, q, tc, twl);
     __syncthreads();
     TL::template store<fwd_last<L>()>(v, S, q, tc);
     __syncthreads();
@@ -82,7 +84,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
       const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : mk(a[L - k], b[L - k]);
       v[e] = dct3_in(mk(a[k], b[k]), pn, qtw[k]);
     }
-    InvFFTA<T, L, AD>::run(v, S, q, tc, tw);
+    InvFFTA<T, L, AD>::run(v, S, q, tc, twl);
     T* oa = out + pa * L;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -104,6 +106,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy + y0;
   const uint32_t rowb = (uint32_t)p.ny * sizeof(T);
@@ -130,7 +134,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
       const int s = mk_src(Tile<T, L, E, FP>::template pos_in<fwd_first<L>()>(tc, e), L);
       v[e] = ld_off(src, (uint32_t)s * rowb + q * CB);
     }
-    FwdFFT<T, L, FP>::run(v, F, q, tc, p.twx);
+    FwdFFT<T, L, FP>::run(v, F, q, tc, twl);
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
     __syncthreads();
@@ -210,7 +214,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
     __syncthreads();
     Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F, q, tc);
   }
-  InvFFT<T, L, FP>::run(v, F, q, tc, p.twx);
+  InvFFT<T, L, FP>::run(v, F, q, tc, twl);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int d = mk_src(Tile<T, L, E, FP>::template pos_in<inv_last<L>()>(tc, e), L);
@@ -232,6 +236,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NT>(TW, p.twz, tid);
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
   static_assert(fwd_first<L>() == inv_last<L>(), "plan symmetry");
@@ -245,7 +251,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
       const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : ld_off(src, (uint32_t)(L - k) * rowb + q * CB);
       v[e] = dct3_in(ld_off(src, (uint32_t)k * rowb + q * CB), pn, qtw[k]);
     }
-    InvFFT<T, L, FP>::run(v, F, q, tc, p.twz);
+    InvFFT<T, L, FP>::run(v, F, q, tc, twl);
   } else {
     for (int i = tid; i < NC * L; i += NT) {
       const int c = i / L, z = i - c * L;
@@ -276,7 +282,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
       const double s = block_sum(acc, red);
       if (tid == 0) p.partial[blockIdx.x] = s;
     }
-    FwdFFT<T, L, FP>::run(v, F, q, tc, p.twz);
+    FwdFFT<T, L, FP>::run(v, F, q, tc, twl);
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
     __syncthreads();

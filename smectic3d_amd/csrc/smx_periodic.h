@@ -76,6 +76,8 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
   using TL = TileA<T, L, E, AD>;
   __shared__ cplx<T> S[NC * AD::LP];
   const int tid = threadIdx.x, tc = tid % TPC, col = tid / TPC;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NC * TPC>(TW, tw, tid);
   const int64_t pen = (int64_t)blockIdx.x * NC + col;
   const bool valid = pen < npencils;
   const cplx<T>* src = in + pen * L;
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
       v[e] = mk(-k * v[e].y, k * v[e].x);
     }
   }
-  FFT::run(v, S, col, tc, tw);
+  FFT::run(v, S, col, tc, twl);
   if (valid) {
     cplx<T>* dst = out + pen * L;
 #pragma unroll
@@ -105,16 +107,25 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
 // x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1, rows x.
 enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
 
-template <typename T, int L, int NC, int MODE, int V = 0>
+// ORD (X_STEP only) orders the carry traffic.  0: R is read and rewritten element by
+// element between the two transforms.  1: all of R is read first, the new R is kept in
+// registers through the inverse transform and every global store is issued last.  A
+// wave's loads and stores retire through one in-order counter, so a load issued behind
+// a store waits for that store; which order hides more latency depends on the precision
+// and tile shape, and the autotuner measures both.
+template <typename T, int L, int NC, int MODE, int V = 0, int ORD = 0>
 __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
                                                                cplx<T>* __restrict__ rbuf,
                                                                const cplx<T>* __restrict__ psi0,
                                                                PParams<T> p) {
+  static_assert(ORD == 0 || MODE == X_STEP, "the late-store order is a form of the CNAB2 step");
   constexpr int E = PlanV<L, V>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
   constexpr uint32_t CB = sizeof(cplx<T>);
   __shared__ cplx<T> S[L * NCP];
   __shared__ double red[64];
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;  // bytes between kx rows
@@ -157,9 +168,15 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
     for (int i = tid; i < p.npartial; i += NT) a += p.partial[i];
     nsum = block_sum(a, red);  // valid in thread 0 (= col 0, tc 0)
   }
+  cplx<T> cr[ORD == 1 ? E : 1];  // ORD 1: R, then the new R
   if constexpr (MODE != X_INV) {
-    FwdFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
+    FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
     const T kyv = p.ky[ky], kzv = p.kz[kz];
+    if constexpr (ORD == 1) {
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        cr[e] = ld_off(rb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs);
+    }
     if (herm) __syncthreads();  // S receives R' below; the FFT's last reads must be done
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -181,7 +198,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
         if constexpr (MODE == X_STEP) {
           // psi_new = (C1 psi + dt/2 (3 N - N_past))/C2 carried as R = C1 psi - dt/2 N_past
           // (pf3d.cpp:476-478): psi_new = (R + 3 dt/2 N)/C2, R' = C1 psi_new - dt/2 N.
-          const cplx<T> r = ld_off<cplx<T>>(rb, o);
+          const cplx<T> r = (ORD == 0) ? ld_off<cplx<T>>(rb, o) : cr[e];
           const T ic2 = recip(C2);
           cplx<T> ps = mk(ic2 * (r.x + (T)3 * p.dtd2 * nn.x), ic2 * (r.y + (T)3 * p.dtd2 * nn.y));
           cplx<T> rn = mk(C1 * ps.x - p.dtd2 * nn.x, C1 * ps.y - p.dtd2 * nn.y);
@@ -195,7 +212,8 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
             rn = mk((T)p.mean_state[0], (T)0);
           }
           if (herm) S[kxi * NCP + col] = rn;
-          else st_off(rb, o, rn);
+          else if constexpr (ORD == 0) st_off(rb, o, rn);
+          if constexpr (ORD == 1) cr[e] = rn;
           if (p.rec_psi != nullptr) st_off(p.rec_psi + plane, o, ps);
           v[e] = ps;
         } else {  // X_BOOT: R = C1 psi0 - dt/2 N_past with N_past = N(psi0) (pf3d.cpp:363-374)
@@ -219,7 +237,9 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
       for (int e = 0; e < E; ++e) {
         const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
         const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
-        st_off(rb, (uint32_t)kxi * rowb + cofs, mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)));
+        const cplx<T> h = mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y));
+        if constexpr (ORD == 0) st_off(rb, (uint32_t)kxi * rowb + cofs, h);
+        else cr[e] = h;
       }
     }
   }
@@ -229,10 +249,15 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
     for (int e = 0; e < E; ++e)
       st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
   } else {
-    InvFFT<T, L, NCP, V>::run(v, S, col, tc, p.twx);
+    InvFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
 #pragma unroll
     for (int e = 0; e < E; ++e)
       st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
+  }
+  if constexpr (ORD == 1) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      st_off(rb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs, cr[e]);
   }
 }
 
@@ -257,6 +282,8 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
+  __shared__ cplx<T> TW[L];
+  const cplx<T>* twl = tw_lds<L, NT>(TW, p.twz, tid);
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * CB;  // bytes between kz rows of one column
   const uint32_t qofs = (uint32_t)q * 2 * CB;  // the lane's column pair within a row
@@ -277,7 +304,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
       }
       v[e] = (n <= L / 2) ? mk(A.x - B.y, A.y + B.x) : mk(A.x + B.y, B.x - A.y);
     }
-    InvFFT<T, L, FP, V>::run(v, F, q, tc, p.twz);
+    InvFFT<T, L, FP, V>::run(v, F, q, tc, twl);
   } else {
     for (int i = tid; i < NC * L; i += NT) {
       const int c = i / L, n = i - c * L;
@@ -340,7 +367,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
       const double s = block_sum(acc, red);
       if (tid == 0) p.partial[blockIdx.x] = s;
     }
-    FwdFFT<T, L, FP, V>::run(v, F, q, tc, p.twz);
+    FwdFFT<T, L, FP, V>::run(v, F, q, tc, twl);
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L, V>()>(v, F, q, tc);
     __syncthreads();

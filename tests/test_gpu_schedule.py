@@ -1,7 +1,8 @@
 """Tile schedules and the autotuner of the periodic step (libsmectic through the C ABI).
 
-Every schedule candidate (columns per tile 8/16/32, radix-plan variant 0/1) of every
-step kernel must give the oracle's result to the usual tolerance: FP64 relative
+Every schedule candidate (columns per tile 8/16/32, radix-plan variant 0/1, and for the
+x pass the late carry-store forms 2/3) of every step kernel must give the oracle's
+result to the usual tolerance: FP64 relative
 L2(psi) <= 1e-11 after 5 steps, FP32-mixed <= 2e-5.  The autotuner must leave the
 state untouched, return a valid schedule, and the tuned step must keep parity.
 """
@@ -19,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 NCS = (8, 16, 32)
 VARIANTS = (0, 1)
+VARIANTS_X = (0, 1, 2, 3)
 DEFAULT = [(16, 0)] * 4
 
 
@@ -47,7 +49,7 @@ def test_every_schedule_candidate_matches_oracle(shape, prec):
         assert s.schedule() == DEFAULT
         for slot in range(4):
             for nc in NCS:
-                for v in VARIANTS:
+                for v in (VARIANTS_X if slot == 3 else VARIANTS):
                     sched = list(DEFAULT)
                     sched[slot] = (nc, v)
                     try:
@@ -78,9 +80,10 @@ def test_every_schedule_candidate_matches_oracle(shape, prec):
 def test_invalid_schedules_rejected():
     say("invalid schedules")
     with Solver(cfg_reference(64, 48, 32)) as s:
-        for bad in ([(7, 0)] + DEFAULT[1:], [(16, 2)] + DEFAULT[1:], DEFAULT[:3] + [(32, 0)]):
+        for bad in ([(7, 0)] + DEFAULT[1:], [(16, 2)] + DEFAULT[1:], DEFAULT[:3] + [(32, 0)],
+                    DEFAULT[:3] + [(16, 4)]):
             with pytest.raises(SmxError) as ei:
-                s.set_schedule(bad)  # nc 7; variant 2; x tiles of 32 do not divide ny = 48
+                s.set_schedule(bad)  # nc 7; y variant 2; x tiles of 32 do not divide ny = 48; x variant 4
             assert ei.value.code == A.SMX_ERR_UNSUPPORTED
         assert s.schedule() == DEFAULT
     with Solver(cfg_reference(32, 32, 32, bc=A.SMX_BC_NEUMANN)) as s:
@@ -110,7 +113,7 @@ def test_autotune_keeps_state_and_parity(prec):
         chosen = s.autotune(5)
         say("  chosen", chosen)
         assert s.schedule() == chosen
-        assert all(nc in NCS and v in VARIANTS for nc, v in chosen)
+        assert all(nc in NCS and v in (VARIANTS_X if k == 3 else VARIANTS) for k, (nc, v) in enumerate(chosen))
         assert np.array_equal(s.get_psi(), before)  # tuning ran on scratch buffers only
         assert s.energy() == e_before
         s.step(10)
