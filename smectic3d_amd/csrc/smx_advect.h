@@ -339,14 +339,9 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_advnl
 #pragma unroll
     for (int e = 0; e < E; ++e) nacc[e] = mk(nacc[e].x - u[e].x * v[e].x, nacc[e].y - u[e].y * v[e].y);
   }
-  if (p.partial != nullptr) {
-    double acc = 0.0;
-#pragma unroll
-    for (int e = 0; e < E; ++e) acc += (double)nacc[e].x + (double)nacc[e].y;
-    const double s = block_sum(acc, red);
-    if (tid == 0) p.partial[blockIdx.x] = s;
-  }
-  Z::r2c_store(nacc, nout, f0, rowb, q, tc, tid, F, twl);
+  if (p.partial != nullptr) wave_partial<E, Z::NT>(nacc, red, tid);
+  Z::r2c_store(nacc, nout, f0, rowb, q, tc, tid, F, twl);  // its barriers order red[] for:
+  if (p.partial != nullptr) store_partial<Z::NT>(red, p.partial + blockIdx.x, tid);
 }
 
 }  // namespace smx

@@ -302,6 +302,37 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// FP64 partial over a tile of the sum of x + y of the E complex values each thread holds,
+// without a barrier of its own: a pairwise tree per thread, a shuffle tree per wave into
+// red[] (wave_partial), and thread 0 adds the waves in order (store_partial) after some
+// later barrier of the kernel has made red[] visible.  red: >= 64 doubles.
+template <int NT> constexpr int partial_slots() { return (NT % 64 == 0) ? NT / 64 : NT; }
+template <int E, int NT, typename T>
+__device__ __forceinline__ void wave_partial(const cplx<T>* v, double* red, int tid) {
+  static_assert(NT % 64 == 0 || NT <= 64, "partial sums: whole waves or one partial wave");
+  double t[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) t[e] = (double)v[e].x + (double)v[e].y;
+#pragma unroll
+  for (int w = 1; w < E; w *= 2)
+#pragma unroll
+    for (int e = 0; e + w < E; e += 2 * w) t[e] += t[e + w];
+  if constexpr (NT % 64 == 0) {
+    const double ws = wave_sum(t[0]);
+    if ((tid & 63) == 0) red[tid >> 6] = ws;
+  } else {
+    red[tid] = t[0];
+  }
+}
+template <int NT>
+__device__ __forceinline__ void store_partial(const double* red, double* out, int tid) {
+  if (tid == 0) {
+    double s = 0.0;
+    for (int w = 0; w < partial_slots<NT>(); ++w) s += red[w];
+    *out = s;
+  }
+}
+
 // Sum over the block; result valid in thread 0.  scratch: >= max(nwaves, 64)
 // doubles when blockDim.x is not a multiple of 64, else >= nwaves doubles.
 // Shuffles are used only when every wave of the block is full.

@@ -272,16 +272,16 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
     }
   }
   if constexpr ((OUT & Z_OUT_SPEC) != 0) {
-    double acc = 0.0;
+    if (op == Z_OP_NL) {  // cosNoAdvMorph.cpp:502-503
+      if (p.nl_model == 1) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (op == Z_OP_NL) v[e] = mk(nonlin(p, v[e].x), nonlin(p, v[e].y));  // cosNoAdvMorph.cpp:502-503
-      acc += (double)v[e].x + (double)v[e].y;
+        for (int e = 0; e < E; ++e) v[e] = mk(nonlin3(p, v[e].x), nonlin3(p, v[e].y));
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = mk(nonlin5(p, v[e].x), nonlin5(p, v[e].y));
+      }
     }
-    if (p.partial != nullptr) {
-      const double s = block_sum(acc, red);
-      if (tid == 0) p.partial[blockIdx.x] = s;
-    }
+    if (p.partial != nullptr) wave_partial<E, NT>(v, red, tid);  // stored after the split
     FwdFFT<T, L, FP>::run(v, F, q, tc, twl);
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
@@ -291,6 +291,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
       const int k = i / NPR, qq = i - k * NPR;
       st_off(dst, (uint32_t)k * rowb + qq * CB, dct2_out(F[k * FP + qq], F[((L - k) % L) * FP + qq], qtw[k]));
     }
+    if (p.partial != nullptr) store_partial<NT>(red, p.partial + blockIdx.x, tid);
   }
 }
 

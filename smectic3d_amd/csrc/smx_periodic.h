@@ -54,12 +54,20 @@ __device__ __forceinline__ T nonlin_prime(const PParams<T>& p, T s) {
   return s2 * ((T)3 * p.beta - (T)5 * p.gamma * s2);
 }
 
-// Local nonlinearity: beta psi^3 - gamma psi^5 (pf3d.cpp:460-461) or the cubic model.
+// Local nonlinearity: beta psi^3 - gamma psi^5 (pf3d.cpp:460-461) or the cubic model
+// beta psi^2 - gamma psi^3.
+template <typename T>
+__device__ __forceinline__ T nonlin3(const PParams<T>& p, T s) {
+  return s * s * (p.beta - p.gamma * s);
+}
+template <typename T>
+__device__ __forceinline__ T nonlin5(const PParams<T>& p, T s) {
+  const T s2 = s * s;
+  return s2 * s * (p.beta - p.gamma * s2);
+}
 template <typename T>
 __device__ __forceinline__ T nonlin(const PParams<T>& p, T s) {
-  if (p.nl_model == 1) return s * s * (p.beta - p.gamma * s);
-  T s2 = s * s;
-  return s2 * s * (p.beta - p.gamma * s2);
+  return (p.nl_model == 1) ? nonlin3(p, s) : nonlin5(p, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -343,17 +351,28 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
         row[1] = nonlin_prime(p, v[e].y);
       }
     }
-    double acc = 0.0;
+    // the op and model branches are uniform: one straight loop per case
+    if (op == Z_OP_MUL) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (op == Z_OP_MUL) {
+      for (int e = 0; e < E; ++e) {
         const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
         const T* row = p.mul_np + (int64_t)n * p.nxy + f0 + 2 * q;
         v[e] = mk(row[0] * v[e].x, row[1] * v[e].y);
-      } else if (op == Z_OP_NL) {
+      }
+    } else if (op == Z_OP_NL && !pen) {
+      if (p.nl_model == 1) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = mk(nonlin3(p, v[e].x), nonlin3(p, v[e].y));
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = mk(nonlin5(p, v[e].x), nonlin5(p, v[e].y));
+      }
+    } else if (op == Z_OP_NL) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
         const int n = Tile<T, L, E, FP>::template pos_in<RP>(tc, e);
         T a = nonlin(p, v[e].x), b = nonlin(p, v[e].y);
-        if (pen && n == p.mid) {
+        if (n == p.mid) {
           // 2 wMid (psi_{k-1} - 2 psi_k + psi_{k+1})/dz^2 at k = mid (pf3dMorph.cpp:572-581)
           const cplx<T> lo = F[(n - 1) * FP + q], hi = F[(n + 1) * FP + q];
           a += p.pen * (lo.x - (T)2 * v[e].x + hi.x);
@@ -361,12 +380,9 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
         }
         v[e] = mk(a, b);
       }
-      acc += (double)v[e].x + (double)v[e].y;
     }
-    if (p.partial != nullptr) {
-      const double s = block_sum(acc, red);
-      if (tid == 0) p.partial[blockIdx.x] = s;
-    }
+    // FP64 partial of sum N over the tile; stored after the split (its barriers order it)
+    if (p.partial != nullptr) wave_partial<E, NT>(v, red, tid);
     FwdFFT<T, L, FP, V>::run(v, F, q, tc, twl);
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L, V>()>(v, F, q, tc);
@@ -381,6 +397,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
       xy.b = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
       st_off(dst, (uint32_t)k * rowb + (uint32_t)qq * 2 * CB, xy);
     }
+    if (p.partial != nullptr) store_partial<NT>(red, p.partial + blockIdx.x, tid);
   }
 }
 
