@@ -188,9 +188,10 @@ int smx_batch_graph_nodes(smx_batch *b, int64_t *nodes);
  * Tile schedules of the four kernels of the periodic step, in launch order (slot 0: inverse
  * y pass, 1: z pass with N(psi), 2: forward y pass, 3: x pass with the CNAB2 update): the
  * columns per tile (8, 16 or 32) and the radix-plan variant (0: default; 1: the alternative
- * plan of the power-of-two lengths).  Slot 3 also takes variants 2 and 3: plans 0 and 1
- * with the x pass's R stores issued after its inverse transform instead of between the
- * transforms (bit-identical results).  The step's result depends on the schedule only
+ * plan of the power-of-two lengths).  Slot 3 also takes variants 2 to 5: plans 0 and 1
+ * with the x pass's R stores issued after its inverse transform (2, 3), or with all of R
+ * read before any of it is rewritten (4, 5), instead of R read and rewritten element by
+ * element (bit-identical results).  The step's result depends on the schedule only
  * through rounding.  The default is nc = 16, variant = 0 in every slot.  The advected step
  * uses the schedules of slots 0 and 2 for its eleven y passes.
  */
@@ -206,7 +207,8 @@ int smx_set_schedule(smx_handle *h, const smx_schedule *s);
 /* Autotuner: slot by slot, time every valid candidate (reps whole step chains each, HIP
  * events on the handle's stream) on a scratch copy of the state -- hashed noise when none
  * is set; the state itself is not touched -- install the fastest and return it in *chosen
- * (may be NULL).  Results are cached per (device,
+ * (may be NULL); with SMX_TUNE_VERBOSE=1 in the environment each candidate's median is
+ * printed to stderr.  Results are cached per (device,
  * precision, grid) for the life of the process.  Replaces the FFTW_MEASURE planning of
  * pf3d.cpp:226-291 (which times candidate FFT algorithms at set-up).  Periodic handles
  * only; not callable during stream capture. */

@@ -4,6 +4,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -921,6 +922,8 @@ static int autotune_T(smx_handle* h, int reps, PassSched* best) {
   };
   PassSched cur[NSLOTS];
   for (int k = 0; k < NSLOTS; ++k) cur[k] = PassSched{};
+  const char* vb = std::getenv("SMX_TUNE_VERBOSE");  // per-candidate medians on stderr
+  const bool verbose = vb != nullptr && vb[0] != '\0' && vb[0] != '0';
   std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
   hipError_t e = hipSuccess;
   for (size_t i = 0; i < ev.size() && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
@@ -930,7 +933,7 @@ static int autotune_T(smx_handle* h, int reps, PassSched* best) {
     double tbest = 1e300;
     PassSched pick = cur[k];
     const int* vs = (k == SLOT_X) ? kSchedVX : kSchedV;
-    const int nv = (k == SLOT_X) ? 4 : 2;
+    const int nv = (k == SLOT_X) ? 6 : 2;
     for (int iv = 0; iv < nv && rc == SMX_OK; ++iv)
       for (int nc : kSchedNC) {
         const PassSched c{nc, vs[iv]};
@@ -950,6 +953,9 @@ static int autotune_T(smx_handle* h, int reps, PassSched* best) {
         }
         std::nth_element(dur.begin(), dur.begin() + reps / 2, dur.end());
         const double us = 1000.0 * dur[reps / 2];
+        if (verbose)
+          std::fprintf(stderr, "smx_autotune %s slot %d nc %2d variant %d: %8.2f us\n", h->fp32 ? "fp32" : "fp64", k,
+                       nc, vs[iv], us);
         if (us < tbest) {
           tbest = us;
           pick = c;

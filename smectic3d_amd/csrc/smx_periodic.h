@@ -116,11 +116,12 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
 enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
 
 // ORD (X_STEP only) orders the carry traffic.  0: R is read and rewritten element by
-// element between the two transforms.  1: all of R is read first, the new R is kept in
-// registers through the inverse transform and every global store is issued last.  A
-// wave's loads and stores retire through one in-order counter, so a load issued behind
-// a store waits for that store; which order hides more latency depends on the precision
-// and tile shape, and the autotuner measures both.
+// element between the two transforms.  1: all of R (and k_x) is read first, the new R is
+// kept in registers through the inverse transform and every global store is issued last.
+// 2: all of R and k_x is read first and each new R is stored as soon as it is formed.  A
+// wave's loads and stores retire through one in-order counter, so a load issued behind a
+// store waits for that store; which order hides more latency depends on the precision
+// and tile shape, and the autotuner measures all three.
 template <typename T, int L, int NC, int MODE, int V = 0, int ORD = 0>
 __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
                                                                cplx<T>* __restrict__ rbuf,
@@ -176,14 +177,18 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
     for (int i = tid; i < p.npartial; i += NT) a += p.partial[i];
     nsum = block_sum(a, red);  // valid in thread 0 (= col 0, tc 0)
   }
-  cplx<T> cr[ORD == 1 ? E : 1];  // ORD 1: R, then the new R
+  cplx<T> cr[ORD >= 1 ? E : 1];  // ORD 1, 2: R (ORD 1: then the new R)
+  T kxr[ORD >= 1 ? E : 1];
   if constexpr (MODE != X_INV) {
     FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
     const T kyv = p.ky[ky], kzv = p.kz[kz];
-    if constexpr (ORD == 1) {
+    if constexpr (ORD >= 1) {
 #pragma unroll
-      for (int e = 0; e < E; ++e)
-        cr[e] = ld_off(rb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs);
+      for (int e = 0; e < E; ++e) {
+        const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
+        cr[e] = ld_off(rb, (uint32_t)kxi * rowb + cofs);
+        kxr[e] = p.kx[kxi];
+      }
     }
     if (herm) __syncthreads();  // S receives R' below; the FFT's last reads must be done
 #pragma unroll
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
           v[e] = mk((T)p.mean_state[1], (T)0);
         }
       } else {
-        const T kxv = p.kx[kxi];
+        const T kxv = (ORD >= 1) ? kxr[e] : p.kx[kxi];
         const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
         const T d = k2 - p.q02;
         const T aLin = p.eps - p.alpha * d * d;  // pf3d.cpp:205-206
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
             rn = mk((T)p.mean_state[0], (T)0);
           }
           if (herm) S[kxi * NCP + col] = rn;
-          else if constexpr (ORD == 0) st_off(rb, o, rn);
+          else if constexpr (ORD != 1) st_off(rb, o, rn);
           if constexpr (ORD == 1) cr[e] = rn;
           if (p.rec_psi != nullptr) st_off(p.rec_psi + plane, o, ps);
           v[e] = ps;
@@ -246,7 +251,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
         const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
         const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
         const cplx<T> h = mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y));
-        if constexpr (ORD == 0) st_off(rb, (uint32_t)kxi * rowb + cofs, h);
+        if constexpr (ORD != 1) st_off(rb, (uint32_t)kxi * rowb + cofs, h);
         else cr[e] = h;
       }
     }
