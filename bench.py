@@ -274,7 +274,7 @@ def main():
 
     if a.leg:  # one secondary workload alone (profiling runs)
         runs = {"neumann": (run_neumann, 4000), "advected": (run_advected, 1000),
-                "batch": (run_batch, 500), "adjoint": (lambda k: run_adjoint(k, 50), 2000)}
+                "batch": (lambda k: run_batch(k, tune=not a.no_tune), 500), "adjoint": (lambda k: run_adjoint(k, 50), 2000)}
         fn, default_steps = runs[a.leg]
         print(json.dumps({"leg": a.leg, "result": fn(a.leg_steps or default_steps)}, default=str))
         return

@@ -204,7 +204,8 @@ int smx_get_schedule(const smx_handle *h, smx_schedule *s);
  * many threads or too much LDS per tile, or ny / nx*ny not a multiple of nc), or if a
  * non-default schedule is given to a Neumann handle. */
 int smx_set_schedule(smx_handle *h, const smx_schedule *s);
-/* Autotuner: slot by slot, time every valid candidate (reps whole step chains each, HIP
+/* Autotuner: slot by slot, time every valid candidate (reps whole step chains each -- up to
+ * 16x more on grids smaller than 256^3, whose kernels are short --, HIP
  * events on the handle's stream) on a scratch copy of the state -- hashed noise when none
  * is set; the state itself is not touched -- install the fastest and return it in *chosen
  * (may be NULL); with SMX_TUNE_VERBOSE=1 in the environment each candidate's median is

@@ -885,7 +885,11 @@ static int sched_check(smx_handle* h, const PassSched* s) {
 // with the others at their picks so far.  The handle's own state and mean mode are not
 // touched.
 template <typename T>
-static int autotune_T(smx_handle* h, int reps, PassSched* best) {
+static int autotune_T(smx_handle* h, int reps_in, PassSched* best) {
+  // small grids have kernels of a few microseconds, where event timing jitters by a
+  // similar amount: scale the repetitions up (x1 at 256^3, up to x16)
+  const double pts = (double)h->cfg.nx * h->cfg.ny * h->cfg.nz;
+  const int reps = std::min(1000, reps_in * (int)std::min(16.0, std::max(1.0, 16777216.0 / pts)));
   PParams<T> p = params<T>(h, h->fp32);
   if (p.partial) p.partial = h->partial + partial_len(h);  // the tuner's own partial-sum region
   p.mean_state = h->mean_state + 4;                        // scratch entries 4..7
