@@ -57,12 +57,16 @@ def hyperboloid_ic(n, dx, q0=1.0, a=0.4, noise=1e-3, seed=0):
     return psi + noise * (2.0 * u.reshape(n, n, n) - 1.0)
 
 
-def run_path(precision, psi0, steps, warmup, prof_steps, tune=True):
+def run_path(precision, psi0, steps, warmup, prof_steps, tune=True, fixed=None):
     from smectic3d_amd import Solver
     cfg = workload_cfg(precision)
     s = Solver(cfg)
     t0 = time.perf_counter()
-    sched = s.autotune(20) if tune else s.schedule()
+    if fixed is not None:  # schedules of an earlier run (profiled runs use the same kernels)
+        s.set_schedule([tuple(x) for x in fixed])
+        sched = s.schedule()
+    else:
+        sched = s.autotune(20) if tune else s.schedule()
     tune_s = time.perf_counter() - t0
     s.set_psi(psi0)
     s.step(warmup)
@@ -192,6 +196,12 @@ def run_adjoint(steps, ckpt):
         s.step(steps)
         s.sync()
         tf = time.perf_counter() - t0
+        # the forward solve with the same host I/O as the gradient call (psi0 in, one field out)
+        t0 = time.perf_counter()
+        s.set_psi(psi0)
+        s.step(steps)
+        s.get_psi()
+        tfio = time.perf_counter() - t0
         t0 = time.perf_counter()
         s.adjoint_grad(psi0, steps, ckpt)  # first call: allocates and first-touches the record sets
         tcold = time.perf_counter() - t0
@@ -199,11 +209,15 @@ def run_adjoint(steps, ckpt):
         J, d, de, dq = s.adjoint_grad(psi0, steps, ckpt)
         tg = time.perf_counter() - t0
         res, rec, nbytes = s.adjoint_info()
-    return {"steps": steps, "ckpt_every": ckpt, "grads_per_s": 1.0 / tg, "forward_s": tf, "gradient_s": tg,
-            "gradient_over_forward": tg / tf, "first_call_s": tcold, "first_call_over_forward": tcold / tf,
+    return {"steps": steps, "ckpt_every": ckpt, "grads_per_s": 1.0 / tg, "forward_s": tf,
+            "forward_with_io_s": tfio, "gradient_s": tg,
+            "gradient_over_forward": tg / tfio, "gradient_over_forward_steps_only": tg / tf,
+            "first_call_s": tcold, "first_call_over_forward": tcold / tfio,
             "resident_steps": res, "recomputed_steps": rec, "device_GB": nbytes / 1e9,
             "note": "gradient_s: a repeated call (record sets kept in the handle); first_call_s includes "
-                    "allocating and first-touching them; both include the host transfers of psi0 and dJ/dpsi0",
+                    "allocating and first-touching them; both include the host transfers of psi0 and "
+                    "dJ/dpsi0, as forward_with_io_s includes those of psi0 and psi_T (the forward solve "
+                    "host to host); forward_s times the steps alone",
             "loss": J, "dJ_deps": de, "dJ_dq0": dq}
 
 
@@ -265,6 +279,9 @@ def main():
     ap.add_argument("--leg", choices=["neumann", "advected", "batch", "adjoint"],
                     help="run only this secondary workload (with --leg-steps steps) and print its result")
     ap.add_argument("--leg-steps", type=int, default=None)
+    ap.add_argument("--schedules-from", default=None,
+                    help="install the tile schedules recorded in an earlier bench JSON line instead of "
+                         "autotuning (profiled runs then time the same kernels)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
@@ -283,9 +300,13 @@ def main():
     dx = (n / 8 * 2 * math.pi) / n
     psi0 = hyperboloid_ic(n, dx)
     log(f"workload 256^3, steps={a.steps} warmup={a.warmup}")
-    r32 = run_path(A.SMX_FP32_MIXED, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune)
+    f32 = f64 = None
+    if a.schedules_from:
+        prev = json.load(open(a.schedules_from))
+        f32, f64 = prev["schedule_fp32"], prev["fp64"]["schedule"]
+    r32 = run_path(A.SMX_FP32_MIXED, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune, fixed=f32)
     log(f"fp32-mixed: {r32['steps_per_s']:.1f} steps/s schedule {r32['schedule']}")
-    r64 = run_path(A.SMX_FP64, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune)
+    r64 = run_path(A.SMX_FP64, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune, fixed=f64)
     log(f"fp64: {r64['steps_per_s']:.1f} steps/s")
 
     err_psi = float(np.linalg.norm(r32["psi"] - r64["psi"]) / np.linalg.norm(r64["psi"]))

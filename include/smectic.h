@@ -157,7 +157,7 @@ int smx_adjoint_grad(smx_handle *h, const double *psi0, int64_t row_stride, int6
                      double target_amp, double *loss, double *dpsi0, double *deps, double *dq0);
 /* Device memory the adjoint may spend on keeping recorded segments resident (N'(psi_n) and
  * psi_hat_n of whole checkpoint segments, recorded during the forward sweep and never
- * recomputed).  bytes < 0: 60% of the free device memory at the time of the call (default);
+ * recomputed).  bytes < 0: 80% of the free device memory at the time of the call (default);
  * 0: checkpoints only (every segment recomputed once).  The gradient does not depend
  * on it.  The record sets stay allocated in the handle between smx_adjoint_grad calls;
  * this call and smx_destroy release them.  No counterpart in the reference. */

@@ -97,7 +97,7 @@ struct smx_handle {
   bool has_state = false, has_prev = false;
   int64_t steps = 0;
   PassSched sched[NSLOTS];  // tile schedules of the periodic step kernels
-  int64_t adj_budget = -1;  // device bytes the adjoint may keep resident (-1: 60% of free memory)
+  int64_t adj_budget = -1;  // device bytes the adjoint may keep resident (-1: 80% of free memory)
   int64_t adj_info[3] = {0, 0, 0};  // last adjoint: resident steps, recomputed steps, device bytes
   // adjoint record sets kept between calls (N'(psi_n) and psi_hat_n of one segment each)
   std::vector<void*> adj_np, adj_psi;
@@ -604,11 +604,11 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     for (void* q : tmp) (void)hipFree(q);
   };
   int64_t budget = h->adj_budget;
-  if (budget < 0) {  // 60% of the device memory that is free or already held by the record sets
+  if (budget < 0) {  // 80% of the device memory that is free or already held by the record sets
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     const double held = (double)h->adj_np.size() * (double)(h->adj_np_bytes + h->adj_psi_bytes);
-    budget = (int64_t)(0.6 * ((double)fr + held));
+    budget = (int64_t)(0.8 * ((double)fr + held));
   }
   const int64_t nres_want = std::min<int64_t>(nseg, budget / (int64_t)setbytes);
   void *AS = nullptr, *AC = nullptr;

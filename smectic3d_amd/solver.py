@@ -121,7 +121,7 @@ class Solver:
         return J.value, d, de.value, dq.value
 
     def set_adjoint_budget(self, nbytes):
-        """Device bytes the adjoint may keep resident (-1: 60% of free memory; 0: checkpoints only)."""
+        """Device bytes the adjoint may keep resident (-1: 80% of free memory; 0: checkpoints only)."""
         check(lib().smx_set_adjoint_budget(self._h, int(nbytes)))
 
     def adjoint_info(self):
