@@ -147,10 +147,9 @@ def run_advected(steps):
             "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
 
 
-def run_batch(steps, tune=True):
+def batch_cfgs():
     """24 independent periodic boxes, sizes from {64^3, 96^3, 128^3} by seed, eps ~ U[0.2,0.6],
-    q0 ~ U[0.8,1.2], gamma=1, dt=0.05, noise 0.05, advanced in one captured hipGraph (FP32-mixed)."""
-    from smectic3d_amd import Batch
+    q0 ~ U[0.8,1.2], gamma=1, dt=0.05 (FP32-mixed)."""
     from smectic3d_amd import _abi as A
     rng = np.random.default_rng(2024)
     cfgs = []
@@ -160,6 +159,13 @@ def run_batch(steps, tune=True):
         cfgs.append(A.make_config(n, lx=L, ly=L, lz=L, eps=float(rng.uniform(0.2, 0.6)),
                                   q0=float(rng.uniform(0.8, 1.2)), beta=0.0, gamma=1.0, dt=0.05,
                                   nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED))
+    return cfgs
+
+
+def run_batch(steps, tune=True):
+    """The 24 boxes of batch_cfgs(), noise 0.05, advanced in one captured hipGraph."""
+    from smectic3d_amd import Batch
+    cfgs = batch_cfgs()
     pts = sum(c.nx * c.ny * c.nz for c in cfgs)
     with Batch(cfgs) as b:
         t0 = time.perf_counter()
@@ -240,7 +246,7 @@ def measured_traffic(slot, schedule, n):
     inst = {"k_ypass_inv": f"smx::k_ypass<float, {n}, 1, {nc}, {v}>",
             "k_zpass_nl": f"smx::k_zpass<float, {n}, {nc}, 0, 1, {v}>",
             "k_ypass_fwd": f"smx::k_ypass<float, {n}, -1, {nc}, {v}>",
-            "k_xpass_step": f"smx::k_xpass<float, {n}, {nc}, 0, {v}>"}[slot]
+            "k_xpass_step": f"smx::k_xpass<float, {n}, {nc}, 0, {v & 1}, {v >> 1}>"}[slot]
     k = kernels.get(inst)
     if not k or "FETCH_SIZE_bytes" not in k or "WRITE_SIZE_bytes" not in k:
         return None
@@ -361,12 +367,12 @@ def main():
         "cpu_baseline": None,
     }
     if not a.no_extra:
+        log("batch ...")
+        line["batch_24_boxes_fp32"] = run_batch(500, tune=not a.no_tune)
         log("neumann + surftrack ...")
         line["neumann_surftrack_256x256x384_fp32"] = run_neumann(4000)
         log("advected ...")
         line["advected_256_fp32"] = run_advected(1000)
-        log("batch ...")
-        line["batch_24_boxes_fp32"] = run_batch(500, tune=not a.no_tune)
         log("adjoint ...")
         line["adjoint_256_fp32"] = run_adjoint(2000, 50)
     if not a.no_cpu:
