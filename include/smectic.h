@@ -127,6 +127,15 @@ int smx_dominant_wavenumber(smx_handle *h, double *q);
  * surfTrack.cpp:611,622. */
 int smx_dtpsi(smx_handle *h, double *dtpsi, int64_t row_stride);
 
+/* Local amplitude psiAvg of the current state (host, reference layout): the trapezoid
+ * integral of psi^2 over the window of +-nw/2 points about each point (clipped to the
+ * grid), divided by the same integral over the reference box [nw/2, 3nw/2]^3 times
+ * l1 l2 l3 / lambda^3 (l: clipped window lengths, lambda = 2 pi/q0).  Replaces the
+ * O(N^3 nw^3) loops of cosFFT/comet/morphSurfAmp.cpp:1005-1134 (rank 0, lambda^3 = 8 pi^3
+ * for q0 = 1) by three separable window passes on the GPU in FP64.  SMX_ERR_INVALID unless
+ * nw >= 2 and 3(nw/2) <= n - 1 on every axis. */
+int smx_psi_avg(smx_handle *h, int nw, double *out, int64_t row_stride);
+
 /* surfTrack column scan at the psi = 0 crossing of every (i,j) column.
  * Periodic: surfTrack.cpp:767-806 (scan k upward, arm at psi > arm, spectral
  * derivatives, H ~ (psi_xx+psi_yy)/|grad psi|, K = (psi_xx psi_yy - psi_xy^2)/|grad psi|^2).

@@ -58,6 +58,9 @@ def lib():
                                         C.c_uint64, _dp]
         L.or_growth_sweep.argtypes = [C.POINTER(SmxConfig), C.c_double, C.c_double, C.c_int, _dp]
         L.or_growth_sweep.restype = C.c_int
+        L.or_psi_avg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                 C.c_double, _dp, _dp]
+        L.or_psi_avg.restype = C.c_int
         L.or_adjoint_grad.argtypes = [C.POINTER(SmxConfig), _dp, C.c_int64, C.c_double, _dp, _dp, _dp, _dp]
         L.or_adjoint_grad.restype = C.c_int
         _lib = L
@@ -141,6 +144,16 @@ def growth_sweep(cfg, mq0, dmq, nq):
     out = np.zeros(nq)
     rc = lib().or_growth_sweep(C.byref(cfg), mq0, dmq, nq, _p(out))
     assert rc == 0
+    return out
+
+
+def psi_avg(psi, dx, dy, dz, nw, q0=1.0):
+    """morphSurfAmp.cpp:1005-1134 literally (O(N^3 nw^3)): small grids only."""
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    out = np.zeros_like(psi)
+    rc = lib().or_psi_avg(*psi.shape, dx, dy, dz, nw, q0, _p(psi), _p(out))
+    if rc != 0:
+        raise ValueError("reference box [nw/2, 3nw/2]^3 does not fit the grid")
     return out
 
 

@@ -903,6 +903,60 @@ void or_ic_hyperboloid(int nx, int ny, int nz, double dx, double dy, double dz, 
 }
 
 /* pf3dVal.cpp:214-326 */
+/* trapz of morphSurfAmp.cpp:76-86: 0.5 h sum_{n < nend-nstart} (f[n] + f[n+1]) */
+static double ref_trapz(int nstart, int nend, double h, const double *f) {
+  double t = 0.0;
+  for (int n = 0; n < nend - nstart; n++) t = t + f[n] + f[n + 1];
+  return 0.5 * h * t;
+}
+
+/* trapz over x (innermost), then y, then z of psi^2 on [i0,i1]x[j0,j1]x[k0,k1]
+ * (morphSurfAmp.cpp:1018-1049 and 1085-1114) */
+static double ref_box(int ny, int nz, double dx, double dy, double dz, const double *psi, int i0, int i1, int j0,
+                      int j1, int k0, int k1, double *fx, double *fy, double *fz) {
+  int c3 = 0;
+  for (int k2 = k0; k2 < k1 + 1; k2++) {
+    int c2 = 0;
+    for (int j2 = j0; j2 < j1 + 1; j2++) {
+      int c1 = 0;
+      for (int i2 = i0; i2 < i1 + 1; i2++) {
+        const double v = psi[((int64_t)i2 * ny + j2) * nz + k2];
+        fx[c1++] = v * v;
+      }
+      fy[c2++] = ref_trapz(i0, i1, dx, fx);
+    }
+    fz[c3++] = ref_trapz(j0, j1, dy, fy);
+  }
+  return ref_trapz(k0, k1, dz, fz);
+}
+
+int or_psi_avg(int nx, int ny, int nz, double dx, double dy, double dz, int nw, double q0, const double *psi,
+               double *out) {
+  const int nl = nw / 2; /* nlambda = Nw/2, :1011 */
+  if (nl < 1 || 3 * nl > nx - 1 || 3 * nl > ny - 1 || 3 * nl > nz - 1) return -1;
+  const int wmax = 2 * nl + 1;
+  double *fx = (double *)malloc(sizeof(double) * 3 * wmax);
+  if (!fx) return -2;
+  double *fy = fx + wmax, *fz = fy + wmax;
+  /* psiRef over [nl, 3 nl]^3, :1013-1049 */
+  const double ref = ref_box(ny, nz, dx, dy, dz, psi, nl, 3 * nl, nl, 3 * nl, nl, 3 * nl, fx, fy, fz);
+  const double lam = 2.0 * M_PI / q0, lam3 = lam * lam * lam; /* 8 pi^3 at q0 = 1, :1053 */
+  for (int k = 0; k < nz; k++) {
+    const int k0 = k - nl < 0 ? 0 : k - nl, k1 = k + nl > nz - 1 ? nz - 1 : k + nl;
+    for (int j = 0; j < ny; j++) {
+      const int j0 = j - nl < 0 ? 0 : j - nl, j1 = j + nl > ny - 1 ? ny - 1 : j + nl;
+      for (int i = 0; i < nx; i++) {
+        const int i0 = i - nl < 0 ? 0 : i - nl, i1 = i + nl > nx - 1 ? nx - 1 : i + nl;
+        const double l1 = (i1 - i0) * dx, l2 = (j1 - j0) * dy, l3 = (k1 - k0) * dz; /* :1078-1080 */
+        const double t = ref_box(ny, nz, dx, dy, dz, psi, i0, i1, j0, j1, k0, k1, fx, fy, fz);
+        out[((int64_t)i * ny + j) * nz + k] = t / (ref * l1 * l2 * l3 / lam3); /* :1118 */
+      }
+    }
+  }
+  free(fx);
+  return 0;
+}
+
 int or_growth_sweep(const smx_config *cfg, double mq0, double dmq, int nq, double *growth) {
   or_state *s = or_create(cfg);
   if (!s) return SMX_ERR_INVALID;

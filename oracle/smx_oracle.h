@@ -59,6 +59,14 @@ void or_ic_hyperboloid(int nx, int ny, int nz, double dx, double dy, double dz, 
  * psi0 = 1e-4 cos(mq*k*dz), for nq values mq = mq0 + i*dmq. */
 int or_growth_sweep(const smx_config *cfg, double mq0, double dmq, int nq, double *growth);
 
+/* morphSurfAmp.cpp:1005-1134: local amplitude psiAvg of a dense [x][y][z] field --
+ * trapezoid integral of psi^2 over the window of +-nw/2 points about each point (clipped
+ * to the grid), over the same integral on the reference box [nw/2, 3nw/2]^3 times
+ * l1 l2 l3 / lambda^3, lambda = 2 pi/q0.  Literal O(N^3 nw^3) restatement.  Returns -1
+ * when the reference box does not fit the grid. */
+int or_psi_avg(int nx, int ny, int nz, double dx, double dy, double dz, int nw, double q0, const double *psi,
+               double *out);
+
 /* Discrete adjoint of the periodic CNAB2 step (no advection/penalty):
  * J = 1/2 sum (psi_T - amp cos(q0 z))^2 and dJ/dpsi0, dJ/deps, dJ/dq0. */
 int or_adjoint_grad(const smx_config *cfg, const double *psi0, int64_t T, double amp, double *loss, double *dpsi0,

@@ -1345,6 +1345,33 @@ int smx_dtpsi(smx_handle* h, double* out, int64_t row_stride) {
   return field_to_host(h, h->real_io, out, row_stride);
 }
 
+int smx_psi_avg(smx_handle* h, int nw, double* out, int64_t row_stride) {
+  if (!h || !out) return fail(SMX_ERR_INVALID, "null argument");
+  if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  const smx_config& c = h->cfg;
+  if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
+  const int hw = nw / 2;  // nlambda = Nw/2, morphSurfAmp.cpp:1011
+  if (hw < 1 || 3 * hw > c.nx - 1 || 3 * hw > c.ny - 1 || 3 * hw > c.nz - 1)
+    return fail(SMX_ERR_INVALID, "psiAvg: nw >= 2 and the reference box [nw/2, 3nw/2]^3 inside the grid");
+  HIPCHK(hipSetDevice(c.device));
+  CHK(c2r_state(h, h->real_io));  // psi; D1 and D2 are free again
+  double* t1 = (double*)h->D1;
+  double* t2 = (double*)h->D2;
+  const int64_t N = h->N;
+  // trapz along x (innermost in the reference), then y, then z
+  hipLaunchKernelGGL(k_window_trapz<true>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->real_io, t1, N, c.nx,
+                     (int64_t)c.ny * c.nz, hw, h->dx);
+  hipLaunchKernelGGL(k_window_trapz<false>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, t1, t2, N, c.ny,
+                     (int64_t)c.nz, hw, h->dy);
+  hipLaunchKernelGGL(k_window_trapz<false>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, t2, t1, N, c.nz,
+                     (int64_t)1, hw, h->dz);
+  const double lam = 2.0 * M_PI / c.q0;
+  hipLaunchKernelGGL(k_psi_avg_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, t1, h->real_io, c.nx, c.ny, c.nz,
+                     hw, h->dx, h->dy, h->dz, lam * lam * lam);
+  HIPCHK(hipGetLastError());
+  return field_to_host(h, h->real_io, out, row_stride);
+}
+
 int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, double* vel, double* H, double* K,
                   int64_t* found) {
   if (!h || !opts || !surfz || !vel || !H || !K) return fail(SMX_ERR_INVALID, "null argument");

@@ -235,6 +235,47 @@ __global__ void k_fill_hash(T* __restrict__ a, int64_t n, T amp, uint32_t seed) 
   }
 }
 
+// One axis of the separable psiAvg integral (morphSurfAmp.cpp:1005-1134) on a dense FP64
+// [x][y][z] field: out = d (sum of f over the window [c - h, c + h] clipped to [0, n - 1]
+// less half its two end values) -- the reference's trapz (:76-86) along that axis, in
+// another order of the same terms; f = in^2 when SQUARE.  Axis of length n, stride s.
+template <bool SQUARE>
+__global__ void k_window_trapz(const double* __restrict__ in, double* __restrict__ out, int64_t N, int n, int64_t s,
+                               int h, double d) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)((p / s) % n);
+    const int a = c - h < 0 ? 0 : c - h, b = c + h > n - 1 ? n - 1 : c + h;
+    const double* f = in + (p - (int64_t)(c - a) * s);  // window start, inside the field
+    double t = 0.0;
+    for (int m = 0; m <= b - a; ++m) {
+      const double v = f[(int64_t)m * s];
+      t += SQUARE ? v * v : v;
+    }
+    double fa = f[0], fb = f[(int64_t)(b - a) * s];
+    if (SQUARE) {
+      fa *= fa;
+      fb *= fb;
+    }
+    out[p] = (b > a) ? d * (t - 0.5 * (fa + fb)) : 0.0;
+  }
+}
+
+// psiAvg = T / (ref l1 l2 l3 / lambda^3) (morphSurfAmp.cpp:1078-1080, 1118): ref is T at
+// (2h, 2h, 2h), the integral over the reference box [h, 3h]^3 (:1009-1049); l the clipped
+// window lengths.
+__global__ void k_psi_avg_norm(const double* __restrict__ T, double* __restrict__ out, int nx, int ny, int nz, int h,
+                               double dx, double dy, double dz, double lam3) {
+  const double ref = T[((int64_t)2 * h * ny + 2 * h) * nz + 2 * h];
+  const int64_t N = (int64_t)nx * ny * nz;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((int64_t)ny * nz));
+    const double l1 = (double)(min(i + h, nx - 1) - max(i - h, 0)) * dx;
+    const double l2 = (double)(min(j + h, ny - 1) - max(j - h, 0)) * dy;
+    const double l3 = (double)(min(k + h, nz - 1) - max(k - h, 0)) * dz;
+    out[p] = T[p] / (ref * l1 * l2 * l3 / lam3);
+  }
+}
+
 // a_T = psi_T - amp cos(q0 z) in place (FP64 real, dense [x][y][z]); per-block partials of
 // J = 1/2 sum a_T^2 and of the explicit target term sum a_T (-dg/dq0) = sum a_T amp z sin(q0 z).
 __global__ void k_adj_target(double* __restrict__ a, int64_t n, int nz, double dz, double q0, double amp,

@@ -8,14 +8,18 @@
 //           [--fp32] [--advect 0|1] [--force sh|mu] [--wmid W] [--ic noise|hyperboloid|pf3d|morph|dct|file:PATH]
 //           [--amp A] [--seed S] [--steps N] [--l1-limit L] [--l1-every K] [--save-every K]
 //           [--surf-every K] [--arm A] [--vsign S] [--out DIR] [--max-steps N] [--ic-only]
-//           [--resume CKPT] [--checkpoint CKPT]
+//           [--resume CKPT] [--checkpoint CKPT] [--psi-avg NW]
+// --psi-avg writes psiAvg.dat, the local amplitude field of the final state over windows
+// of NW points (cosFFT/comet/morphSurfAmp.cpp:1005-1134), in the psi dump format
 // --resume continues bit for bit from an exact binary checkpoint written by --checkpoint
 // (at the end of the run) instead of re-bootstrapping from a text dump.
 //
 // Files (one value per line, `os << v << "\n "`, default 6 significant digits, loop
 // order i, j, k as pf3d.cpp:346-352): psiIC.dat, L1.dat (appended every l1-every
 // steps, pf3d.cpp:510-513), psi.dat / psiB.dat (alternating on Neumann runs,
-// cosNoAdvMorph.cpp:578-589), surfPsi.dat velSurf.dat curvH.dat curvK.dat.
+// cosNoAdvMorph.cpp:578-589), surfPsi.dat velSurf.dat curvH.dat curvK.dat, and with them
+// psiMid.dat, the (x, z) slice at y = ny/2 appended at each surface output
+// (cosNoAdvMorphCurv.cpp:981-1001, which fixes j = 256, the middle of its 512-point runs).
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -43,6 +47,7 @@ struct Opts {
   long surf_every = 0;     // in L1 checks (0: off)
   long max_steps = 1000000;
   bool ic_only = false;  // write psiIC.dat and stop (no device needed)
+  int psi_avg_nw = 0;    // > 0: write psiAvg.dat of the final state
   std::string resume, checkpoint;  // exact binary checkpoint to continue from / to write at the end
   double arm = -1, vsign = 1.0;
   int nw = 16;
@@ -119,6 +124,15 @@ void write_field(const std::string& path, const std::vector<double>& psi) {
   std::ofstream os(path);
   if (!os) throw std::runtime_error("cannot write " + path);
   for (double v : psi) os << v << "\n ";  // pf3d.cpp:351
+}
+
+// psi[(i*ny + j)*nz + k] for every i, k at j = ny/2, one value a line (cosNoAdvMorphCurv.cpp:989-1001)
+void append_mid_slice(const std::string& path, const std::vector<double>& psi, int nx, int ny, int nz) {
+  std::ofstream os(path, std::ios_base::app);
+  if (!os) throw std::runtime_error("cannot write " + path);
+  const int j = ny / 2;
+  for (int i = 0; i < nx; ++i)
+    for (int k = 0; k < nz; ++k) os << psi[((size_t)i * ny + j) * nz + k] << "\n";
 }
 
 void append_line(const std::string& path, double v) {
@@ -210,6 +224,7 @@ int run(int argc, char** argv) {
     else if (k == "--ic-only") o.ic_only = true;
     else if (k == "--resume" && has) o.resume = argv[++a];
     else if (k == "--checkpoint" && has) o.checkpoint = argv[++a];
+    else if (k == "--psi-avg" && has) o.psi_avg_nw = (int)num(a);
     else throw std::runtime_error("unknown or incomplete option " + k);
   }
   if (nx > 0 || ny > 0 || nz > 0) {
@@ -231,6 +246,7 @@ int run(int argc, char** argv) {
   smx_handle* h = nullptr;
   if (smx_create(&o.c, &h) != SMX_OK) throw std::runtime_error(std::string("smx_create: ") + smx_last_error());
   std::ofstream(dir + "L1.dat").close();
+  if (o.surf_every > 0) std::ofstream(dir + "psiMid.dat").close();  // created empty, :482-486
   if (!o.resume.empty()) {
     if (smx_load_checkpoint(h, o.resume.c_str()) != SMX_OK) throw std::runtime_error(smx_last_error());
   } else if (smx_set_psi(h, psi.data(), o.c.nz) != SMX_OK) {
@@ -271,6 +287,8 @@ int run(int argc, char** argv) {
         fh << sh[q] << "\n";
         fk << sk[q] << "\n";
       }
+      if (smx_get_psi(h, psi.data(), o.c.nz) != SMX_OK) throw std::runtime_error(smx_last_error());
+      append_mid_slice(dir + "psiMid.dat", psi, o.c.nx, o.c.ny, o.c.nz);
     }
     if (o.steps < 0 && !(L1 > o.l1_limit)) break;  // while (L1 > limL1), pf3d.cpp:390
   }
@@ -280,6 +298,11 @@ int run(int argc, char** argv) {
     throw std::runtime_error(smx_last_error());
   smx_get_psi(h, psi.data(), o.c.nz);
   write_field(dir + "psi.dat", psi);
+  if (o.psi_avg_nw > 0) {
+    std::vector<double> avg(psi.size());
+    if (smx_psi_avg(h, o.psi_avg_nw, avg.data(), o.c.nz) != SMX_OK) throw std::runtime_error(smx_last_error());
+    write_field(dir + "psiAvg.dat", avg);  // morphSurfAmp.cpp:1120-1130
+  }
   double F = 0;
   smx_energy(h, &F);
   std::printf("steps %ld  L1 %.6e  energy %.12e  wall %.3f s  (%.1f steps/s)\n", done, L1, F, secs,

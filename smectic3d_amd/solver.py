@@ -93,6 +93,12 @@ class Solver:
         check(lib().smx_dtpsi(self._h, _ptr(out), self.cfg.nz))
         return out
 
+    def psi_avg(self, nw=16):
+        """Local amplitude field psiAvg (cosFFT/comet/morphSurfAmp.cpp:1005-1134)."""
+        out = np.empty(self.shape)
+        check(lib().smx_psi_avg(self._h, int(nw), _ptr(out), self.cfg.nz))
+        return out
+
     def surftrack(self, arm, vel_sign=1.0):
         nx, ny, _ = self.shape
         outs = [np.zeros(nx * ny) for _ in range(4)]

@@ -64,3 +64,33 @@ def test_driver_resume_from_checkpoint_is_exact(driver, tmp_path):
     subprocess.run([driver, *common, "--steps", "100", "--out", str(b), "--checkpoint", str(ck)], check=True)
     subprocess.run([driver, *common, "--steps", "100", "--out", str(b), "--resume", str(ck)], check=True)
     assert (a / "psi.dat").read_text() == (b / "psi.dat").read_text()
+
+
+@pytest.mark.gpu
+def test_driver_psimid_slice(driver, tmp_path):
+    """psiMid.dat gets the (x, z) slice at y = ny/2 at each surface output, empty file at start
+    (cosNoAdvMorphCurv.cpp:482-486, 981-1001); the last one equals that slice of the final psi.dat."""
+    nx, ny, nz = 16, 32, 24
+    subprocess.run([driver, "--preset", "plain", "--nx", str(nx), "--ny", str(ny), "--nz", str(nz), "--neumann",
+                    "--l1-every", "10", "--surf-every", "2", "--steps", "40", "--save-every", "1000",
+                    "--out", str(tmp_path)], check=True)
+    mid = np.array([float(v) for v in (tmp_path / "psiMid.dat").read_text().split()])
+    assert mid.size == 2 * nx * nz  # two surface outputs (after 20 and 40 steps)
+    psi = np.array([float(v) for v in (tmp_path / "psi.dat").read_text().split()]).reshape(nx, ny, nz)
+    np.testing.assert_allclose(mid[nx * nz:].reshape(nx, nz), psi[:, ny // 2, :], rtol=0, atol=0)
+
+
+@pytest.mark.gpu
+def test_driver_psi_avg_file(driver, tmp_path):
+    """--psi-avg writes psiAvg.dat of the final state in the dump format; its values are the
+    oracle's psiAvg of the printed psi.dat to the 6 printed digits."""
+    n, nw = 16, 4
+    subprocess.run([driver, "--preset", "plain", "--n", str(n), "--l1-every", "20", "--steps", "40",
+                    "--save-every", "1000", "--psi-avg", str(nw), "--out", str(tmp_path)], check=True)
+    txt = (tmp_path / "psiAvg.dat").read_text()
+    assert txt.endswith("\n ")
+    got = np.array([float(v) for v in txt.split()]).reshape(n, n, n)
+    psi = np.array([float(v) for v in (tmp_path / "psi.dat").read_text().split()]).reshape(n, n, n)
+    d = 2 * math.pi / 8  # plain preset: 8 points per wavelength, q0 = 1
+    want = O.psi_avg(psi, d, d, d, nw)
+    np.testing.assert_allclose(got, want, rtol=1e-4)
