@@ -127,6 +127,15 @@ int smx_dominant_wavenumber(smx_handle *h, double *q);
  * surfTrack.cpp:611,622. */
 int smx_dtpsi(smx_handle *h, double *dtpsi, int64_t row_stride);
 
+/* Eckhaus scan of energyPsi.m:68-234 on the current state psi0: energy[m] is the
+ * finite-difference + trapz energy (periodic second differences, trapezoid weights 1/2
+ * at both ends of every axis, energyPsi.m:217-234) of psi0 + amp[m] dpsiz
+ * cos((1 + q[m]) (k + 1) dz), with dpsiz the script's 4-point z stencil of psi0 exactly
+ * as written (:100-101) and k the 0-based z index (:118); amp[m] = 0 gives psi0's own
+ * energy, the scan's first entry.  All nq energies come from ceil(nq/4) FP64 passes over
+ * the GPU-resident field (the script re-evaluates a full energy per Q on the host). */
+int smx_eckhaus_scan(smx_handle *h, int nq, const double *q, const double *amp, double *energy);
+
 /* Local amplitude psiAvg of the current state (host, reference layout): the trapezoid
  * integral of psi^2 over the window of +-nw/2 points about each point (clipped to the
  * grid), divided by the same integral over the reference box [nw/2, 3nw/2]^3 times

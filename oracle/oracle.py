@@ -61,6 +61,8 @@ def lib():
         L.or_psi_avg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                  C.c_double, _dp, _dp]
         L.or_psi_avg.restype = C.c_int
+        L.or_eckhaus_scan.argtypes = [C.POINTER(SmxConfig), _dp, C.c_int, _dp, _dp, _dp]
+        L.or_eckhaus_scan.restype = C.c_int
         L.or_adjoint_grad.argtypes = [C.POINTER(SmxConfig), _dp, C.c_int64, C.c_double, _dp, _dp, _dp, _dp]
         L.or_adjoint_grad.restype = C.c_int
         _lib = L
@@ -144,6 +146,17 @@ def growth_sweep(cfg, mq0, dmq, nq):
     out = np.zeros(nq)
     rc = lib().or_growth_sweep(C.byref(cfg), mq0, dmq, nq, _p(out))
     assert rc == 0
+    return out
+
+
+def eckhaus_scan(cfg, psi0, q, amp):
+    """energyPsi.m:68-234: FD + trapz energies of psi0 + amp[m] dpsiz cos((1+q[m]) (k+1) dz)."""
+    psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    amp = np.ascontiguousarray(np.broadcast_to(amp, q.shape), dtype=np.float64)
+    out = np.zeros(len(q))
+    rc = lib().or_eckhaus_scan(C.byref(cfg), _p(psi0), len(q), _p(q), _p(amp), _p(out))
+    assert rc == 0, rc
     return out
 
 

@@ -633,6 +633,37 @@ double or_energy_fd_trapz(const or_state *s) {
   return tot * s->dx * s->dy * s->dz;
 }
 
+int or_eckhaus_scan(const smx_config *cfg, const double *psi0, int nq, const double *q, const double *amp,
+                    double *energy) {
+  or_state *s = or_create(cfg);
+  if (!s) return -1;
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  double *dpz = (double *)malloc(sizeof(double) * (size_t)s->N);
+  if (!dpz) {
+    or_destroy(s);
+    return -2;
+  }
+  /* energyPsi.m:68-101, periodic neighbours of :86-98 */
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const int kp = (k + 1) % nz, kpp = (k + 2) % nz, km = (k + nz - 1) % nz, kmm = (k + nz - 2) % nz;
+        const double *c = psi0 + ((int64_t)i * ny + j) * nz;
+        dpz[((int64_t)i * ny + j) * nz + k] =
+            -(1.0 / 12) * c[kpp] + (2.0 / 3) * c[kp] - (2.0 / 3) * c[kmm] + (1.0 / 12) * c[km];
+      }
+  for (int m = 0; m < nq; ++m) { /* :108-124 */
+    for (int64_t x = 0; x < s->N; ++x) {
+      const int k = (int)(x % nz);
+      s->psi[x] = psi0[x] + dpz[x] * amp[m] * cos((1.0 + q[m]) * (k + 1) * s->dz);
+    }
+    energy[m] = or_energy_fd_trapz(s); /* :131-234 */
+  }
+  free(dpz);
+  or_destroy(s);
+  return 0;
+}
+
 double or_dominant_wavenumber(const or_state *s) {
   double num = 0.0, den = 0.0;
   if (s->c.bc == SMX_BC_PERIODIC) {

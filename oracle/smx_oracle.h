@@ -59,6 +59,12 @@ void or_ic_hyperboloid(int nx, int ny, int nz, double dx, double dy, double dz, 
  * psi0 = 1e-4 cos(mq*k*dz), for nq values mq = mq0 + i*dmq. */
 int or_growth_sweep(const smx_config *cfg, double mq0, double dmq, int nq, double *growth);
 
+/* energyPsi.m:68-234 (Eckhaus scan): with dpsiz the script's 4-point z stencil of psi0
+ * (:100-101, coefficients as written), energy[m] = FD + trapz energy (or_energy_fd_trapz)
+ * of psi0 + amp[m] dpsiz cos((1 + q[m]) (k + 1) dz), k the 0-based z index (:118). */
+int or_eckhaus_scan(const smx_config *cfg, const double *psi0, int nq, const double *q, const double *amp,
+                    double *energy);
+
 /* morphSurfAmp.cpp:1005-1134: local amplitude psiAvg of a dense [x][y][z] field --
  * trapezoid integral of psi^2 over the window of +-nw/2 points about each point (clipped
  * to the grid), over the same integral on the reference box [nw/2, 3nw/2]^3 times

@@ -18,7 +18,7 @@ EXPORTS = [
     "smx_calibration_copy", "smx_adjoint_grad", "smx_batch_create", "smx_batch_destroy",
     "smx_batch_handle", "smx_batch_run", "smx_batch_graph_nodes", "smx_get_schedule", "smx_set_schedule",
     "smx_autotune", "smx_batch_autotune", "smx_set_adjoint_budget", "smx_adjoint_info",
-    "smx_save_checkpoint", "smx_load_checkpoint", "smx_psi_avg",
+    "smx_save_checkpoint", "smx_load_checkpoint", "smx_psi_avg", "smx_eckhaus_scan",
 ]
 
 _lib = None
@@ -55,6 +55,7 @@ def lib():
         getattr(L, n).argtypes = [vp, _dp]
     L.smx_surftrack.argtypes = [vp, C.POINTER(SmxSurfOpts), _dp, _dp, _dp, _dp, C.POINTER(C.c_int64)]
     L.smx_psi_avg.argtypes = [vp, C.c_int, _dp, C.c_int64]
+    L.smx_eckhaus_scan.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     L.smx_sync.argtypes = [vp]
     L.smx_step_count.argtypes = [vp, C.POINTER(C.c_int64)]
     L.smx_profile_step.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.c_char_p,

@@ -1345,6 +1345,32 @@ int smx_dtpsi(smx_handle* h, double* out, int64_t row_stride) {
   return field_to_host(h, h->real_io, out, row_stride);
 }
 
+int smx_eckhaus_scan(smx_handle* h, int nq, const double* q, const double* amp, double* energy) {
+  if (!h || (nq > 0 && (!q || !amp || !energy))) return fail(SMX_ERR_INVALID, "null argument");
+  if (nq < 0) return fail(SMX_ERR_INVALID, "nq < 0");
+  if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  const smx_config& c = h->cfg;
+  HIPCHK(hipSetDevice(c.device));
+  CHK(c2r_state(h, h->real_io));  // psi0
+  const double dV = h->dx * h->dy * h->dz;
+  for (int m0 = 0; m0 < nq; m0 += 4) {  // four energies per pass over psi0
+    const int nm = std::min(4, nq - m0);
+    double qq[4] = {0, 0, 0, 0}, aa[4] = {0, 0, 0, 0};
+    for (int m = 0; m < nm; ++m) {
+      qq[m] = q[m0 + m];
+      aa[m] = amp[m0 + m];
+    }
+    hipLaunchKernelGGL(k_eckhaus, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->real_io, c.nx, c.ny, c.nz, h->dx,
+                       h->dy, h->dz, c.eps, c.alpha, c.q0 * c.q0, c.beta, c.gamma, c.nl_model, nm, qq[0], qq[1],
+                       qq[2], qq[3], aa[0], aa[1], aa[2], aa[3], h->red);
+    HIPCHK(hipGetLastError());
+    std::vector<double> s;
+    CHK(host_sum(h, 4, kRedBlocks, s));
+    for (int m = 0; m < nm; ++m) energy[m0 + m] = s[m] * dV;
+  }
+  return SMX_OK;
+}
+
 int smx_psi_avg(smx_handle* h, int nw, double* out, int64_t row_stride) {
   if (!h || !out) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");

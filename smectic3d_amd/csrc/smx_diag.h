@@ -235,6 +235,61 @@ __global__ void k_fill_hash(T* __restrict__ a, int64_t n, T amp, uint32_t seed) 
   }
 }
 
+// Eckhaus scan energies (energyPsi.m:68-234) of up to four perturbed fields in one pass over
+// psi0 (dense FP64 [x][y][z], periodic wrap): psi_m = psi0 + amp_m dpsiz cos((1 + q_m)(k + 1) dz)
+// with dpsiz the script's 4-point z stencil as written (:100-101), energy density
+// -eps/2 psi^2 + alpha/2 (q0^2 psi + lap psi)^2 + f(psi) with second differences (:217-228),
+// trapz weights 1/2 at both ends of every axis (:234).  Per-block Kahan partials of the
+// weighted sums (times dx dy dz on the host), layout [block][4].
+__device__ __forceinline__ double eck_dpz(const double* col, int k, int nz) {
+  const int kp = k + 1 == nz ? 0 : k + 1, kpp = (k + 2) % nz, km = k == 0 ? nz - 1 : k - 1, kmm = (k + nz - 2) % nz;
+  return -(1.0 / 12) * col[kpp] + (2.0 / 3) * col[kp] - (2.0 / 3) * col[kmm] + (1.0 / 12) * col[km];
+}
+__global__ void k_eckhaus(const double* __restrict__ psi, int nx, int ny, int nz, double dx, double dy, double dz,
+                          double eps, double alpha, double q02, double beta, double gamma, int nl_model, int nm,
+                          double q0v, double q1v, double q2v, double q3v, double a0, double a1, double a2, double a3,
+                          double* __restrict__ out) {
+  __shared__ double red[64];
+  const double qv[4] = {q0v, q1v, q2v, q3v}, av[4] = {a0, a1, a2, a3};
+  const double idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy), idz2 = 1.0 / (dz * dz);
+  const int64_t N = (int64_t)nx * ny * nz;
+  Kahan acc[4];
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((int64_t)ny * nz));
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
+    const int km = k == 0 ? nz - 1 : k - 1, kp = k + 1 == nz ? 0 : k + 1;
+    const double* c0 = psi + ((int64_t)i * ny + j) * nz;
+    const double* cxm = psi + ((int64_t)im * ny + j) * nz;
+    const double* cxp = psi + ((int64_t)ip * ny + j) * nz;
+    const double* cym = psi + ((int64_t)i * ny + jm) * nz;
+    const double* cyp = psi + ((int64_t)i * ny + jp) * nz;
+    // psi0 and dpsiz at the 7-point stencil
+    const double p0 = c0[k], pkm = c0[km], pkp = c0[kp], pxm = cxm[k], pxp = cxp[k], pym = cym[k], pyp = cyp[k];
+    const double d0 = eck_dpz(c0, k, nz), dkm = eck_dpz(c0, km, nz), dkp = eck_dpz(c0, kp, nz);
+    const double dxm = eck_dpz(cxm, k, nz), dxp = eck_dpz(cxp, k, nz), dym = eck_dpz(cym, k, nz),
+                 dyp = eck_dpz(cyp, k, nz);
+    const double w = (i == 0 || i == nx - 1 ? 0.5 : 1.0) * (j == 0 || j == ny - 1 ? 0.5 : 1.0) *
+                     (k == 0 || k == nz - 1 ? 0.5 : 1.0);
+    for (int m = 0; m < nm; ++m) {
+      const double f = (1.0 + qv[m]) * dz;
+      const double ck = av[m] * cos(f * (k + 1)), ckm = av[m] * cos(f * (km + 1)), ckp = av[m] * cos(f * (kp + 1));
+      const double c = p0 + d0 * ck;
+      const double lap = idz2 * ((pkm + dkm * ckm) - 2 * c + (pkp + dkp * ckp)) +
+                         idy2 * ((pym + dym * ck) - 2 * c + (pyp + dyp * ck)) +
+                         idx2 * ((pxm + dxm * ck) - 2 * c + (pxp + dxp * ck));
+      const double t = q02 * c + lap, c2 = c * c;
+      const double fl = (nl_model == 1) ? (-beta / 3.0 * c2 * c + gamma / 4.0 * c2 * c2)
+                                        : (-beta / 4.0 * c2 * c2 + gamma / 6.0 * c2 * c2 * c2);
+      acc[m].add(w * (-0.5 * eps * c2 + 0.5 * alpha * t * t + fl));
+    }
+  }
+  for (int m = 0; m < 4; ++m) {
+    const double s = block_sum(acc[m].s, red);
+    if (threadIdx.x == 0) out[(int64_t)blockIdx.x * 4 + m] = s;
+  }
+}
+
 // One axis of the separable psiAvg integral (morphSurfAmp.cpp:1005-1134) on a dense FP64
 // [x][y][z] field: out = d (sum of f over the window [c - h, c + h] clipped to [0, n - 1]
 // less half its two end values) -- the reference's trapz (:76-86) along that axis, in

@@ -93,6 +93,15 @@ class Solver:
         check(lib().smx_dtpsi(self._h, _ptr(out), self.cfg.nz))
         return out
 
+    def eckhaus_scan(self, q, amp):
+        """Energies of psi0 + amp[m] dpsiz cos((1+q[m])(k+1)dz) for the current state psi0
+        (energyPsi.m:68-234); amp may be a scalar."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        amp = np.ascontiguousarray(np.broadcast_to(amp, q.shape), dtype=np.float64)
+        out = np.zeros(len(q))
+        check(lib().smx_eckhaus_scan(self._h, len(q), _ptr(q), _ptr(amp), _ptr(out)))
+        return out
+
     def psi_avg(self, nw=16):
         """Local amplitude field psiAvg (cosFFT/comet/morphSurfAmp.cpp:1005-1134)."""
         out = np.empty(self.shape)

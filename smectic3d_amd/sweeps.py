@@ -4,6 +4,10 @@
 the reference runs one MPI job per wavenumber; here every wavenumber is a box of the same
 grid, all boxes advance together through one captured hipGraph, and the growth rate is
 read at the box centre exactly as the reference reads it (:316-320).
+
+`eckhaus_scan` is the energyPsi.m scan (:108-234): the energies of a state perturbed by
+0.01 dpsiz cos((1 + Q) z) for Q = -dq, 0, dq, ... <= qmax (the first entry unperturbed),
+all evaluated on the GPU-resident field, four per pass.
 """
 import copy
 
@@ -33,3 +37,19 @@ def growth_sweep(cfg, mq0, dmq, nq, amp=1e-4):
         b.run(1, graph_steps=1)
         p3 = np.array([box.get_psi()[centre] for box in b.boxes])
     return (p3 - p2) / (cfg.dt * p2)
+
+
+def eckhaus_scan_q(lz, nz, qmax=1.4):
+    """The scan's Q list and amplitudes, energyPsi.m:25,108-124 (dq = 2 pi/L; Q = -dq:dq:qmax,
+    the first entry unperturbed, the others at amplitude 0.01)."""
+    dq = 2 * np.pi / lz
+    q = -dq + dq * np.arange(int(np.floor((qmax + dq) / dq + 1e-9)) + 1)
+    amp = np.full(len(q), 0.01)
+    amp[0] = 0.0
+    return q, amp
+
+
+def eckhaus_scan(solver, qmax=1.4):
+    """(Q, energies) of the energyPsi.m scan about the solver's current state."""
+    q, amp = eckhaus_scan_q(solver.cfg.lz, solver.cfg.nz, qmax)
+    return q, solver.eckhaus_scan(q, amp)
