@@ -82,6 +82,47 @@ def test_adjoint_central_differences_64():
     assert abs(np.sum(d * v) - fv) < 1e-6 * max(1, abs(fv))
 
 
+def test_adjoint_central_differences_256_2000():
+    """The single large case: 256^3, 2000 steps, checkpoints every 50 (L = 32 wavelengths,
+    eps = 0.3, q0 = 1, gamma = 1, dt = 0.05, psi0 noise 0.05, loss 1/2 |psi_T - 0.5 cos(q0 z)|^2),
+    FP64, against central differences of the forward solve in eps, q0 and a random psi0
+    direction, Richardson-extrapolated from steps h and 2h (the 2000-step map amplifies
+    perturbations, so the h^2 term of a plain central difference is visible at 1e-6)."""
+    n, T, K = 256, 2000, 50
+    say("adjoint FD 256^3 x 2000")
+    cfg = A.make_config(n, lx=n / 8 * 2 * math.pi, ly=n / 8 * 2 * math.pi, lz=n / 8 * 2 * math.pi, eps=0.3,
+                        q0=1.0, beta=0.0, gamma=1.0, dt=0.05, nl_model=A.SMX_NL_CUBIC)
+    psi = np.random.default_rng(9).uniform(-0.05, 0.05, (n, n, n))
+    with Solver(cfg) as s:
+        J, d, de, dq = s.adjoint_grad(psi, T, K)
+        res, rec, _ = s.adjoint_info()
+    say("  J", J, "resident", res, "recomputed", rec)
+
+    def with_(**kw):
+        c = copy.copy(cfg)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+    h = 1e-6
+    v = np.random.default_rng(10).uniform(-1, 1, (n, n, n))
+
+    def central(f, step):
+        return (f(step) - f(-step)) / (2 * step)
+
+    def richardson(f):
+        d1, d2 = central(f, h), central(f, 2 * h)
+        return (4 * d1 - d2) / 3, d1
+
+    fe, fe1 = richardson(lambda t: gpu_loss(with_(eps=cfg.eps + t), psi, T))
+    fq, fq1 = richardson(lambda t: gpu_loss(with_(q0=cfg.q0 + t), psi, T))
+    fv, fv1 = richardson(lambda t: gpu_loss(cfg, psi + t * v, T))
+    dv = float(np.sum(d * v))
+    say("  deps", de, fe, fe1, "dq0", dq, fq, fq1, "dpsi.v", dv, fv, fv1)
+    assert abs(de - fe) < 1e-6 * abs(fe)
+    assert abs(dq - fq) < 1e-6 * abs(fq)
+    assert abs(dv - fv) < 1e-6 * abs(fv)
+
+
 def test_adjoint_fp32_mixed_close_to_fp64():
     shape, T = (32, 32, 32), 40
     psi = O.ic_noise(*shape, 3, 0.1)
