@@ -304,18 +304,25 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
   if constexpr (IN == Z_IN_SPEC) {
     // Hermitian extension of the column pair read straight from HBM; only the real
     // parts of kz = 0 and kz = nz/2 enter (the half-complex stage of an FFTW c2r).
+    // Element e sits at n = tc + c_e (c_e a compile-time constant once unrolled) and reads row
+    // m = min(n, L - n): its byte offset is tc*rowb + c_e*rowb or (L - c_e)*rowb - tc*rowb, one
+    // add on a uniform product, decided at compile time except for the one element that can
+    // straddle L/2.
     const cplx2<T>* col = reinterpret_cast<const cplx2<T>*>(in + f0);
+    const uint32_t tcr = (uint32_t)tc * rowb + qofs;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(tc, e);
-      const int m = (n <= L / 2) ? n : L - n;
-      const cplx2<T> ab = ld_off(col, (uint32_t)m * rowb + qofs);
+      const int c = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(0, e);
+      const int n = tc + c;
+      const bool lo = (c + TPC - 1 <= L / 2) || (c <= L / 2 && n <= L / 2);  // n <= L/2
+      const uint32_t off = lo ? tcr + (uint32_t)c * rowb : (uint32_t)(L - c) * rowb - tcr + 2 * qofs;
+      const cplx2<T> ab = ld_off(col, off);
       cplx<T> A = ab.a, B = ab.b;
       if (n == 0 || n == L / 2) {
         A.y = 0;
         B.y = 0;
       }
-      v[e] = (n <= L / 2) ? mk(A.x - B.y, A.y + B.x) : mk(A.x + B.y, B.x - A.y);
+      v[e] = lo ? mk(A.x - B.y, A.y + B.x) : mk(A.x + B.y, B.x - A.y);
     }
     InvFFT<T, L, FP, V>::run(v, F, q, tc, twl);
   } else {
@@ -392,15 +399,19 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L, V>()>(v, F, q, tc);
     __syncthreads();
-    // split W = X + iY:  X = (W_k + conj W_{-k})/2,  Y = (W_k - conj W_{-k})/(2i); straight to HBM
+    // split W = X + iY:  X = (W_k + conj W_{-k})/2,  Y = (W_k - conj W_{-k})/(2i); straight to HBM.
+    // Thread (q, tc) takes rows k = tc, tc + TPC, ... of its own column pair q.
     cplx2<T>* dst = reinterpret_cast<cplx2<T>*>(out + f0);
-    for (int i = tid; i < NZH * NPR; i += NT) {
-      const int k = i / NPR, qq = i - k * NPR;
-      const cplx<T> W = F[k * FP + qq], Wm = F[((L - k) % L) * FP + qq];
-      cplx2<T> xy;
-      xy.a = mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y));
-      xy.b = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
-      st_off(dst, (uint32_t)k * rowb + (uint32_t)qq * 2 * CB, xy);
+#pragma unroll
+    for (int it = 0; it < (NZH + TPC - 1) / TPC; ++it) {
+      const int k = tc + it * TPC;
+      if (k < NZH) {
+        const cplx<T> W = F[k * FP + q], Wm = F[(k == 0 ? 0 : L - k) * FP + q];
+        cplx2<T> xy;
+        xy.a = mk((T)0.5 * (W.x + Wm.x), (T)0.5 * (W.y - Wm.y));
+        xy.b = mk((T)0.5 * (W.y + Wm.y), (T)0.5 * (Wm.x - W.x));
+        st_off(dst, (uint32_t)tc * rowb + (uint32_t)(it * TPC) * rowb + qofs, xy);
+      }
     }
     if (p.partial != nullptr) store_partial<NT>(red, p.partial + blockIdx.x, tid);
   }
