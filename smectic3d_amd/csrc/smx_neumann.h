@@ -56,6 +56,10 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
   const int tid = threadIdx.x, tc = tid % TPC, q = tid / TPC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, tw, tid);
+  // LDS copy of the quarter-wave twiddles for the DCT-II outputs, read after the transform's
+  // barriers (the DCT-III inputs read qtw before any barrier, from the global table)
+  __shared__ cplx<T> QW[DIR < 0 ? L : 1];
+  if constexpr (DIR < 0) tw_lds<L, NT>(QW, qtw, tid);
   const int64_t pa = (int64_t)blockIdx.x * NC + 2 * q;  // pencils pa (column a), pa + 1 (column b)
   const T* a = in + pa * L;
   const T* b = a + L;
@@ -73,7 +77,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
     T* oa = out + (int64_t)blockIdx.x * NC * L;
     for (int i = tid; i < NPR * L; i += NT) {
       const int qq = i / L, k = i - qq * L;
-      const cplx<T> y = dct2_out(S[AD::at(k, qq)], S[AD::at((L - k) % L, qq)], qtw[k]);
+      const cplx<T> y = dct2_out(S[AD::at(k, qq)], S[AD::at((L - k) % L, qq)], QW[k]);
       oa[(2 * qq) * L + k] = y.x;
       oa[(2 * qq + 1) * L + k] = y.y;
     }
@@ -108,6 +112,11 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
+  // LDS copies of qtw and k_x for the item loop (read after the forward transform's barriers)
+  __shared__ cplx<T> QW[L];
+  __shared__ T KX[L];
+  tw_lds<L, NT>(QW, qtw, tid);
+  for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy + y0;
   const uint32_t rowb = (uint32_t)p.ny * sizeof(T);
@@ -143,14 +152,33 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
     cplx<T>* rb = rbuf ? reinterpret_cast<cplx<T>*>(rbuf + plane) : nullptr;
     const cplx<T>* p0 = psi0 ? reinterpret_cast<const cplx<T>*>(psi0 + plane) : nullptr;
     const T kzv = p.kz[kz];
-    for (int i = tid; i < (L / 2 + 1) * NPR; i += NT) {
-      const int k = i / NPR, qq = i - k * NPR;
+    // Thread (q, tc) takes the items k = tc + it*TPC (k <= L/2) of its own pair q.  All of
+    // their R (X_STEP) or psi0 (X_BOOT) entries are read before the first store: a wave's
+    // loads and stores retire in order, so a read behind a store would wait for it.
+    constexpr int NIT = (L / 2 + TPC) / TPC;
+    const int qq = q;
+    const T kya = p.ky[y0 + 2 * qq], kyb = p.ky[y0 + 2 * qq + 1];
+    cplx<T> cin[NIT][2];
+    if constexpr (MODE != X_FWD_SCALED) {
+      const cplx<T>* cb = (MODE == X_STEP) ? (const cplx<T>*)rb : p0;
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int k = tc + it * TPC, km = (L - k) % L;
+        const bool in = k <= L / 2, two = in && k != 0 && 2 * k != L;
+        cin[it][0] = in ? ld_off(cb, (uint32_t)k * rowb + qq * CB) : mk<T>(0, 0);
+        cin[it][1] = two ? ld_off(cb, (uint32_t)km * rowb + qq * CB) : mk<T>(0, 0);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int k = tc + it * TPC;
+      if (k > L / 2) continue;
       const int km = (L - k) % L;
       const bool two = (k != 0 && 2 * k != L);
       const cplx<T> Z = F[k * FP + qq], Zm = F[km * FP + qq];
       cplx<T> y[2];
-      y[0] = dct2_out(Z, Zm, qtw[k]);
-      y[1] = two ? dct2_out(Zm, Z, qtw[km]) : mk<T>(0, 0);
+      y[0] = dct2_out(Z, Zm, QW[k]);
+      y[1] = two ? dct2_out(Zm, Z, QW[km]) : mk<T>(0, 0);
       const int kk[2] = {k, km};
       cplx<T> ps[2];
 #pragma unroll
@@ -168,8 +196,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
           }
           st_off(dst, o, r);
         } else {
-          const T kxv = p.kx[kk[j]];
-          const T kya = p.ky[y0 + 2 * qq], kyb = p.ky[y0 + 2 * qq + 1];
+          const T kxv = KX[kk[j]];
           const T ka = kxv * kxv + kya * kya + kzv * kzv, kb = kxv * kxv + kyb * kyb + kzv * kzv;
           const T da = ka - p.q02, db = kb - p.q02;
           const T La = p.eps - p.alpha * da * da, Lb = p.eps - p.alpha * db * db;  // cosNoAdvMorph.cpp:431-433
@@ -178,7 +205,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
           const cplx<T> nn = p.scale * y[j];
           cplx<T> s, rn;
           if constexpr (MODE == X_STEP) {  // :524-525 with R = C1 psi - dt/2 N_past
-            const cplx<T> r = ld_off<cplx<T>>(rb, o);
+            const cplx<T> r = cin[it][j];
             s = mk((r.x + (T)3 * p.dtd2 * nn.x) * recip(C2a), (r.y + (T)3 * p.dtd2 * nn.y) * recip(C2b));
             rn = mk(C1a * s.x - p.dtd2 * nn.x, C1b * s.y - p.dtd2 * nn.y);
             if (owner0 && kk[j] == 0 && qq == 0) {
@@ -190,7 +217,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
               rn.x = (T)p.mean_state[0];
             }
           } else {  // X_BOOT: N_past = N(psi0), cosNoAdvMorph.cpp:436-442,490
-            s = ld_off(p0, o);
+            s = cin[it][j];
             rn = mk(C1a * s.x - p.dtd2 * nn.x, C1b * s.y - p.dtd2 * nn.y);
             if (owner0 && kk[j] == 0 && qq == 0) {
               const double ps0 = p.mean_state[1];
@@ -206,8 +233,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
       if constexpr (MODE != X_FWD_SCALED) {
         // DCT-III inputs (the partner of k = 0 is X_L = Y_L = 0; k = L/2 is its own partner)
         const cplx<T> pk = ps[0], pm = two ? ps[1] : (k == 0 ? mk<T>(0, 0) : ps[0]);
-        F[k * FP + qq] = dct3_in(pk, pm, qtw[k]);
-        if (two) F[km * FP + qq] = dct3_in(ps[1], pk, qtw[km]);
+        F[k * FP + qq] = dct3_in(pk, pm, QW[k]);
+        if (two) F[km * FP + qq] = dct3_in(ps[1], pk, QW[km]);
       }
     }
     if constexpr (MODE == X_FWD_SCALED) return;
@@ -238,6 +265,10 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twz, tid);
+  // LDS copy of qtw for the DCT-II outputs of the split (read after barriers): a global
+  // load there would wait behind the previous row's store in the wave's one counter
+  __shared__ cplx<T> QW[(OUT & Z_OUT_SPEC) ? L : 1];
+  if constexpr ((OUT & Z_OUT_SPEC) != 0) tw_lds<L, NT>(QW, qtw, tid);
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
   static_assert(fwd_first<L>() == inv_last<L>(), "plan symmetry");
@@ -289,7 +320,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
     cplx<T>* dst = reinterpret_cast<cplx<T>*>(out + f0);
     for (int i = tid; i < L * NPR; i += NT) {
       const int k = i / NPR, qq = i - k * NPR;
-      st_off(dst, (uint32_t)k * rowb + qq * CB, dct2_out(F[k * FP + qq], F[((L - k) % L) * FP + qq], qtw[k]));
+      st_off(dst, (uint32_t)k * rowb + qq * CB, dct2_out(F[k * FP + qq], F[((L - k) % L) * FP + qq], QW[k]));
     }
     if (p.partial != nullptr) store_partial<NT>(red, p.partial + blockIdx.x, tid);
   }
