@@ -49,6 +49,8 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
+  __shared__ T KX[L];  // k_x for the loops after the forward transform's barriers
+  for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;
@@ -79,12 +81,14 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
   }
   FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
   const T kyv = p.ky[ky], kzv = p.kz[kz];
-  if (herm) __syncthreads();  // S receives R' below
+  // The new R goes to the LDS tile first and to HBM after the loop, so that the loop's R
+  // reads are not issued behind R writes (a wave's loads and stores retire in order).
+  __syncthreads();  // S receives R' below; the transform's last reads must be done
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
     const uint32_t o = (uint32_t)kxi * rowb + cofs;
-    const T kxv = p.kx[kxi];
+    const T kxv = KX[kxi];
     const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
     const T aLin = p.eps - op_sh(p, k2);
     const T C1 = (T)1 + p.dtd2 * aLin, C2 = (T)1 - p.dtd2 * aLin;
@@ -113,8 +117,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
         rn = mk((T)p.mean_state[0], (T)0);
       }
     }
-    if (herm) S[kxi * NCP + col] = rn;
-    else st_off(rb, o, rn);
+    S[kxi * NCP + col] = rn;
     ps[e] = s;
   }
   if (herm) {  // R(k) <- (R(k) + conj R(-k))/2 with -k = (nx - kx, partner column)
@@ -125,6 +128,12 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
       const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
       st_off(rb, (uint32_t)kxi * rowb + cofs, mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)));
     }
+  } else {  // each thread reads back its own entries
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
+      st_off(rb, (uint32_t)kxi * rowb + cofs, S[kxi * NCP + col]);
+    }
   }
   cplx<T>* outs[3] = {outA + plane, outX + plane, outS + plane};
 #pragma unroll 1
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adv(const cpl
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
-      const T kxv = p.kx[kxi];
+      const T kxv = KX[kxi];
       if (o == 0) v[e] = ps[e];
       else if (o == 1) v[e] = mk(-kxv * ps[e].y, kxv * ps[e].x);  // i kx psi_hat (pf3d.cpp:404)
       else v[e] = op_sh(p, kxv * kxv + kyv * kyv + kzv * kzv) * ps[e];  // opSH psi_hat (pf3d.cpp:407)
@@ -210,16 +219,33 @@ struct ZPair {
   static constexpr int NZH = L / 2 + 1, FP = NPR + 1;
   static constexpr uint32_t CB = sizeof(cplx<T>);
 
-  // C2R of one z-stage field straight from HBM: v <- (psi_a, psi_b) at pos_in<fwd_first>.
-  // kzmul: multiply by i kz first (spectral d/dz, Nyquist +pi/dz, pf3d.cpp:184).
-  __device__ static void c2r(const cplx<T>* in, int64_t f0, uint32_t rowb, int q, int tc, const T* kzmul,
-                             cplx<T>* F, const cplx<T>* tw, cplx<T>* v) {
+  // The rows one thread reads of a z-stage field (its column pair, the Hermitian extension's
+  // rows min(n, L - n)).
+  __device__ static void load(const cplx<T>* in, int64_t f0, uint32_t rowb, int q, int tc, cplx2<T>* raw) {
     const cplx2<T>* col = reinterpret_cast<const cplx2<T>*>(in + f0);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(tc, e);
       const int m = (n <= L / 2) ? n : L - n;
-      const cplx2<T> ab = ld_off(col, (uint32_t)m * rowb + (uint32_t)q * 2 * CB);
+      raw[e] = ld_off(col, (uint32_t)m * rowb + (uint32_t)q * 2 * CB);
+    }
+  }
+  // C2R of one z-stage field straight from HBM: v <- (psi_a, psi_b) at pos_in<fwd_first>.
+  // kzmul: multiply by i kz first (spectral d/dz, Nyquist +pi/dz, pf3d.cpp:184).
+  __device__ static void c2r(const cplx<T>* in, int64_t f0, uint32_t rowb, int q, int tc, const T* kzmul,
+                             cplx<T>* F, const cplx<T>* tw, cplx<T>* v) {
+    cplx2<T> raw[E];
+    load(in, f0, rowb, q, tc, raw);
+    c2r_loaded(raw, tc, kzmul, F, tw, q, v);
+  }
+  // C2R from rows already in registers (load()).
+  __device__ static void c2r_loaded(const cplx2<T>* raw, int tc, const T* kzmul, cplx<T>* F, const cplx<T>* tw, int q,
+                                    cplx<T>* v) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int n = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(tc, e);
+      const int m = (n <= L / 2) ? n : L - n;
+      const cplx2<T> ab = raw[e];
       cplx<T> A = ab.a, B = ab.b;
       if (kzmul != nullptr) {
         const T k = kzmul[m];
@@ -270,6 +296,8 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_force
   const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, Z::NT>(TW, p.twz, tid);
+  __shared__ T KZ[L];  // k_z, first read after several transforms' barriers
+  for (int i = tid; i < L; i += Z::NT) KZ[i] = p.kz[i];
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * Z::CB;
   cplx<T> v[E], f[E];
@@ -282,9 +310,14 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_force
   }
   const cplx<T>* src[3] = {bx, by, bpsi};
   cplx<T>* dst[3] = {o0, o1, o2};
+  // the next component's rows are read before this component's stores are issued (a
+  // wave's loads and stores retire in order: a read behind a store would wait for it)
+  cplx2<T> raw[E];
+  Z::load(src[0], f0, rowb, q, tc, raw);
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    Z::c2r(src[a], f0, rowb, q, tc, a == 2 ? p.kz : nullptr, F, twl, v);
+    Z::c2r_loaded(raw, tc, a == 2 ? KZ : nullptr, F, twl, q, v);
+    if (a < 2) Z::load(src[a + 1], f0, rowb, q, tc, raw);
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = mk(f[e].x * v[e].x, f[e].y * v[e].y);
     Z::r2c_store(v, dst[a], f0, rowb, q, tc, tid, F, twl);
@@ -308,6 +341,8 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_advnl
   const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, Z::NT>(TW, p.twz, tid);
+  __shared__ T KZ[L];  // k_z, first read after several transforms' barriers
+  for (int i = tid; i < L; i += Z::NT) KZ[i] = p.kz[i];
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * Z::CB;
   constexpr int RP = fwd_first<L, V>();
@@ -335,7 +370,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_advnl
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
     Z::c2r(us[a], f0, rowb, q, tc, nullptr, F, twl, u);
-    Z::c2r(gs[a], f0, rowb, q, tc, a == 2 ? p.kz : nullptr, F, twl, v);
+    Z::c2r(gs[a], f0, rowb, q, tc, a == 2 ? KZ : nullptr, F, twl, v);
 #pragma unroll
     for (int e = 0; e < E; ++e) nacc[e] = mk(nacc[e].x - u[e].x * v[e].x, nacc[e].y - u[e].y * v[e].y);
   }
