@@ -36,6 +36,8 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
+  __shared__ T KX[L];  // k_x for the loop after the forward transform's barriers
+  for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;
@@ -64,12 +66,14 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
   const double wk = herm ? 1.0 : 2.0;
   const bool sums = psin != nullptr && acc != nullptr;
   double ge = 0.0, gq = 0.0;
-  if (herm) __syncthreads();  // S receives alpha_n below
+  // alpha_n goes to the LDS tile first and to HBM after the loop, so that the loop's reads
+  // (carry, Psi_n) are not issued behind writes (a wave's loads and stores retire in order)
+  __syncthreads();  // S receives alpha_n below; the transform's last reads must be done
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
     const uint32_t o = (uint32_t)kxi * rowb + cofs;
-    const T kxv = p.kx[kxi];
+    const T kxv = KX[kxi];
     const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
     const T d = k2 - p.q02;
     const T aLin = p.eps - p.alpha * d * d;
@@ -86,8 +90,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
       ge += (double)base;
       gq += (double)(base * (k2 - p.q02));
     }
-    if (herm) S[kxi * NCP + col] = a0;
-    else st_off(cb, o, a0);
+    S[kxi * NCP + col] = a0;
     const T m = p.dtd2 * ic2;
     v[e] = mk(m * (cnext * a0.x - a1.x), m * (cnext * a0.y - a1.y));
   }
@@ -99,19 +102,27 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
       const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
       st_off(cb, (uint32_t)kxi * rowb + cofs, mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)));
     }
-  }
-  if (sums) {  // plane weight (1 on the self-conjugate planes, 2 elsewhere) and dL/dq0 = 4 alpha q0 (k^2 - q0^2)
-    const double be = block_sum(ge, red), bq = block_sum(gq, red);
-    if (tid == 0) {
-      const int b = blockIdx.y * gridDim.x + blockIdx.x;
-      acc[2 * b] += wk * be;
-      acc[2 * b + 1] += wk * 4.0 * alphad * q0d * bq;
+  } else {  // each thread reads back its own entries
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
+      st_off(cb, (uint32_t)kxi * rowb + cofs, S[kxi * NCP + col]);
     }
+  }
+  double be = 0.0, bq = 0.0;
+  if (sums) {  // plane weight (1 on the self-conjugate planes, 2 elsewhere) and dL/dq0 = 4 alpha q0 (k^2 - q0^2)
+    be = block_sum(ge, red);
+    bq = block_sum(gq, red);
   }
   InvFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
 #pragma unroll
   for (int e = 0; e < E; ++e)
     st_off(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
+  if (sums && tid == 0) {  // the block's accumulators, read and written last
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    acc[2 * b] += wk * be;
+    acc[2 * b + 1] += wk * 4.0 * alphad * q0d * bq;
+  }
 }
 
 template <typename T>
