@@ -206,10 +206,11 @@ int smx_batch_graph_nodes(smx_batch *b, int64_t *nodes);
  * Tile schedules of the four kernels of the periodic step, in launch order (slot 0: inverse
  * y pass, 1: z pass with N(psi), 2: forward y pass, 3: x pass with the CNAB2 update): the
  * columns per tile (8, 16 or 32) and the radix-plan variant (0: default; 1: the alternative
- * plan of the power-of-two lengths).  Slot 3 also takes variants 2 to 5: plans 0 and 1
- * with the x pass's R stores issued after its inverse transform (2, 3), or with all of R
- * read before any of it is rewritten (4, 5), instead of R read and rewritten element by
- * element (bit-identical results).  The step's result depends on the schedule only
+ * plan of the power-of-two lengths).  Slot 3 also takes variants 2 to 7: plans 0 and 1
+ * with the x pass's R stores issued after its inverse transform (2, 3), with all of R
+ * read before any of it is rewritten (4, 5), or with the new R staged in LDS and written
+ * after the update loop (6, 7), instead of R read and rewritten element by element
+ * (bit-identical results).  The step's result depends on the schedule only
  * through rounding.  The default is nc = 16, variant = 0 in every slot.  The advected step
  * uses the schedules of slots 0 and 2 for its eleven y passes.
  */

@@ -937,7 +937,7 @@ static int autotune_T(smx_handle* h, int reps_in, PassSched* best) {
     double tbest = 1e300;
     PassSched pick = cur[k];
     const int* vs = (k == SLOT_X) ? kSchedVX : kSchedV;
-    const int nv = (k == SLOT_X) ? 6 : 2;
+    const int nv = (k == SLOT_X) ? 8 : 2;
     for (int iv = 0; iv < nv && rc == SMX_OK; ++iv)
       for (int nc : kSchedNC) {
         const PassSched c{nc, vs[iv]};

@@ -51,7 +51,7 @@ struct PassSched {
 enum SchedSlot { SLOT_YINV = 0, SLOT_Z = 1, SLOT_YFWD = 2, SLOT_X = 3, NSLOTS = 4 };
 constexpr int kSchedNC[3] = {8, 16, 32};
 constexpr int kSchedV[2] = {0, 1};
-constexpr int kSchedVX[6] = {0, 1, 2, 3, 4, 5};  // the x pass adds the carry order (v >> 1)
+constexpr int kSchedVX[8] = {0, 1, 2, 3, 4, 5, 6, 7};  // the x pass adds the carry order (v >> 1)
 constexpr size_t kMaxLds = 160 * 1024 - 1024;  // LDS of one workgroup on gfx950, less the reduction scratch
 // (each tile check adds the L-entry twiddle copy every pass kernel keeps in LDS)
 
@@ -144,8 +144,8 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
     return hipErrorInvalidValue;
   }
   if (p.ny % s.nc != 0) return hipErrorInvalidValue;  // whole tiles along y
-  // variant bit 0: radix plan; v >> 1: carry order (k_xpass ORD 0, 1, 2)
-  if (s.v < 0 || s.v > 5) return hipErrorInvalidValue;
+  // variant bit 0: radix plan; v >> 1: carry order (k_xpass ORD 0..3)
+  if (s.v < 0 || s.v > 7) return hipErrorInvalidValue;
   const int ord = s.v >> 1;
   const PassSched sp{s.nc, s.v & 1};
 #define SMX_CASE(L)                                                                                      \
@@ -160,7 +160,8 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
         const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh);                                         \
         if (ord == 0) SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V, 0>), grid, dim3(NT), 0, st, in, out, rbuf, psi0, p); \
         else if (ord == 1) SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V, 1>), grid, dim3(NT), 0, st, in, out, rbuf, psi0, p); \
-        else SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V, 2>), grid, dim3(NT), 0, st, in, out, rbuf, psi0, p);          \
+        else if (ord == 2) SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V, 2>), grid, dim3(NT), 0, st, in, out, rbuf, psi0, p); \
+        else SMX_LAUNCH((k_xpass<T, L, NC, X_STEP, V, 3>), grid, dim3(NT), 0, st, in, out, rbuf, psi0, p);          \
         return hipGetLastError();                                                                        \
       }                                                                                                  \
     });

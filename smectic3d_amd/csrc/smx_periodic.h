@@ -118,10 +118,12 @@ enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
 // ORD (X_STEP only) orders the carry traffic.  0: R is read and rewritten element by
 // element between the two transforms.  1: all of R (and k_x) is read first, the new R is
 // kept in registers through the inverse transform and every global store is issued last.
-// 2: all of R and k_x is read first and each new R is stored as soon as it is formed.  A
-// wave's loads and stores retire through one in-order counter, so a load issued behind a
-// store waits for that store; which order hides more latency depends on the precision
-// and tile shape, and the autotuner measures all three.
+// 2: all of R is read first and each new R is stored as soon as it is formed.  3: the new R
+// goes to the LDS tile (as on the self-conjugate planes) and to HBM after the loop, so the
+// loop issues no stores and keeps no extra registers.  A wave's loads and stores retire
+// through one in-order counter, so a load issued behind a store waits for that store;
+// which order hides more latency depends on the precision and tile shape, and the
+// autotuner measures all four.  k_x comes from an LDS copy in every order.
 template <typename T, int L, int NC, int MODE, int V = 0, int ORD = 0>
 __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
                                                                cplx<T>* __restrict__ rbuf,
@@ -135,6 +137,9 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
+  __shared__ T KX[(MODE == X_STEP || MODE == X_BOOT) ? L : 1];  // read after the transform's barriers
+  if constexpr (MODE == X_STEP || MODE == X_BOOT)
+    for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;  // bytes between kx rows
@@ -177,20 +182,19 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
     for (int i = tid; i < p.npartial; i += NT) a += p.partial[i];
     nsum = block_sum(a, red);  // valid in thread 0 (= col 0, tc 0)
   }
-  cplx<T> cr[ORD >= 1 ? E : 1];  // ORD 1, 2: R (ORD 1: then the new R)
-  T kxr[ORD >= 1 ? E : 1];
+  cplx<T> cr[(ORD == 1 || ORD == 2) ? E : 1];  // ORD 1, 2: R (ORD 1: then the new R)
   if constexpr (MODE != X_INV) {
     FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
     const T kyv = p.ky[ky], kzv = p.kz[kz];
-    if constexpr (ORD >= 1) {
+    if constexpr (ORD == 1 || ORD == 2) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
         cr[e] = ld_off(rb, (uint32_t)kxi * rowb + cofs);
-        kxr[e] = p.kx[kxi];
       }
     }
-    if (herm) __syncthreads();  // S receives R' below; the FFT's last reads must be done
+    constexpr bool kStage = (ORD == 3);  // every plane's R' through the LDS tile
+    if (herm || kStage) __syncthreads();  // S receives R' below; the FFT's last reads must be done
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
           v[e] = mk((T)p.mean_state[1], (T)0);
         }
       } else {
-        const T kxv = (ORD >= 1) ? kxr[e] : p.kx[kxi];
+        const T kxv = KX[kxi];
         const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
         const T d = k2 - p.q02;
         const T aLin = p.eps - p.alpha * d * d;  // pf3d.cpp:205-206
@@ -211,7 +215,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
         if constexpr (MODE == X_STEP) {
           // psi_new = (C1 psi + dt/2 (3 N - N_past))/C2 carried as R = C1 psi - dt/2 N_past
           // (pf3d.cpp:476-478): psi_new = (R + 3 dt/2 N)/C2, R' = C1 psi_new - dt/2 N.
-          const cplx<T> r = (ORD == 0) ? ld_off<cplx<T>>(rb, o) : cr[e];
+          const cplx<T> r = (ORD == 0 || ORD == 3) ? ld_off<cplx<T>>(rb, o) : cr[e];
           const T ic2 = recip(C2);
           cplx<T> ps = mk(ic2 * (r.x + (T)3 * p.dtd2 * nn.x), ic2 * (r.y + (T)3 * p.dtd2 * nn.y));
           cplx<T> rn = mk(C1 * ps.x - p.dtd2 * nn.x, C1 * ps.y - p.dtd2 * nn.y);
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
             ps = mk((T)ps0, (T)0);
             rn = mk((T)p.mean_state[0], (T)0);
           }
-          if (herm) S[kxi * NCP + col] = rn;
+          if (herm || kStage) S[kxi * NCP + col] = rn;
           else if constexpr (ORD != 1) st_off(rb, o, rn);
           if constexpr (ORD == 1) cr[e] = rn;
           if (p.rec_psi != nullptr) st_off(p.rec_psi + plane, o, ps);
@@ -253,6 +257,12 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
         const cplx<T> h = mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y));
         if constexpr (ORD != 1) st_off(rb, (uint32_t)kxi * rowb + cofs, h);
         else cr[e] = h;
+      }
+    } else if (kStage) {  // each thread reads back its own entries
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
+        st_off(rb, (uint32_t)kxi * rowb + cofs, S[kxi * NCP + col]);
       }
     }
   }

@@ -1,7 +1,7 @@
 """Tile schedules and the autotuner of the periodic step (libsmectic through the C ABI).
 
 Every schedule candidate (columns per tile 8/16/32, radix-plan variant 0/1, and for the
-x pass the carry-order forms 2-5) of every step kernel must give the oracle's
+x pass the carry-order forms 2-7) of every step kernel must give the oracle's
 result to the usual tolerance: FP64 relative
 L2(psi) <= 1e-11 after 5 steps, FP32-mixed <= 2e-5.  The autotuner must leave the
 state untouched, return a valid schedule, and the tuned step must keep parity.
@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 NCS = (8, 16, 32)
 VARIANTS = (0, 1)
-VARIANTS_X = (0, 1, 2, 3, 4, 5)
+VARIANTS_X = (0, 1, 2, 3, 4, 5, 6, 7)
 DEFAULT = [(16, 0)] * 4
 
 
@@ -81,9 +81,9 @@ def test_invalid_schedules_rejected():
     say("invalid schedules")
     with Solver(cfg_reference(64, 48, 32)) as s:
         for bad in ([(7, 0)] + DEFAULT[1:], [(16, 2)] + DEFAULT[1:], DEFAULT[:3] + [(32, 0)],
-                    DEFAULT[:3] + [(16, 6)]):
+                    DEFAULT[:3] + [(16, 8)]):
             with pytest.raises(SmxError) as ei:
-                s.set_schedule(bad)  # nc 7; y variant 2; x tiles of 32 do not divide ny = 48; x variant 6
+                s.set_schedule(bad)  # nc 7; y variant 2; x tiles of 32 do not divide ny = 48; x variant 8
             assert ei.value.code == A.SMX_ERR_UNSUPPORTED
         assert s.schedule() == DEFAULT
     with Solver(cfg_reference(32, 32, 32, bc=A.SMX_BC_NEUMANN)) as s:
