@@ -296,8 +296,9 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_force
   const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, Z::NT>(TW, p.twz, tid);
-  __shared__ T KZ[L];  // k_z, first read after several transforms' barriers
+  __shared__ T KZ[L];  // k_z for the d/dz field
   for (int i = tid; i < L; i += Z::NT) KZ[i] = p.kz[i];
+  __syncthreads();  // one-pass plans have no transform barrier before KZ is read
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * Z::CB;
   cplx<T> v[E], f[E];
@@ -341,8 +342,9 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_advnl
   const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, Z::NT>(TW, p.twz, tid);
-  __shared__ T KZ[L];  // k_z, first read after several transforms' barriers
+  __shared__ T KZ[L];  // k_z for the d/dz field
   for (int i = tid; i < L; i += Z::NT) KZ[i] = p.kz[i];
+  __syncthreads();  // one-pass plans have no transform barrier before KZ is read
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * Z::CB;
   constexpr int RP = fwd_first<L, V>();

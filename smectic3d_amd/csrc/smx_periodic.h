@@ -137,9 +137,11 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
-  __shared__ T KX[(MODE == X_STEP || MODE == X_BOOT) ? L : 1];  // read after the transform's barriers
-  if constexpr (MODE == X_STEP || MODE == X_BOOT)
+  __shared__ T KX[(MODE == X_STEP || MODE == X_BOOT) ? L : 1];
+  if constexpr (MODE == X_STEP || MODE == X_BOOT) {
     for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
+    __syncthreads();  // a one-pass plan has no barrier of its own before the update reads KX
+  }
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;  // bytes between kx rows
