@@ -31,7 +31,7 @@ def lib():
         L.or_create.restype = C.c_void_p
         L.or_create.argtypes = [C.POINTER(SmxConfig)]
         L.or_destroy.argtypes = [C.c_void_p]
-        for name in ("or_set_psi", "or_get_psi", "or_dtpsi", "or_get_nqpast"):
+        for name in ("or_set_psi", "or_get_psi", "or_dtpsi", "or_get_nqpast", "or_velocity"):
             getattr(L, name).argtypes = [C.c_void_p, _dp]
             getattr(L, name).restype = C.c_int
         L.or_step.argtypes = [C.c_void_p, C.c_int64]
@@ -48,7 +48,7 @@ def lib():
         for name in ("or_rfft3", "or_irfft3", "or_redft10_3d", "or_redft01_3d"):
             getattr(L, name).argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
             getattr(L, name).restype = C.c_int
-        for name in ("or_redft10_1d", "or_redft01_1d"):
+        for name in ("or_redft10_1d", "or_redft01_1d", "or_rodft10_1d", "or_rodft01_1d"):
             getattr(L, name).argtypes = [C.c_int, _dp, _dp]
         L.or_splitmix_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.or_splitmix_uniform.restype = C.c_double
@@ -122,6 +122,20 @@ def redft01_1d(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     out = np.zeros_like(a)
     lib().or_redft01_1d(len(a), _p(a), _p(out))
+    return out
+
+
+def rodft10_1d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.zeros_like(a)
+    lib().or_rodft10_1d(len(a), _p(a), _p(out))
+    return out
+
+
+def rodft01_1d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.zeros_like(a)
+    lib().or_rodft01_1d(len(a), _p(a), _p(out))
     return out
 
 
@@ -238,6 +252,12 @@ class Oracle:
 
     def dominant_wavenumber(self):
         return lib().or_dominant_wavenumber(self._s)
+
+    def velocity(self):
+        """Stokes velocity (3, nx, ny, nz) of the current Neumann state."""
+        out = np.zeros((3,) + self.shape)
+        assert lib().or_velocity(self._s, _p(out)) == 0
+        return out
 
     def dtpsi(self):
         out = np.zeros(self.shape)

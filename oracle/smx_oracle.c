@@ -257,30 +257,77 @@ void or_redft01_1d(int n, const double *in, double *out) {
   free(v); free(w); plan_destroy(p);
 }
 
+/* DST-II (RODFT10) / DST-III (RODFT01), FFTW's definitions:
+ *   RODFT10: y_k = 2 sum_j x_j sin(pi (j+1/2)(k+1)/n)
+ *   RODFT01: y_k = (-1)^k x_{n-1} + 2 sum_{j<n-1} x_j sin(pi (j+1)(k+1/2)/n)
+ * through the cosine lines: sin(pi (j+1/2)(n-k)/n) = (-1)^j cos(pi (j+1/2) k/n), so
+ * RODFT10[x]_k = REDFT10[(-1)^j x_j]_{n-1-k} and RODFT01[X]_j = (-1)^j REDFT01[X_{n-1-k}]_j.
+ * t: scratch of 2n. */
+static void rodft10_line(const plan1d *p, const double *x, int64_t xs, double *y, int64_t ys, cplx *v, cplx *w,
+                         double *t) {
+  const int n = p->n;
+  for (int j = 0; j < n; ++j) t[j] = (j & 1) ? -x[j * xs] : x[j * xs];
+  redft10_line(p, t, 1, t + n, 1, v, w);
+  for (int k = 0; k < n; ++k) y[k * ys] = t[n + (n - 1 - k)];
+}
+
+static void rodft01_line(const plan1d *p, const double *x, int64_t xs, double *y, int64_t ys, cplx *v, cplx *w,
+                         double *t) {
+  const int n = p->n;
+  for (int k = 0; k < n; ++k) t[k] = x[(n - 1 - k) * xs];
+  redft01_line(p, t, 1, t + n, 1, v, w);
+  for (int j = 0; j < n; ++j) y[j * ys] = (j & 1) ? -t[n + j] : t[n + j];
+}
+
+void or_rodft10_1d(int n, const double *in, double *out) {
+  plan1d *p = plan_create(n);
+  cplx *v = (cplx *)malloc(sizeof(cplx) * n), *w = (cplx *)malloc(sizeof(cplx) * n);
+  double *t = (double *)malloc(sizeof(double) * 2 * n);
+  rodft10_line(p, in, 1, out, 1, v, w, t);
+  free(v); free(w); free(t); plan_destroy(p);
+}
+
+void or_rodft01_1d(int n, const double *in, double *out) {
+  plan1d *p = plan_create(n);
+  cplx *v = (cplx *)malloc(sizeof(cplx) * n), *w = (cplx *)malloc(sizeof(cplx) * n);
+  double *t = (double *)malloc(sizeof(double) * 2 * n);
+  rodft01_line(p, in, 1, out, 1, v, w, t);
+  free(v); free(w); free(t); plan_destroy(p);
+}
+
+/* kind: 10 REDFT10, 1 REDFT01, 110 RODFT10, 101 RODFT01 */
 static void r2r_axis(double *a, int n, int64_t st, int64_t nouter, int64_t ostride, int64_t ninner, int kind) {
   plan1d *p = plan_create(n);
 #pragma omp parallel
   {
     cplx *v = (cplx *)malloc(sizeof(cplx) * (size_t)n), *w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
     double *line = (double *)malloc(sizeof(double) * (size_t)n);
+    double *t = (double *)malloc(sizeof(double) * 2 * (size_t)n);
 #pragma omp for collapse(2) schedule(static)
     for (int64_t o = 0; o < nouter; ++o)
       for (int64_t q = 0; q < ninner; ++q) {
         double *b = a + o * ostride + q;
         if (kind == 10) redft10_line(p, b, st, line, 1, v, w);
-        else redft01_line(p, b, st, line, 1, v, w);
+        else if (kind == 1) redft01_line(p, b, st, line, 1, v, w);
+        else if (kind == 110) rodft10_line(p, b, st, line, 1, v, w, t);
+        else rodft01_line(p, b, st, line, 1, v, w, t);
         for (int i = 0; i < n; ++i) b[i * st] = line[i];
       }
-    free(v); free(w); free(line);
+    free(v); free(w); free(line); free(t);
   }
   plan_destroy(p);
 }
 
-static void r2r3(int nx, int ny, int nz, const double *in, double *out, int kind) {
+/* separable 3-D r2r transform of a dense [x][y][z] array, one kind per axis */
+static void r2r3k(int nx, int ny, int nz, const double *in, double *out, int kx, int ky, int kz) {
   if (out != in) memcpy(out, in, sizeof(double) * (size_t)nx * ny * nz);
-  r2r_axis(out, nz, 1, (int64_t)nx * ny, nz, 1, kind);
-  r2r_axis(out, ny, nz, nx, (int64_t)ny * nz, nz, kind);
-  r2r_axis(out, nx, (int64_t)ny * nz, 1, 0, (int64_t)ny * nz, kind);
+  r2r_axis(out, nz, 1, (int64_t)nx * ny, nz, 1, kz);
+  r2r_axis(out, ny, nz, nx, (int64_t)ny * nz, nz, ky);
+  r2r_axis(out, nx, (int64_t)ny * nz, 1, 0, (int64_t)ny * nz, kx);
+}
+
+static void r2r3(int nx, int ny, int nz, const double *in, double *out, int kind) {
+  r2r3k(nx, ny, nz, in, out, kind, kind, kind);
 }
 
 int or_redft10_3d(int nx, int ny, int nz, const double *in, double *out) { r2r3(nx, ny, nz, in, out, 10); return 0; }
@@ -508,21 +555,120 @@ static void periodic_step(or_state *s) {
   free(psiq); free(nq); free(nr);
 }
 
-/* One Neumann step, cosFFT/mesabi/cosNoAdvMorph.cpp:490-530,619. */
+#define NIDX(i, j, k) (((int64_t)(i) * s->ny + (j)) * s->nz + (k))
+
+/* Advective term -U.grad(psi) of the Neumann step, added into nr; psiq = REDFT10(psi),
+ * unnormalised.  The reference has the sine/cosine Stokes machinery for its Neumann box
+ * (morphSurfK.cpp:331-348 wavenumbers, :392-418 plans, :864-882 projection) but never applies
+ * it (:896-898) and projects every component in one all-sine basis; this is the formulation
+ * with per-component parity.  psi is a cosine series along every axis (REDFT10 grid,
+ * wavenumbers pi k/L).  d_a psi, the force component S_a and the velocity U_a are sine series
+ * along axis a (RODFT10/01: entry m <-> wavenumber pi (m+1)/L_a, so m = n_a - 1 is pi/d_a)
+ * and cosine series along the other two axes.  The Stokes projection
+ * U = s (S - K (K.S)/|K|^2) / (rho eta |K|^2) acts per wavevector K = (Kx, Ky, Kz),
+ * 0 <= K_a <= n_a, on the components present there (a sine component needs K_a >= 1, a
+ * cosine one K_b <= n_b - 1).  With the force psiSH grad psi (pf3d.cpp:417-425) or
+ * mu grad psi, this is the periodic step (pf3d.cpp:399-465) of the box mirrored evenly to
+ * [0, 2L) along every axis, restricted to [0, L). */
+static void neumann_advection(or_state *s, const double *psiq, double *nr, double *uout) {
+  const int64_t N = s->N;
+  const int n[3] = {s->nx, s->ny, s->nz};
+  const double d[3] = {s->dx, s->dy, s->dz};
+  const double sc = s->scale;
+  double *g[3], *S[3], *Sq[3], *Uq[3];
+  double *shq = (double *)malloc(sizeof(double) * N), *sh = (double *)malloc(sizeof(double) * N);
+  for (int a = 0; a < 3; ++a) {
+    g[a] = (double *)malloc(sizeof(double) * N);
+    S[a] = (double *)malloc(sizeof(double) * N);
+    Sq[a] = (double *)malloc(sizeof(double) * N);
+    Uq[a] = (double *)calloc((size_t)N, sizeof(double));
+  }
+  /* d/dx cos(q_m x) = -q_m sin(q_m x): sine entry m holds -q_{m+1} psiq_{m+1}; entry n-1 is 0 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        const int64_t id = NIDX(i, j, k);
+        g[0][id] = (i + 1 < s->nx) ? -sc * s->vqx[i + 1] * psiq[NIDX(i + 1, j, k)] : 0.0;
+        g[1][id] = (j + 1 < s->ny) ? -sc * s->vqy[j + 1] * psiq[NIDX(i, j + 1, k)] : 0.0;
+        g[2][id] = (k + 1 < s->nz) ? -sc * s->vqz[k + 1] * psiq[NIDX(i, j, k + 1)] : 0.0;
+        const double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
+        shq[id] = sc * op_sh(s, k2) * psiq[id];
+      }
+  r2r3k(s->nx, s->ny, s->nz, g[0], g[0], 101, 1, 1);
+  r2r3k(s->nx, s->ny, s->nz, g[1], g[1], 1, 101, 1);
+  r2r3k(s->nx, s->ny, s->nz, g[2], g[2], 1, 1, 101);
+  or_redft01_3d(s->nx, s->ny, s->nz, shq, sh);
+#pragma omp parallel for schedule(static)
+  for (int64_t x = 0; x < N; ++x) {
+    double f = sh[x];
+    if (s->c.force_form == SMX_FORCE_MU) f = sh[x] - s->c.eps * s->psi[x] - nl(s, s->psi[x]);
+    for (int a = 0; a < 3; ++a) S[a][x] = f * g[a][x];
+  }
+  r2r3k(s->nx, s->ny, s->nz, S[0], Sq[0], 110, 10, 10);
+  r2r3k(s->nx, s->ny, s->nz, S[1], Sq[1], 10, 110, 10);
+  r2r3k(s->nx, s->ny, s->nz, S[2], Sq[2], 10, 10, 110);
+  const double reta = s->c.rho * s->c.eta;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int K0 = 0; K0 <= n[0]; ++K0)
+    for (int K1 = 0; K1 <= n[1]; ++K1)
+      for (int K2 = 0; K2 <= n[2]; ++K2) {
+        const int K[3] = {K0, K1, K2};
+        double q[3], k2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          q[a] = M_PI * K[a] / (d[a] * n[a]);
+          k2 += q[a] * q[a];
+        }
+        int64_t idx[3];
+        int present[3];
+        double dot = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          int e[3];
+          present[a] = K[a] >= 1;
+          for (int b = 0; b < 3; ++b) {
+            e[b] = (b == a) ? K[b] - 1 : K[b];
+            if (b != a && K[b] > n[b] - 1) present[a] = 0;
+          }
+          idx[a] = present[a] ? NIDX(e[0], e[1], e[2]) : -1;
+          if (present[a]) dot += q[a] * Sq[a][idx[a]];
+        }
+        if (k2 == 0.0) continue;
+        const double cm1 = 1.0 / (reta * k2);
+        for (int a = 0; a < 3; ++a)
+          if (present[a]) Uq[a][idx[a]] = sc * cm1 * (Sq[a][idx[a]] - (q[a] / k2) * dot);
+      }
+  r2r3k(s->nx, s->ny, s->nz, Uq[0], Uq[0], 101, 1, 1);
+  r2r3k(s->nx, s->ny, s->nz, Uq[1], Uq[1], 1, 101, 1);
+  r2r3k(s->nx, s->ny, s->nz, Uq[2], Uq[2], 1, 1, 101);
+  if (uout)
+    for (int a = 0; a < 3; ++a) memcpy(uout + a * N, Uq[a], sizeof(double) * (size_t)N);
+  if (nr) {
+#pragma omp parallel for schedule(static)
+    for (int64_t x = 0; x < N; ++x) nr[x] -= Uq[0][x] * g[0][x] + Uq[1][x] * g[1][x] + Uq[2][x] * g[2][x];
+  }
+  for (int a = 0; a < 3; ++a) {
+    free(g[a]); free(S[a]); free(Sq[a]); free(Uq[a]);
+  }
+  free(shq); free(sh);
+}
+
+/* One Neumann step, cosFFT/mesabi/cosNoAdvMorph.cpp:490-530,619 (plus the advective term
+ * above when advection is on). */
 static void neumann_step(or_state *s) {
   const int64_t N = s->N;
   const double sc = s->scale, dtd2 = 0.5 * s->c.dt;
   double *nr = (double *)malloc(sizeof(double) * N);
   double *nq = (double *)malloc(sizeof(double) * N);
   double *psiq = (double *)malloc(sizeof(double) * N);
-  nl_field(s, s->psi, nr);                               /* :494-504 */
-  or_redft10_3d(s->nx, s->ny, s->nz, nr, nq);            /* :508 */
   or_redft10_3d(s->nx, s->ny, s->nz, s->psi, psiq);      /* :512 */
+  nl_field(s, s->psi, nr);                               /* :494-504 */
+  if (s->c.advect) neumann_advection(s, psiq, nr, NULL);
+  or_redft10_3d(s->nx, s->ny, s->nz, nr, nq);            /* :508 */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int i = 0; i < s->nx; ++i)
     for (int j = 0; j < s->ny; ++j)
       for (int k = 0; k < s->nz; ++k) {
-        int64_t id = ((int64_t)i * s->ny + j) * s->nz + k;
+        int64_t id = NIDX(i, j, k);
         double k2 = s->vqx[i] * s->vqx[i] + s->vqy[j] * s->vqy[j] + s->vqz[k] * s->vqz[k];
         double aLin = s->c.eps - op_sh(s, k2); /* :431-433 */
         double C1 = 1.0 + dtd2 * aLin, C2 = 1.0 - dtd2 * aLin;
@@ -534,6 +680,18 @@ static void neumann_step(or_state *s) {
   s->psi = t;
   or_redft01_3d(s->nx, s->ny, s->nz, psiq, s->psi); /* :530 */
   free(nr); free(nq); free(psiq);
+}
+
+/* Stokes velocity (U_x, U_y, U_z), each dense [x][y][z], of the current Neumann state, as the
+ * advected step forms it. */
+int or_velocity(or_state *s, double *u) {
+  if (!s->has_state) return SMX_ERR_STATE;
+  if (s->c.bc != SMX_BC_NEUMANN) return SMX_ERR_UNSUPPORTED;
+  double *psiq = (double *)malloc(sizeof(double) * (size_t)s->N);
+  or_redft10_3d(s->nx, s->ny, s->nz, s->psi, psiq);
+  neumann_advection(s, psiq, NULL, u);
+  free(psiq);
+  return 0;
 }
 
 int or_step(or_state *s, int64_t nsteps) {

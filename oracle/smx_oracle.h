@@ -30,6 +30,8 @@ int or_redft10_3d(int nx, int ny, int nz, const double *in, double *out); /* DCT
 int or_redft01_3d(int nx, int ny, int nz, const double *in, double *out); /* DCT-III all axes */
 void or_redft10_1d(int n, const double *in, double *out);
 void or_redft01_1d(int n, const double *in, double *out);
+void or_rodft10_1d(int n, const double *in, double *out); /* DST-II */
+void or_rodft01_1d(int n, const double *in, double *out); /* DST-III */
 
 /* ---- solver following the reference dataflow ---- */
 typedef struct or_state or_state;
@@ -38,6 +40,7 @@ void or_destroy(or_state *s);
 int or_set_psi(or_state *s, const double *psi);    /* dense [x][y][z]; bootstrap NqPast */
 int or_get_psi(const or_state *s, double *psi);
 int or_step(or_state *s, int64_t nsteps);
+int or_velocity(or_state *s, double *u);        /* Neumann Stokes velocity, 3 dense fields */
 double or_l1(const or_state *s);                   /* over the last step */
 double or_energy(const or_state *s);               /* spectral quadrature */
 double or_energy_fd_trapz(const or_state *s);      /* energyPsi.m:217-234 */
