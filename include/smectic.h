@@ -70,7 +70,9 @@ typedef struct smx_config {
   double lx, ly, lz; /* box lengths; dx = lx/nx etc. */
   double eps, q0, alpha, beta, gamma, dt; /* L(k) = eps - alpha*(|k|^2 - q0^2)^2 */
   int nl_model;    /* SMX_NL_* */
-  int advect;      /* 1: spectral Stokes advection (periodic only), pf3d.cpp:399-465 */
+  int advect;      /* 1: spectral Stokes advection, pf3d.cpp:399-465; with SMX_BC_NEUMANN the
+                      sine/cosine projection of per-component parity (morphSurfK.cpp:331-348,
+                      864-882 has the machinery, never applied, :896-898) */
   double eta, rho; /* Stokes viscosity and density, pf3d.cpp:54-55,210 */
   int force_form;  /* SMX_FORCE_* */
   double w_mid;    /* mid-plane penalty weight, pf3dMorph.cpp:102 (0 = off) */

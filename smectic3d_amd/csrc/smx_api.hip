@@ -283,7 +283,11 @@ static int bootstrap(smx_handle* h) {
   CHK(xpass<T>(h, p, X_FWD_SCALED, h->A, h->P, nullptr, nullptr));
   CHK(zpass<T>(h, p, Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, h->A, h->real_io, nullptr));
   CHK(ypass<T>(h, h->A, h->A, -1));
-  if (h->cfg.advect) {
+  if (h->cfg.advect && h->neumann) {
+    hipError_t e = NeuAdvOps<T>::xpass_adv(X_BOOT, (const T*)h->A, (T*)h->R, (const T*)h->P, (T*)h->A, (T*)h->Ax,
+                                           (T*)h->As, p, tables<T>(h).qtwx, h->st);
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("x pass (advected): ") + hipGetErrorString(e));
+  } else if (h->cfg.advect) {
     hipError_t e = AdvectOps<T>::xpass_adv(X_BOOT, (const cplx<T>*)h->A, (cplx<T>*)h->R, (const cplx<T>*)h->P,
                                            (cplx<T>*)h->A, (cplx<T>*)h->Ax, (cplx<T>*)h->As, p, h->st);
     if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("x pass (advected): ") + hipGetErrorString(e));
@@ -300,9 +304,15 @@ static const char* kSlotNamesA[] = {"k_ypass_inv_dy", "k_ypass_inv_psi", "k_ypas
                                     "k_zpass_force", "k_ypass_fwd_s0", "k_ypass_fwd_s1", "k_ypass_fwd_s2",
                                     "k_xpass_stokes", "k_ypass_inv_u0", "k_ypass_inv_u1", "k_ypass_inv_u2",
                                     "k_zpass_advnl", "k_ypass_fwd_n", "k_xpass_adv"};
+static const char* kSlotNamesNA[] = {"k_dct_ypass_inv_dy", "k_dct_ypass_inv_psi", "k_dct_ypass_inv_dx",
+                                     "k_dct_ypass_inv_sh", "k_dctz_force", "k_dct_ypass_fwd_s0",
+                                     "k_dct_ypass_fwd_s1", "k_dct_ypass_fwd_s2", "k_dctx_stokes",
+                                     "k_dct_ypass_inv_u0", "k_dct_ypass_inv_u1", "k_dct_ypass_inv_u2",
+                                     "k_dctz_advnl", "k_dct_ypass_fwd_n", "k_dctx_adv"};
 static int num_slots(const smx_handle* h) { return h->cfg.advect ? 15 : 4; }
 static const char* slot_name(const smx_handle* h, int s) {
-  return h->neumann ? kSlotNamesN[s] : (h->cfg.advect ? kSlotNamesA[s] : kSlotNamesP[s]);
+  if (h->cfg.advect) return h->neumann ? kSlotNamesNA[s] : kSlotNamesA[s];
+  return h->neumann ? kSlotNamesN[s] : kSlotNamesP[s];
 }
 
 // One advected periodic step (pf3d.cpp:395-485): 16 launches, in place per tile.
@@ -372,8 +382,68 @@ static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
   return SMX_OK;
 }
 
+// One advected Neumann step: the 15 launches of step_advect on the DCT grid, with the sine
+// parities of d_a psi, S_a and U_a (smx_neuadv.h).
+template <typename T>
+static int step_advect_neumann(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
+  using NO = NeumannOps<T>;
+  using NA = NeuAdvOps<T>;
+  const PParams<T>& p = ap.p;
+  const DevTables<T>& t = tables<T>(h);
+  T *A = (T*)h->A, *Ax = (T*)h->Ax, *As = (T*)h->As, *By = (T*)h->By;
+  T *S1 = (T*)h->S1, *S2 = (T*)h->S2, *R = (T*)h->R;
+  const int64_t np = h->npencils;
+  const T qnx = (T)(M_PI / h->dx), qny = (T)(M_PI / h->dy), qnz = (T)(M_PI / h->dz);
+  int slot = 0;
+  hipError_t e = hipSuccess;
+  SlotTimer tm(ev);
+#define SMX_REC()                                                                                          \
+  do {                                                                                                     \
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e)); \
+    if (++slot < 15) tm.arm(slot);                                                                         \
+  } while (0)
+  tm.arm(0);
+  e = NO::ypass(A, By, p.ny, np, +1, t.twy, t.qtwy, h->st, DP_DSIN, t.ky);  // d_y psi (sine along y)
+  SMX_REC();
+  e = NO::ypass(A, A, p.ny, np, +1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NO::ypass(Ax, Ax, p.ny, np, +1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NO::ypass(As, As, p.ny, np, +1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NA::zforce(A, Ax, By, As, As, S1, S2, ap, t.qtwz, h->st);  // S = f grad psi (pf3d.cpp:417-425)
+  SMX_REC();
+  e = NO::ypass(As, As, p.ny, np, -1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NO::ypass(S1, S1, p.ny, np, -1, t.twy, t.qtwy, h->st, DP_SIN);
+  SMX_REC();
+  e = NO::ypass(S2, S2, p.ny, np, -1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NA::stokes(As, S1, S2, ap, t.qtwx, qnx, qny, qnz, h->st);  // (pf3d.cpp:433-447)
+  SMX_REC();
+  e = NO::ypass(As, As, p.ny, np, +1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NO::ypass(S1, S1, p.ny, np, +1, t.twy, t.qtwy, h->st, DP_SIN);
+  SMX_REC();
+  e = NO::ypass(S2, S2, p.ny, np, +1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NA::zadvnl(As, S1, S2, A, Ax, By, As, ap, t.qtwz, h->st);  // N(psi) - U.grad psi (:455-465)
+  SMX_REC();
+  e = NO::ypass(As, As, p.ny, np, -1, t.twy, t.qtwy, h->st);
+  SMX_REC();
+  e = NA::xpass_adv(X_STEP, As, R, nullptr, A, Ax, As, p, t.qtwx, h->st);  // CNAB2 (cosNoAdvMorph.cpp:524-525)
+  SMX_REC();
+#undef SMX_REC
+  return SMX_OK;
+}
+
 template <typename T>
 static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
+  if (h->cfg.advect && h->neumann) {
+    AParams<T> ap = aparams<T>(h, h->fp32);
+    ap.p.npartial = NeuAdvOps<T>::zpass_blocks(ap.p);  // FP64 partials of the advected z pass
+    return step_advect_neumann<T>(h, ap, ev);
+  }
   if (h->cfg.advect) {
     AParams<T> ap = aparams<T>(h, h->fp32);
     ap.p.npartial = AdvectOps<T>::zpass_blocks(ap.p);  // FP64 partials of the advected z pass
@@ -1095,8 +1165,6 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
   if (c.precision != SMX_FP64 && c.precision != SMX_FP32_MIXED) return fail(SMX_ERR_INVALID, "bad precision");
   if (c.nl_model != SMX_NL_QUINTIC && c.nl_model != SMX_NL_CUBIC) return fail(SMX_ERR_INVALID, "bad nl_model");
   if (!(c.lx > 0 && c.ly > 0 && c.lz > 0 && c.dt > 0)) return fail(SMX_ERR_INVALID, "lengths and dt must be > 0");
-  if (c.advect && c.bc == SMX_BC_NEUMANN)
-    return fail(SMX_ERR_UNSUPPORTED, "advection is periodic only (the reference's Neumann drivers do not advect)");
   for (int n : {c.nx, c.ny, c.nz})
     if (!length_supported(n))
       return fail(SMX_ERR_UNSUPPORTED, "grid length " + std::to_string(n) +

@@ -3,4 +3,5 @@
 template struct smx::PeriodicOps<float>;
 template struct smx::NeumannOps<float>;
 template struct smx::AdvectOps<float>;
+template struct smx::NeuAdvOps<float>;
 template struct smx::AdjointOps<float>;

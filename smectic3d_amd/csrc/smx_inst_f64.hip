@@ -3,4 +3,5 @@
 template struct smx::PeriodicOps<double>;
 template struct smx::NeumannOps<double>;
 template struct smx::AdvectOps<double>;
+template struct smx::NeuAdvOps<double>;
 template struct smx::AdjointOps<double>;

@@ -7,7 +7,7 @@
 #include <type_traits>
 
 #include "smx_adjoint.h"
-#include "smx_neumann.h"
+#include "smx_neuadv.h"
 
 namespace smx {
 
@@ -220,8 +220,9 @@ template <typename T> constexpr int neumann_nc() { return sizeof(T) == 4 ? 32 : 
 template <typename T>
 struct NeumannOps {
   // y pass on real pencils (npencils = nz*nx), dir -1: DCT-II, +1: DCT-III.
+  // par: parity of the pencils along y (DPar; DP_DSIN with dir +1 and kq = k_y).
   static hipError_t ypass(const T* in, T* out, int ny, int64_t npencils, int dir, const cplx<T>* tw,
-                          const cplx<T>* qtw, hipStream_t st);
+                          const cplx<T>* qtw, hipStream_t st, int par = DP_COS, const T* kq = nullptr);
   static hipError_t xpass(int mode, const T* in, T* out, T* rbuf, const T* psi0, const PParams<T>& p,
                           const cplx<T>* qtw, hipStream_t st);
   static hipError_t zpass(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
@@ -231,16 +232,20 @@ struct NeumannOps {
 
 template <typename T>
 hipError_t NeumannOps<T>::ypass(const T* in, T* out, int ny, int64_t npencils, int dir, const cplx<T>* tw,
-                                const cplx<T>* qtw, hipStream_t st) {
+                                const cplx<T>* qtw, hipStream_t st, int par, const T* kq) {
   constexpr int NC = neumann_nc<T>();
   if (npencils % NC != 0) return hipErrorInvalidValue;  // whole tiles of pencil pairs
+  if (par == DP_DSIN && (dir < 0 || kq == nullptr)) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(npencils / NC));
-#define SMX_CASE(L)                                                                                 \
-  if (ny == L) {                                                                                    \
-    const dim3 block((NC / 2) * (L / Plan<L>::E));                                                  \
-    if (dir < 0) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC>), grid, block, 0, st, in, out, tw, qtw);     \
-    else SMX_LAUNCH((k_dct_ypass<T, L, +1, NC>), grid, block, 0, st, in, out, tw, qtw);             \
-    return hipGetLastError();                                                                       \
+#define SMX_CASE(L)                                                                                       \
+  if (ny == L) {                                                                                          \
+    const dim3 block((NC / 2) * (L / Plan<L>::E));                                                        \
+    if (dir < 0 && par == DP_COS) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC>), grid, block, 0, st, in, out, tw, qtw, kq); \
+    else if (dir < 0) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC, DP_SIN>), grid, block, 0, st, in, out, tw, qtw, kq);     \
+    else if (par == DP_COS) SMX_LAUNCH((k_dct_ypass<T, L, +1, NC>), grid, block, 0, st, in, out, tw, qtw, kq);       \
+    else if (par == DP_SIN) SMX_LAUNCH((k_dct_ypass<T, L, +1, NC, DP_SIN>), grid, block, 0, st, in, out, tw, qtw, kq); \
+    else SMX_LAUNCH((k_dct_ypass<T, L, +1, NC, DP_DSIN>), grid, block, 0, st, in, out, tw, qtw, kq);                \
+    return hipGetLastError();                                                                             \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
@@ -407,6 +412,84 @@ struct AdvectOps {
     constexpr int V = vz<L>();                                                                          \
     SMX_LAUNCH((k_zpass_advnl<T, L, NCZ, V>), grid, dim3((NCZ / 2) * (L / PlanV<L, V>::E)), 0, st, u0, u1, u2, \
                bpsi, bx, by, nout, ap);                                                                 \
+    return hipGetLastError();                                                                           \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  // partial sums written by zadvnl: one per z tile
+  static int zpass_blocks(const PParams<T>& p) { return (int)(p.nxy / NCZ); }
+};
+
+// Kernels of the advected Neumann step (smx_neuadv.h).  The z kernels take the Neumann z
+// pass's tiles, so their FP64 partials match its count; the Stokes pass holds three LDS
+// tiles, 64-byte rows.
+template <typename T>
+struct NeuAdvOps {
+  static constexpr int NCX = 16;
+  static constexpr int NCS = sizeof(T) == 4 ? 16 : 8;
+  static constexpr int NCZ = neumann_nc<T>();
+  static hipError_t xpass_adv(int mode, const T* in, T* rbuf, const T* psi0, T* oa, T* ox, T* os,
+                              const PParams<T>& p, const cplx<T>* qtw, hipStream_t st) {
+    if (p.ny % NCX) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.ny / NCX), (unsigned)p.nzh);
+#define SMX_CASE(L)                                                                                      \
+  if (p.nx == L) {                                                                                       \
+    const dim3 block((NCX / 2) * (L / Plan<L>::E));                                                      \
+    if (mode == X_STEP)                                                                                  \
+      SMX_LAUNCH((k_dctx_adv<T, L, NCX, X_STEP>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p, qtw); \
+    else if (mode == X_BOOT)                                                                             \
+      SMX_LAUNCH((k_dctx_adv<T, L, NCX, X_BOOT>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p, qtw); \
+    else                                                                                                 \
+      return hipErrorInvalidValue;                                                                       \
+    return hipGetLastError();                                                                            \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  static hipError_t stokes(T* s0, T* s1, T* s2, const AParams<T>& ap, const cplx<T>* qtw, T qnx, T qny, T qnz,
+                           hipStream_t st) {
+    const PParams<T>& p = ap.p;
+    if (p.ny % NCS) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.ny / NCS), (unsigned)p.nzh);
+#define SMX_CASE(L)                                                                                     \
+  if (p.nx == L) {                                                                                      \
+    static_assert((3 * (size_t)L * (NCS / 2 + 1) + 2 * L) * sizeof(cplx<T>) + L * sizeof(T) <= kMaxLds, \
+                  "Stokes tiles exceed LDS");                                                            \
+    SMX_LAUNCH((k_dctx_stokes<T, L, NCS>), grid, dim3((NCS / 2) * (L / Plan<L>::E)), 0, st, s0, s1, s2, ap, qtw, \
+               qnx, qny, qnz);                                                                          \
+    return hipGetLastError();                                                                           \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  static hipError_t zforce(const T* bpsi, const T* bx, const T* by, const T* bs, T* o0, T* o1, T* o2,
+                           const AParams<T>& ap, const cplx<T>* qtw, hipStream_t st) {
+    const PParams<T>& p = ap.p;
+    if (p.nxy % NCZ) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.nxy / NCZ));
+#define SMX_CASE(L)                                                                                     \
+  if (p.nz == L) {                                                                                      \
+    SMX_LAUNCH((k_dctz_force<T, L, NCZ>), grid, dim3((NCZ / 2) * (L / Plan<L>::E)), 0, st, bpsi, bx, by, bs, o0, \
+               o1, o2, ap, qtw);                                                                        \
+    return hipGetLastError();                                                                           \
+  }
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  }
+  static hipError_t zadvnl(const T* u0, const T* u1, const T* u2, const T* bpsi, const T* bx, const T* by, T* nout,
+                           const AParams<T>& ap, const cplx<T>* qtw, hipStream_t st) {
+    const PParams<T>& p = ap.p;
+    if (p.nxy % NCZ) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.nxy / NCZ));
+#define SMX_CASE(L)                                                                                     \
+  if (p.nz == L) {                                                                                      \
+    SMX_LAUNCH((k_dctz_advnl<T, L, NCZ>), grid, dim3((NCZ / 2) * (L / Plan<L>::E)), 0, st, u0, u1, u2, bpsi, bx, \
+               by, nout, ap, qtw);                                                                      \
     return hipGetLastError();                                                                           \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)

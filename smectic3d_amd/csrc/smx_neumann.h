@@ -28,6 +28,17 @@ namespace smx {
 
 __device__ __forceinline__ int mk_src(int m, int n) { return (m < (n + 1) / 2) ? 2 * m : 2 * (n - 1 - m) + 1; }
 
+// Parity of a line along the axis a pass transforms (the advected Neumann step):
+//   DP_COS   cosine series, spectrum slot k = wavenumber pi k/L;
+//   DP_SIN   sine series (RODFT10/01) through the cosine kernels: the real line is held
+//            folded, x_j (-1)^j, and the spectrum in slot order, slot s in 1..n-1 =
+//            wavenumber pi s/L, slot 0 = pi n/L.  RODFT10[x]_m = REDFT10[(-1)^j x_j]_{n-1-m}:
+//            the DCT-II output at k goes to slot (n-k) mod n; RODFT01[X]_j =
+//            (-1)^j REDFT01[X_{n-1-m}]_j: the DCT-III input at k reads slots (n-k) mod n and k;
+//   DP_DSIN  inverse only: the derivative of a cosine series, slot s times -kq[s]
+//            (d/dx cos(q x) = -q sin(q x); kq[0] = 0), then as DP_SIN.
+enum DPar { DP_COS = 0, DP_SIN = 1, DP_DSIN = 2 };
+
 // REDFT10 outputs of a pair at k from Z[k] and Zm = Z[(n-k) mod n]; w = e^{-i pi k/2n}.
 template <typename T>
 __device__ __forceinline__ cplx<T> dct2_out(cplx<T> Z, cplx<T> Zm, cplx<T> w) {
@@ -41,14 +52,30 @@ __device__ __forceinline__ cplx<T> dct3_in(cplx<T> p, cplx<T> pn, cplx<T> w) {
   return mk(w.x * ux + w.y * uy, w.x * uy - w.y * ux);
 }
 
+// REDFT01 (or, by PAR, sine) FFT input at k of a line whose spectrum holds rk at slot k and rm
+// at slot (L-k) mod L.
+template <int PAR, typename T>
+__device__ __forceinline__ cplx<T> dct3_par(cplx<T> rk, cplx<T> rm, int k, int L, const T* kq, cplx<T> w) {
+  if constexpr (PAR == DP_COS) return dct3_in(rk, (k == 0) ? mk<T>(0, 0) : rm, w);
+  cplx<T> P = (k == 0) ? rk : rm, Pn = (k == 0) ? mk<T>(0, 0) : rk;
+  if constexpr (PAR == DP_DSIN) {
+    P = -kq[(L - k) % L] * P;
+    Pn = -kq[k] * Pn;
+  }
+  return dct3_in(P, Pn, w);
+}
+
 // ---------------------------------------------------------------------------
 // y pass: contiguous real pencils of length L = ny; pencils g = kz*nx + x, pairs of adjacent
 // pencils.  Lanes run along the pencils (tc fastest).  DIR -1: DCT-II, +1: DCT-III.
 // The grid covers whole tiles (nx*nz is a multiple of NC; the launcher checks it).
-template <typename T, int L, int DIR, int NC>
+// PAR: parity of the pencils along y (DP_DSIN: DIR +1 only, kq = k_y).
+template <typename T, int L, int DIR, int NC, int PAR = DP_COS>
 __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const T* in, T* out,
                                                                            const cplx<T>* __restrict__ tw,
-                                                                           const cplx<T>* __restrict__ qtw) {
+                                                                           const cplx<T>* __restrict__ qtw,
+                                                                           const T* __restrict__ kq) {
+  static_assert(DIR > 0 || PAR != DP_DSIN, "the derivative is an inverse-transform input");
   constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
   using AD = ColAddr<L>;
   using TL = TileA<T, L, E, AD>;
@@ -78,15 +105,21 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
     for (int i = tid; i < NPR * L; i += NT) {
       const int qq = i / L, k = i - qq * L;
       const cplx<T> y = dct2_out(S[AD::at(k, qq)], S[AD::at((L - k) % L, qq)], QW[k]);
-      oa[(2 * qq) * L + k] = y.x;
-      oa[(2 * qq + 1) * L + k] = y.y;
+      const int o = (PAR == DP_COS) ? k : (L - k) % L;
+      oa[(2 * qq) * L + o] = y.x;
+      oa[(2 * qq + 1) * L + o] = y.y;
     }
   } else {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = TL::template pos_in<inv_first<L>()>(tc, e);
-      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : mk(a[L - k], b[L - k]);
-      v[e] = dct3_in(mk(a[k], b[k]), pn, qtw[k]);
+      if constexpr (PAR == DP_COS) {
+        const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : mk(a[L - k], b[L - k]);
+        v[e] = dct3_in(mk(a[k], b[k]), pn, qtw[k]);
+      } else {
+        const int m = (L - k) % L;
+        v[e] = dct3_par<PAR>(mk(a[k], b[k]), mk(a[m], b[m]), k, L, kq, qtw[k]);
+      }
     }
     InvFFTA<T, L, AD>::run(v, S, q, tc, twl);
     T* oa = out + pa * L;
