@@ -147,6 +147,39 @@ def run_advected(steps):
             "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
 
 
+def run_neumann_advected(steps):
+    """Advection-coupled Neumann box 256x256x384: the DCT grid of cosNoAdvMorph with the
+    sine/cosine Stokes projection, mu grad psi force, eta=1, eps=0.5, q0=1, gamma=1, dt=0.02,
+    8 points per wavelength, hyperboloid IC; FP32-mixed."""
+    from smectic3d_amd import Solver
+    from smectic3d_amd import _abi as A
+    nx, ny, nz = 256, 256, 384
+    dx = 2 * math.pi / 8
+    cfg = A.make_config(nx, ny, nz, lx=nx * dx, ly=ny * dx, lz=nz * dx, bc=A.SMX_BC_NEUMANN, eps=0.5, q0=1.0,
+                        beta=0.0, gamma=1.0, dt=0.02, nl_model=A.SMX_NL_CUBIC, advect=1, eta=1.0, rho=1.0,
+                        force_form=A.SMX_FORCE_MU, precision=A.SMX_FP32_MIXED)
+    c = (np.arange(nx) - nx // 2) * dx
+    cz = (np.arange(nz) - nz // 2) * dx
+    r2 = np.maximum(c[:, None, None] ** 2 + c[None, :, None] ** 2 - cz[None, None, :] ** 2, 0.0)
+    with Solver(cfg) as s:
+        s.set_psi(0.4 * np.cos(np.sqrt(r2 + 4.0)))
+        s.step(5)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t1 = time.perf_counter()
+        e = s.energy()
+        ks = s.profile_step(10)
+    S = nx * ny * nz * 4  # one real FP32 stage array
+    sps = steps / (t1 - t0)
+    alg = 20 * S  # minimal DAG of the advected step, 8 S_r + 12 S_c, with S_r = S_c = S on the DCT grid
+    return {"steps": steps, "steps_per_s": sps, "energy": e,
+            "step_roofline": {"alg_bytes_per_step": alg, "achieved_GBs": alg * sps / 1e9,
+                              "frac": alg * sps / 1e9 / HBM_PEAK_GBS},
+            "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
+
+
 def batch_cfgs():
     """24 independent periodic boxes, sizes from {64^3, 96^3, 128^3} by seed, eps ~ U[0.2,0.6],
     q0 ~ U[0.8,1.2], gamma=1, dt=0.05 (FP32-mixed)."""
@@ -282,7 +315,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the Neumann/advected/batch/adjoint legs")
     ap.add_argument("--no-tune", action="store_true", help="default tile schedules (no autotuning)")
-    ap.add_argument("--leg", choices=["neumann", "advected", "batch", "adjoint"],
+    ap.add_argument("--leg", choices=["neumann", "advected", "neumann_advected", "batch", "adjoint"],
                     help="run only this secondary workload (with --leg-steps steps) and print its result")
     ap.add_argument("--leg-steps", type=int, default=None)
     ap.add_argument("--schedules-from", default=None,
@@ -297,6 +330,7 @@ def main():
 
     if a.leg:  # one secondary workload alone (profiling runs)
         runs = {"neumann": (run_neumann, 4000), "advected": (run_advected, 1000),
+                "neumann_advected": (run_neumann_advected, 500),
                 "batch": (lambda k: run_batch(k, tune=not a.no_tune), 500), "adjoint": (lambda k: run_adjoint(k, 50), 2000)}
         fn, default_steps = runs[a.leg]
         print(json.dumps({"leg": a.leg, "result": fn(a.leg_steps or default_steps)}, default=str))
@@ -373,6 +407,8 @@ def main():
         line["neumann_surftrack_256x256x384_fp32"] = run_neumann(4000)
         log("advected ...")
         line["advected_256_fp32"] = run_advected(1000)
+        log("advected neumann ...")
+        line["advected_neumann_256x256x384_fp32"] = run_neumann_advected(500)
         log("adjoint ...")
         line["adjoint_256_fp32"] = run_adjoint(2000, 50)
     if not a.no_cpu:

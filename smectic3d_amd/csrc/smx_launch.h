@@ -422,45 +422,61 @@ struct AdvectOps {
   static int zpass_blocks(const PParams<T>& p) { return (int)(p.nxy / NCZ); }
 };
 
-// Kernels of the advected Neumann step (smx_neuadv.h).  The z kernels take the Neumann z
-// pass's tiles, so their FP64 partials match its count; the Stokes pass holds three LDS
-// tiles, 64-byte rows.
+// Kernels of the advected Neumann step (smx_neuadv.h): 128-byte rows (x kernels where ny
+// allows it); the z kernels take the Neumann z pass's tiles.
 template <typename T>
 struct NeuAdvOps {
-  static constexpr int NCX = 16;
-  static constexpr int NCS = sizeof(T) == 4 ? 16 : 8;
-  static constexpr int NCZ = neumann_nc<T>();
+  static constexpr int NCW = neumann_nc<T>();  // 128-byte rows
+  template <int L> static constexpr int ncz() { return NCW; }
+  template <int L, int NC>
+  static hipError_t xadv_launch(int mode, const T* in, T* rbuf, const T* psi0, T* oa, T* ox, T* os,
+                                const PParams<T>& p, const cplx<T>* qtw, hipStream_t st) {
+    const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh), block((NC / 2) * (L / Plan<L>::E));
+    if (mode == X_STEP)
+      SMX_LAUNCH((k_dctx_adv<T, L, NC, X_STEP>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p, qtw);
+    else if (mode == X_BOOT)
+      SMX_LAUNCH((k_dctx_adv<T, L, NC, X_BOOT>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p, qtw);
+    else
+      return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   static hipError_t xpass_adv(int mode, const T* in, T* rbuf, const T* psi0, T* oa, T* ox, T* os,
                               const PParams<T>& p, const cplx<T>* qtw, hipStream_t st) {
-    if (p.ny % NCX) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(p.ny / NCX), (unsigned)p.nzh);
+    if (p.ny % 16) return hipErrorInvalidValue;
+    const bool wide = (p.ny % NCW == 0);
 #define SMX_CASE(L)                                                                                      \
   if (p.nx == L) {                                                                                       \
-    const dim3 block((NCX / 2) * (L / Plan<L>::E));                                                      \
-    if (mode == X_STEP)                                                                                  \
-      SMX_LAUNCH((k_dctx_adv<T, L, NCX, X_STEP>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p, qtw); \
-    else if (mode == X_BOOT)                                                                             \
-      SMX_LAUNCH((k_dctx_adv<T, L, NCX, X_BOOT>), grid, block, 0, st, in, rbuf, psi0, oa, ox, os, p, qtw); \
-    else                                                                                                 \
-      return hipErrorInvalidValue;                                                                       \
-    return hipGetLastError();                                                                            \
+    if (wide) return xadv_launch<L, NCW>(mode, in, rbuf, psi0, oa, ox, os, p, qtw, st);                  \
+    return xadv_launch<L, 16>(mode, in, rbuf, psi0, oa, ox, os, p, qtw, st);                             \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
     return hipErrorInvalidValue;
   }
+  // Stokes pass: the E = 8 plan where there is one (it holds every component's spectrum of its
+  // slots in registers), blocks of at most 256 threads
+  template <int L> static constexpr int vs() { return HasPlanV<L, 1>::value ? 1 : 0; }
+  template <int L> static constexpr int ncs() { return std::min(NCW, 512 / (L / PlanV<L, vs<L>()>::E)); }
+  template <int L, int NC>
+  static hipError_t stokes_launch(T* s0, T* s1, T* s2, const AParams<T>& ap, const cplx<T>* qtw, T qnx, T qny,
+                                  T qnz, hipStream_t st) {
+    constexpr int V = vs<L>();
+    static_assert(((size_t)L * (NC / 2 + 1) + 2 * L) * sizeof(cplx<T>) + L * sizeof(T) <= kMaxLds,
+                  "Stokes tile exceeds LDS");
+    const dim3 grid((unsigned)(ap.p.ny / NC), (unsigned)ap.p.nzh);
+    SMX_LAUNCH((k_dctx_stokes<T, L, NC, V>), grid, dim3((NC / 2) * (L / PlanV<L, V>::E)), 0, st, s0, s1, s2, ap, qtw,
+               qnx, qny, qnz);
+    return hipGetLastError();
+  }
   static hipError_t stokes(T* s0, T* s1, T* s2, const AParams<T>& ap, const cplx<T>* qtw, T qnx, T qny, T qnz,
                            hipStream_t st) {
     const PParams<T>& p = ap.p;
-    if (p.ny % NCS) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(p.ny / NCS), (unsigned)p.nzh);
-#define SMX_CASE(L)                                                                                     \
-  if (p.nx == L) {                                                                                      \
-    static_assert((3 * (size_t)L * (NCS / 2 + 1) + 2 * L) * sizeof(cplx<T>) + L * sizeof(T) <= kMaxLds, \
-                  "Stokes tiles exceed LDS");                                                            \
-    SMX_LAUNCH((k_dctx_stokes<T, L, NCS>), grid, dim3((NCS / 2) * (L / Plan<L>::E)), 0, st, s0, s1, s2, ap, qtw, \
-               qnx, qny, qnz);                                                                          \
-    return hipGetLastError();                                                                           \
+    if (p.ny % 16) return hipErrorInvalidValue;
+#define SMX_CASE(L)                                                                           \
+  if (p.nx == L) {                                                                            \
+    constexpr int NC = ncs<L>();                                                              \
+    if (p.ny % NC == 0) return stokes_launch<L, NC>(s0, s1, s2, ap, qtw, qnx, qny, qnz, st);  \
+    return stokes_launch<L, 16>(s0, s1, s2, ap, qtw, qnx, qny, qnz, st);                      \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
@@ -469,12 +485,12 @@ struct NeuAdvOps {
   static hipError_t zforce(const T* bpsi, const T* bx, const T* by, const T* bs, T* o0, T* o1, T* o2,
                            const AParams<T>& ap, const cplx<T>* qtw, hipStream_t st) {
     const PParams<T>& p = ap.p;
-    if (p.nxy % NCZ) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(p.nxy / NCZ));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    SMX_LAUNCH((k_dctz_force<T, L, NCZ>), grid, dim3((NCZ / 2) * (L / Plan<L>::E)), 0, st, bpsi, bx, by, bs, o0, \
-               o1, o2, ap, qtw);                                                                        \
+    constexpr int NC = ncz<L>();                                                                        \
+    if (p.nxy % NC) return hipErrorInvalidValue;                                                        \
+    SMX_LAUNCH((k_dctz_force<T, L, NC>), grid_of(p, NC), dim3((NC / 2) * (L / Plan<L>::E)), 0, st, bpsi, bx, by, \
+               bs, o0, o1, o2, ap, qtw);                                                                \
     return hipGetLastError();                                                                           \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
@@ -484,20 +500,27 @@ struct NeuAdvOps {
   static hipError_t zadvnl(const T* u0, const T* u1, const T* u2, const T* bpsi, const T* bx, const T* by, T* nout,
                            const AParams<T>& ap, const cplx<T>* qtw, hipStream_t st) {
     const PParams<T>& p = ap.p;
-    if (p.nxy % NCZ) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(p.nxy / NCZ));
 #define SMX_CASE(L)                                                                                     \
   if (p.nz == L) {                                                                                      \
-    SMX_LAUNCH((k_dctz_advnl<T, L, NCZ>), grid, dim3((NCZ / 2) * (L / Plan<L>::E)), 0, st, u0, u1, u2, bpsi, bx, \
-               by, nout, ap, qtw);                                                                      \
+    constexpr int NC = ncz<L>();                                                                        \
+    if (p.nxy % NC) return hipErrorInvalidValue;                                                        \
+    SMX_LAUNCH((k_dctz_advnl<T, L, NC>), grid_of(p, NC), dim3((NC / 2) * (L / Plan<L>::E)), 0, st, u0, u1, u2,   \
+               bpsi, bx, by, nout, ap, qtw);                                                            \
     return hipGetLastError();                                                                           \
   }
     SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
     return hipErrorInvalidValue;
   }
+  static dim3 grid_of(const PParams<T>& p, int nc) { return dim3((unsigned)(p.nxy / nc)); }
   // partial sums written by zadvnl: one per z tile
-  static int zpass_blocks(const PParams<T>& p) { return (int)(p.nxy / NCZ); }
+  static int zpass_blocks(const PParams<T>& p) {
+#define SMX_CASE(L) \
+  if (p.nz == L) return (int)(p.nxy / ncz<L>());
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return 0;
+  }
 };
 
 }  // namespace smx

@@ -42,32 +42,35 @@ struct DZ {
   static constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
   static constexpr uint32_t CB = sizeof(cplx<T>);
 
-  // rows k and (L-k) mod L of this thread's column pair, k = pos_in<inv_first>(tc, e)
-  __device__ static void load(const T* in, int64_t f0, uint32_t rowb, int q, int tc, cplx<T>* rk, cplx<T>* rm) {
+  // One field's rows of the tile, staged through LDS, to the FFT inputs of the inverse
+  // transform along z (parity par, uniform): each thread loads its rows k = pos_in<inv_first>
+  // once and reads the partner rows (L-k) mod L from the tile.  With DZ2, v2 also receives the
+  // inputs of the derivative d/dz (DP_DSIN) of the same (cosine) field.  kq, qw: LDS tables.
+  template <bool DZ2 = false>
+  __device__ static void stage_form(const T* in, int64_t f0, uint32_t rowb, int q, int tc, int par, const T* kq,
+                                    const cplx<T>* qw, cplx<T>* F, cplx<T>* v, cplx<T>* v2 = nullptr) {
     const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in + f0);
+    cplx<T> rk[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      rk[e] = ld_off(src, (uint32_t)Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e) * rowb +
+                              (uint32_t)q * CB);
+    __syncthreads();  // the tile's earlier reads are done
+#pragma unroll
+    for (int e = 0; e < E; ++e) F[Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e) * FP + q] = rk[e];
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
-      rk[e] = ld_off(src, (uint32_t)k * rowb + (uint32_t)q * CB);
-      rm[e] = ld_off(src, (uint32_t)((L - k) % L) * rowb + (uint32_t)q * CB);
+      const cplx<T> rm = F[((L - k) % L) * FP + q];
+      v[e] = (par == DP_COS) ? dct3_par<DP_COS>(rk[e], rm, k, L, kq, qw[k])
+                             : (par == DP_SIN ? dct3_par<DP_SIN>(rk[e], rm, k, L, kq, qw[k])
+                                              : dct3_par<DP_DSIN>(rk[e], rm, k, L, kq, qw[k]));
+      if constexpr (DZ2) v2[e] = dct3_par<DP_DSIN>(rk[e], rm, k, L, kq, qw[k]);
     }
   }
-  // inverse transform along z (parity PAR) straight from HBM; kq, qw: LDS tables
-  template <int PAR>
-  __device__ static void c2r(const T* in, int64_t f0, uint32_t rowb, int q, int tc, const T* kq,
-                             const cplx<T>* qw, cplx<T>* F, const cplx<T>* tw, cplx<T>* v) {
-    cplx<T> rk[E], rm[E];
-    load(in, f0, rowb, q, tc, rk, rm);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
-      v[e] = dct3_par<PAR>(rk[e], rm[e], k, L, kq, qw[k]);
-    }
-    InvFFT<T, L, FP>::run(v, F, q, tc, tw);
-  }
-  // forward transform along z (parity PAR) of the pairs in v and store of the spectra
-  template <int PAR>
-  __device__ static void r2c_store(cplx<T>* v, T* out, int64_t f0, uint32_t rowb, int q, int tc, int tid,
+  // forward transform along z (parity par, uniform) of the pairs in v and store of the spectra
+  __device__ static void r2c_store(cplx<T>* v, T* out, int par, int64_t f0, uint32_t rowb, int q, int tc, int tid,
                                    const cplx<T>* qw, cplx<T>* F, const cplx<T>* tw) {
     FwdFFT<T, L, FP>::run(v, F, q, tc, tw);
     __syncthreads();
@@ -76,7 +79,7 @@ struct DZ {
     cplx<T>* dst = reinterpret_cast<cplx<T>*>(out + f0);
     for (int i = tid; i < L * NPR; i += NT) {
       const int k = i / NPR, qq = i - k * NPR;
-      const int s = (PAR == DP_COS) ? k : (L - k) % L;
+      const int s = (par == DP_COS) ? k : (L - k) % L;
       st_off(dst, (uint32_t)s * rowb + (uint32_t)qq * CB, dct2_out(F[k * FP + qq], F[((L - k) % L) * FP + qq], qw[k]));
     }
   }
@@ -84,11 +87,11 @@ struct DZ {
 
 // LDS tables of a z kernel: twiddles, quarter-wave twiddles and k_z, filled before any use.
 #define SMX_DZ_TABLES(NT_)                                  \
-  __shared__ cplx<T> TW[L], QW[L];                          \
-  __shared__ T KZ[L];                                       \
+  __shared__ cplx<T> TW[L], QWl[L];                         \
+  __shared__ T KZl[L];                                      \
   const cplx<T>* twl = tw_lds<L, NT_>(TW, p.twz, tid);      \
-  tw_lds<L, NT_>(QW, qtw, tid);                             \
-  for (int i = tid; i < L; i += NT_) KZ[i] = p.kz[i];       \
+  tw_lds<L, NT_>(QWl, qtw, tid);                            \
+  for (int i = tid; i < L; i += NT_) KZl[i] = p.kz[i];      \
   __syncthreads();
 
 // Force S = f grad psi (pf3d.cpp:417-425), f = psiSH or mu = psiSH - eps psi - N(psi).
@@ -107,26 +110,40 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_force(cons
   SMX_DZ_TABLES(Z::NT)
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
-  cplx<T> v[E], f[E];
-  Z::template c2r<DP_COS>(bs, f0, rowb, q, tc, KZ, QW, F, twl, f);  // psiSH
-  if (ap.force == F_MU) {
-    Z::template c2r<DP_COS>(bpsi, f0, rowb, q, tc, KZ, QW, F, twl, v);
+  // Fields in turn through one inverse and one forward transform instance (a straight-line
+  // kernel with one instance per field holds more registers and code):
+  //   0 psiSH -> f;  1 psi -> f = mu (F_MU only; also forms d_z psi's inputs);  2 d_z psi -> S2;
+  //   3 d_x psi -> S0;  4 d_y psi -> S1.
+  const bool mu = (ap.force == F_MU);
+  cplx<T> w[E], f[E], g[E];
+#pragma unroll 1
+  for (int i = 0; i < 5; ++i) {
+    if (i == 1 && !mu) continue;
+    const int z = opaque_zero();  // tables reloaded per field (see opaque_zero)
+    const cplx<T>* tw = twl + z;
+    const cplx<T>* QW = QWl + z;
+    const T* KZ = KZl + z;
+    if (i == 0) Z::stage_form(bs, f0, rowb, q, tc, DP_COS, KZ, QW, F, w);
+    else if (i == 1) Z::template stage_form<true>(bpsi, f0, rowb, q, tc, DP_COS, KZ, QW, F, w, g);
+    else if (i == 2 && mu) {
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      f[e] = mk(f[e].x - p.eps * v[e].x - nonlin(p, v[e].x), f[e].y - p.eps * v[e].y - nonlin(p, v[e].y));
+      for (int e = 0; e < E; ++e) w[e] = g[e];
+    } else Z::stage_form(i == 2 ? bpsi : (i == 3 ? bx : by), f0, rowb, q, tc, i == 2 ? DP_DSIN : DP_COS, KZ, QW, F,
+                         w);
+    InvFFT<T, L, Z::FP>::run(w, F, q, tc, tw);
+    if (i == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) f[e] = w[e];
+    } else if (i == 1) {
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        f[e] = mk(f[e].x - p.eps * w[e].x - nonlin(p, w[e].x), f[e].y - p.eps * w[e].y - nonlin(p, w[e].y));
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) w[e] = mk(f[e].x * w[e].x, f[e].y * w[e].y);
+      Z::r2c_store(w, i == 2 ? o2 : (i == 3 ? o0 : o1), i == 2 ? DP_SIN : DP_COS, f0, rowb, q, tc, tid, QW, F, tw);
+    }
   }
-  Z::template c2r<DP_COS>(bx, f0, rowb, q, tc, KZ, QW, F, twl, v);
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = mk(f[e].x * v[e].x, f[e].y * v[e].y);
-  Z::template r2c_store<DP_COS>(v, o0, f0, rowb, q, tc, tid, QW, F, twl);
-  Z::template c2r<DP_COS>(by, f0, rowb, q, tc, KZ, QW, F, twl, v);
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = mk(f[e].x * v[e].x, f[e].y * v[e].y);
-  Z::template r2c_store<DP_COS>(v, o1, f0, rowb, q, tc, tid, QW, F, twl);
-  Z::template c2r<DP_DSIN>(bpsi, f0, rowb, q, tc, KZ, QW, F, twl, v);  // d_z psi, folded along z
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = mk(f[e].x * v[e].x, f[e].y * v[e].y);
-  Z::template r2c_store<DP_SIN>(v, o2, f0, rowb, q, tc, tid, QW, F, twl);
 }
 
 // N = N(psi) - U.grad(psi) (pf3d.cpp:455-465), then DCT-II along z; U2 and d_z psi are sine
@@ -145,24 +162,38 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
   SMX_DZ_TABLES(Z::NT)
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
-  cplx<T> v[E], u[E], nacc[E];
-  Z::template c2r<DP_COS>(bpsi, f0, rowb, q, tc, KZ, QW, F, twl, v);
+  // Fields in turn through one inverse transform instance:
+  //   0 psi -> N(psi) (also forms d_z psi's inputs);  1 d_z psi;  2 U2;  3 U0;  4 d_x psi;
+  //   5 U1;  6 d_y psi,
+  // subtracting U_a d_a psi as each pair completes.
+  cplx<T> w[E], aux[E], nacc[E];
+#pragma unroll 1
+  for (int i = 0; i < 7; ++i) {
+    const int z = opaque_zero();  // tables reloaded per field (see opaque_zero)
+    const cplx<T>* tw = twl + z;
+    const cplx<T>* QW = QWl + z;
+    const T* KZ = KZl + z;
+    if (i == 0) Z::template stage_form<true>(bpsi, f0, rowb, q, tc, DP_COS, KZ, QW, F, w, aux);
+    else if (i == 1) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) nacc[e] = mk(nonlin(p, v[e].x), nonlin(p, v[e].y));
-  Z::template c2r<DP_COS>(u0, f0, rowb, q, tc, KZ, QW, F, twl, u);
-  Z::template c2r<DP_COS>(bx, f0, rowb, q, tc, KZ, QW, F, twl, v);
+      for (int e = 0; e < E; ++e) w[e] = aux[e];
+    } else
+      Z::stage_form(i == 2 ? u2 : (i == 3 ? u0 : (i == 4 ? bx : (i == 5 ? u1 : by))), f0, rowb, q, tc,
+                    i == 2 ? DP_SIN : DP_COS, KZ, QW, F, w);
+    InvFFT<T, L, Z::FP>::run(w, F, q, tc, tw);
+    if (i == 0) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) nacc[e] = mk(nacc[e].x - u[e].x * v[e].x, nacc[e].y - u[e].y * v[e].y);
-  Z::template c2r<DP_COS>(u1, f0, rowb, q, tc, KZ, QW, F, twl, u);
-  Z::template c2r<DP_COS>(by, f0, rowb, q, tc, KZ, QW, F, twl, v);
+      for (int e = 0; e < E; ++e) nacc[e] = mk(nonlin(p, w[e].x), nonlin(p, w[e].y));
+    } else if (i & 1) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) nacc[e] = mk(nacc[e].x - u[e].x * v[e].x, nacc[e].y - u[e].y * v[e].y);
-  Z::template c2r<DP_SIN>(u2, f0, rowb, q, tc, KZ, QW, F, twl, u);
-  Z::template c2r<DP_DSIN>(bpsi, f0, rowb, q, tc, KZ, QW, F, twl, v);
+      for (int e = 0; e < E; ++e) aux[e] = w[e];
+    } else {
 #pragma unroll
-  for (int e = 0; e < E; ++e) nacc[e] = mk(nacc[e].x - u[e].x * v[e].x, nacc[e].y - u[e].y * v[e].y);
+      for (int e = 0; e < E; ++e) nacc[e] = mk(nacc[e].x - aux[e].x * w[e].x, nacc[e].y - aux[e].y * w[e].y);
+    }
+  }
   if (p.partial != nullptr) wave_partial<E, Z::NT>(nacc, red, tid);
-  Z::template r2c_store<DP_COS>(nacc, nout, f0, rowb, q, tc, tid, QW, F, twl);  // its barriers order red[]
+  Z::r2c_store(nacc, nout, DP_COS, f0, rowb, q, tc, tid, QWl, F, twl);  // its barriers order red[]
   if (p.partial != nullptr) store_partial<Z::NT>(red, p.partial + blockIdx.x, tid);
 }
 
@@ -170,113 +201,143 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
 
 // Stokes x pass: forward x of S0 (sine along x), S1, S2 (cosine), the projection per slot
 // triple, inverse x of U0 (sine), U1, U2; in place per tile.  Tile: plane kz, columns
-// y0 .. y0+NC-1 (pairs of adjacent ky), one LDS tile per component.  qn: pi/d per axis.
-template <typename T, int L, int NC>
-__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctx_stokes(T* s0, T* s1, T* s2, AParams<T> ap,
+// y0 .. y0+NC-1 (pairs of adjacent ky); V: plan variant.  One LDS tile serves the six transforms: each
+// component's spectrum goes from the tile to the registers of the thread that owns its slots
+// (k, L-k), where the projection takes all three.  qn: pi/d per axis.
+template <typename T, int L, int NC, int V = 0>
+__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_dctx_stokes(T* s0, T* s1, T* s2, AParams<T> ap,
                                                                              const cplx<T>* __restrict__ qtw,
                                                                              T qnx, T qny, T qnz) {
-  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
+  constexpr int E = PlanV<L, V>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
+  constexpr int NIT = (L / 2 + TPC) / TPC;
   constexpr uint32_t CB = sizeof(cplx<T>);
-  __shared__ cplx<T> F[3][L * FP];
+  __shared__ cplx<T> F[L * FP];
   const PParams<T>& p = ap.p;
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
-  __shared__ cplx<T> TW[L], QW[L];
+  __shared__ cplx<T> TW[L], QWl[L];
   __shared__ T KX[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
-  tw_lds<L, NT>(QW, qtw, tid);
+  tw_lds<L, NT>(QWl, qtw, tid);
   for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy + y0;
   const uint32_t rowb = (uint32_t)p.ny * sizeof(T);
   cplx<T>* arr[3] = {reinterpret_cast<cplx<T>*>(s0 + plane), reinterpret_cast<cplx<T>*>(s1 + plane),
                      reinterpret_cast<cplx<T>*>(s2 + plane)};
-  cplx<T> v[E];
-#pragma unroll 1
-  for (int a = 0; a < 3; ++a) {
+  const T kzv = p.kz[kz], kya = p.ky[y0 + 2 * q], kyb = p.ky[y0 + 2 * q + 1];
+  cplx<T>* const a0 = arr[0];
+  cplx<T>* const a1 = arr[1];
+  cplx<T>* const a2 = arr[2];
+  auto comp = [&](int a) { return a == 0 ? a0 : (a == 1 ? a1 : a2); };
+  auto load = [&](int a, cplx<T>* w) {
+    const cplx<T>* src = comp(a);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int s = mk_src(Tile<T, L, E, FP>::template pos_in<fwd_first<L>()>(tc, e), L);
-      v[e] = ld_off<cplx<T>>(arr[a], (uint32_t)s * rowb + (uint32_t)q * CB);
+      const int s = mk_src(Tile<T, L, E, FP>::template pos_in<fwd_first<L, V>()>(tc, e), L);
+      w[e] = ld_off<cplx<T>>(src, (uint32_t)s * rowb + (uint32_t)q * CB);
     }
-    FwdFFT<T, L, FP>::run(v, F[a], q, tc, twl);
+  };
+  // spectra at slots k = tc + it*TPC and L-k, per component; the components go through one
+  // transform instance each way (loops over a; one instance per component would hold more
+  // registers and code)
+  cplx<T> v[E];
+  cplx<T> fk0[NIT], fm0[NIT], fk1[NIT], fm1[NIT], fk2[NIT], fm2[NIT];
+#pragma unroll 1
+  for (int a = 0; a < 3; ++a) {
+    const int z = opaque_zero();  // tables reloaded per component (see opaque_zero)
+    const cplx<T>* tw = twl + z;
+    const cplx<T>* QW = QWl + z;
+    load(a, v);
+    FwdFFT<T, L, FP, V>::run(v, F, q, tc, tw);
+    __syncthreads();  // the previous component's reads of the tile are done
+    Tile<T, L, E, FP>::template store<fwd_last<L, V>()>(v, F, q, tc);
     __syncthreads();
-    Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F[a], q, tc);
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int k = tc + it * TPC;
+      const int km = (L - k) % L;
+      const bool in = k <= L / 2, two = in && k != 0 && 2 * k != L;
+      const cplx<T> Zk = F[(in ? k : 0) * FP + q], Zm = F[(in ? km : 0) * FP + q];
+      const cplx<T> yk = in ? dct2_out(Zk, Zm, QW[in ? k : 0]) : mk<T>(0, 0);
+      const cplx<T> ym = two ? dct2_out(Zm, Zk, QW[km]) : mk<T>(0, 0);
+      if (a == 0) {  // sine along x: slot k holds the DCT-II output at (L-k) mod L
+        fk0[it] = two ? ym : yk;
+        fm0[it] = yk;
+      } else if (a == 1) {
+        fk1[it] = yk;
+        fm1[it] = ym;
+      } else {
+        fk2[it] = yk;
+        fm2[it] = ym;
+      }
+    }
   }
-  __syncthreads();
-  const T kzv = p.kz[kz], kya = p.ky[y0 + 2 * q], kyb = p.ky[y0 + 2 * q + 1];
   const bool ya0 = (y0 + 2 * q == 0), z0 = (kz == 0);  // column b never has ky slot 0
   const T cs = p.scale * ap.cm;
-  // U of one column at x slot sx: f = (S0, S1, S2) there, qy / y0s: its k_y and ky-slot-0 flag
-  auto proj1 = [&](T fx, T fy, T fz, int sx, T qy, bool y0s, T& ux, T& uy, T& uz) {
+  // U of one column at x slot sx from f = (S0, S1, S2) there; qy / y0s: its k_y, ky-slot-0 flag
+  struct U3 { T x, y, z; };
+  auto proj1 = [&](T fx, T fy, T fz, int sx, T qy, bool y0s) -> U3 {
     const T qx = KX[sx];
     const T k2 = qx * qx + qy * qy + kzv * kzv;
     const T dot = qx * fx + qy * fy + kzv * fz;
     const T ik2 = (k2 > (T)0) ? recip(k2) : (T)0;
     const T c = cs * ik2;
-    if (sx != 0) ux = c * (fx - qx * dot * ik2);
-    else {
-      const T ia = recip(k2 + qnx * qnx);
-      ux = cs * fx * k2 * ia * ia;
-    }
-    if (!y0s) uy = c * (fy - qy * dot * ik2);
-    else {
-      const T ia = recip(k2 + qny * qny);
-      uy = cs * fy * k2 * ia * ia;
-    }
-    if (!z0) uz = c * (fz - kzv * dot * ik2);
-    else {
-      const T ia = recip(k2 + qnz * qnz);
-      uz = cs * fz * k2 * ia * ia;
-    }
+    const T iax = recip(k2 + qnx * qnx), iay = recip(k2 + qny * qny), iaz = recip(k2 + qnz * qnz);
+    U3 u;
+    u.x = (sx != 0) ? c * (fx - qx * dot * ik2) : cs * fx * k2 * iax * iax;
+    u.y = (!y0s) ? c * (fy - qy * dot * ik2) : cs * fy * k2 * iay * iay;
+    u.z = (!z0) ? c * (fz - kzv * dot * ik2) : cs * fz * k2 * iaz * iaz;
+    return u;
   };
-  auto proj = [&](const cplx<T>* f, int sx, cplx<T>* u) {
-    proj1(f[0].x, f[1].x, f[2].x, sx, kya, ya0, u[0].x, u[1].x, u[2].x);
-    proj1(f[0].y, f[1].y, f[2].y, sx, kyb, false, u[0].y, u[1].y, u[2].y);
-  };
-  // one item: slots k and L-k of pair q, all three components
-#pragma unroll 1
-  for (int k = tc; k <= L / 2; k += TPC) {
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int k = tc + it * TPC;
     const int km = (L - k) % L;
-    const bool two = (k != 0 && 2 * k != L);
-    cplx<T> fk[3], fm[3], uk[3], um[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const cplx<T> Zk = F[a][k * FP + q], Zm = F[a][km * FP + q];
-      const cplx<T> yk = dct2_out(Zk, Zm, QW[k]);
-      const cplx<T> ym = two ? dct2_out(Zm, Zk, QW[km]) : mk<T>(0, 0);
-      if (a == 0) {  // sine along x: slot k holds the DCT-II output at (L-k) mod L
-        fk[0] = two ? ym : yk;
-        fm[0] = yk;
-      } else {
-        fk[a] = yk;
-        fm[a] = ym;
-      }
-    }
-    proj(fk, k, uk);
-    if (two) proj(fm, km, um);
-    // DCT-III inputs: cosine components from slots k, L-k; the sine one from L-k, k
-#pragma unroll
-    for (int a = 1; a < 3; ++a) {
-      const cplx<T> pm = two ? um[a] : (k == 0 ? mk<T>(0, 0) : uk[a]);
-      F[a][k * FP + q] = dct3_in(uk[a], pm, QW[k]);
-      if (two) F[a][km * FP + q] = dct3_in(um[a], uk[a], QW[km]);
-    }
-    if (k == 0) F[0][q] = dct3_in(uk[0], mk<T>(0, 0), QW[0]);
-    else if (!two) F[0][k * FP + q] = dct3_in(uk[0], uk[0], QW[k]);
-    else {
-      F[0][k * FP + q] = dct3_in(um[0], uk[0], QW[k]);
-      F[0][km * FP + q] = dct3_in(uk[0], um[0], QW[km]);
-    }
+    const bool in = k <= L / 2;
+    const int kk = in ? k : 0, kmm = in ? km : 0;
+    const U3 a = proj1(fk0[it].x, fk1[it].x, fk2[it].x, kk, kya, ya0);
+    const U3 b = proj1(fk0[it].y, fk1[it].y, fk2[it].y, kk, kyb, false);
+    const U3 c = proj1(fm0[it].x, fm1[it].x, fm2[it].x, kmm, kya, ya0);
+    const U3 d = proj1(fm0[it].y, fm1[it].y, fm2[it].y, kmm, kyb, false);
+    fk0[it] = mk(a.x, b.x); fk1[it] = mk(a.y, b.y); fk2[it] = mk(a.z, b.z);
+    fm0[it] = mk(c.x, d.x); fm1[it] = mk(c.y, d.y); fm2[it] = mk(c.z, d.z);
   }
-  __syncthreads();
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F[a], q, tc);
-    InvFFT<T, L, FP>::run(v, F[a], q, tc, twl);
+    const int z = opaque_zero();
+    const cplx<T>* tw = twl + z;
+    const cplx<T>* QW = QWl + z;
+    __syncthreads();  // the last reads of the tile (items or the previous inverse) are done
+    // DCT-III inputs: cosine components from slots k, L-k; the sine one from L-k, k.  One call
+    // per component, so that each reads its own arrays with constant indexes.
+    auto inputs = [&](const cplx<T>* fk, const cplx<T>* fm, bool sine) {
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int k = tc + it * TPC;
+        if (k > L / 2) continue;
+        const int km = (L - k) % L;
+        const bool two = (k != 0 && 2 * k != L);
+        const cplx<T> uk = fk[it], um = fm[it];
+        if (!sine || k == 0 || !two) {  // the sine one at k = 0 (slot 0 alone) and L/2 as cosine
+          F[k * FP + q] = dct3_in(uk, two ? um : (k == 0 ? mk<T>(0, 0) : uk), QW[k]);
+          if (two) F[km * FP + q] = dct3_in(um, uk, QW[km]);
+        } else {
+          F[k * FP + q] = dct3_in(um, uk, QW[k]);
+          F[km * FP + q] = dct3_in(uk, um, QW[km]);
+        }
+      }
+    };
+    if (a == 0) inputs(fk0, fm0, true);
+    else if (a == 1) inputs(fk1, fm1, false);
+    else inputs(fk2, fm2, false);
+    __syncthreads();
+    Tile<T, L, E, FP>::template load<inv_first<L, V>()>(v, F, q, tc);
+    InvFFT<T, L, FP, V>::run(v, F, q, tc, tw);
+    cplx<T>* dst = comp(a);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int d = mk_src(Tile<T, L, E, FP>::template pos_in<inv_last<L>()>(tc, e), L);
-      st_off(arr[a], (uint32_t)d * rowb + (uint32_t)q * CB, v[e]);
+      const int d = mk_src(Tile<T, L, E, FP>::template pos_in<inv_last<L, V>()>(tc, e), L);
+      st_off(dst, (uint32_t)d * rowb + (uint32_t)q * CB, v[e]);
     }
   }
 }
@@ -392,6 +453,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctx_adv(const 
   T* outs[3] = {outA + plane, outX + plane, outS + plane};
 #pragma unroll 1
   for (int o = 0; o < 3; ++o) {
+    const cplx<T>* tw = twl + opaque_zero();  // twiddles reloaded per output (see opaque_zero)
     if (o > 0) {
       __syncthreads();  // the previous inverse transform's LDS reads are done
 #pragma unroll
@@ -422,7 +484,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctx_adv(const 
     }
     __syncthreads();
     Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F, q, tc);
-    InvFFT<T, L, FP>::run(v, F, q, tc, twl);
+    InvFFT<T, L, FP>::run(v, F, q, tc, tw);
     cplx<T>* dst = reinterpret_cast<cplx<T>*>(outs[o]);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
