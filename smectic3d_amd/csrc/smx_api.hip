@@ -315,7 +315,6 @@ static const char* slot_name(const smx_handle* h, int s) {
   return h->neumann ? kSlotNamesN[s] : kSlotNamesP[s];
 }
 
-// One advected periodic step (pf3d.cpp:395-485): 16 launches, in place per tile.
 // Per-kernel timing of a step chain: slot k's kernel is launched with the event pair
 // (ev[2k], ev[2k+1]) (hipExtLaunchKernelGGL stamps them with the kernel's start and end).
 // ev == nullptr: no timing.  Disarms on scope exit.
@@ -330,6 +329,7 @@ struct SlotTimer {
   }
 };
 
+// One advected periodic step (pf3d.cpp:395-485): 15 launches, in place per tile.
 template <typename T>
 static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
   using Ops = PeriodicOps<T>;
