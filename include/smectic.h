@@ -125,8 +125,9 @@ int smx_energy(smx_handle *h, double *F);
 int smx_dominant_wavenumber(smx_handle *h, double *q);
 
 /* dpsi/dt = F^-1[L(k) psi_hat + N_hat] of the current state (host, reference
- * layout).  Replaces the dTpsi transform of cosNoAdvMorphCurv.cpp:744,764 and
- * surfTrack.cpp:611,622. */
+ * layout); with advection on, N_hat includes -U.grad psi of the current state
+ * (surfTrack.cpp:585-606).  Replaces the dTpsi transform of
+ * cosNoAdvMorphCurv.cpp:744,764 and surfTrack.cpp:606,622. */
 int smx_dtpsi(smx_handle *h, double *dtpsi, int64_t row_stride);
 
 /* Eckhaus scan of energyPsi.m:68-234 on the current state psi0: energy[m] is the

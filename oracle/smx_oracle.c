@@ -858,7 +858,8 @@ double or_dominant_wavenumber(const or_state *s) {
 }
 
 /* dpsi/dt = F^-1[s (aLin psiq + Nq)] of the current state:
- * surfTrack.cpp:611,622 (periodic), cosNoAdvMorphCurv.cpp:744,764 (Neumann). */
+ * surfTrack.cpp:606,622 (periodic; Nq includes the advective term, :585-594),
+ * cosNoAdvMorphCurv.cpp:744,764 (Neumann; with advection on, the Neumann advective term). */
 int or_dtpsi(or_state *s, double *out) {
   const int64_t N = s->N;
   double *nr = (double *)malloc(sizeof(double) * N);
@@ -880,6 +881,7 @@ int or_dtpsi(or_state *s, double *out) {
   } else {
     double *psiq = (double *)malloc(sizeof(double) * N), *nq = (double *)malloc(sizeof(double) * N);
     or_redft10_3d(s->nx, s->ny, s->nz, s->psi, psiq);
+    if (s->c.advect) neumann_advection(s, psiq, nr, NULL);
     or_redft10_3d(s->nx, s->ny, s->nz, nr, nq);
     for (int i = 0; i < s->nx; ++i)
       for (int j = 0; j < s->ny; ++j)

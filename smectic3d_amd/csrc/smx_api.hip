@@ -91,6 +91,7 @@ struct smx_handle {
   void* S2 = nullptr;  // S_z then U_z
   double* surf = nullptr;    // 4 x nxy persistent surfTrack outputs (zero-initialised)
   double* sfield = nullptr;  // surfTrack real fields, 8 x N (allocated on first use)
+  void* advw[6] = {};        // FP64 stage arrays of the advected dpsi/dt (allocated on first use)
   DevTables<float> t32;
   DevTables<double> t64;
   std::vector<void*> allocs;
@@ -329,124 +330,119 @@ struct SlotTimer {
   }
 };
 
-// One advected periodic step (pf3d.cpp:395-485): 15 launches, in place per tile.
+// The advected chain up to N (pf3d.cpp:399-467; the first 14 launches of a step), in place per
+// tile: from psi, d_x psi and psiSH at (x, ky, kz) in A, Ax, As to N at (x, ky, kz) in As (A, Ax
+// are left at (x, y, kz); By, S1, S2 are work arrays).  Periodic: the handle's tuned y schedules
+// yi, yf; Neumann: the sine parities of d_a psi, S_a and U_a (smx_neuadv.h).  Kernel k is timed
+// by tm slot `slot0 + k` when tm is armed.
 template <typename T>
-static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
-  using Ops = PeriodicOps<T>;
-  using Adv = AdvectOps<T>;
+static int advect_to_n(smx_handle* h, const AParams<T>& ap, void* vA, void* vAx, void* vAs, void* vBy, void* vS1,
+                       void* vS2, SlotTimer* tm, PassSched yi, PassSched yf) {
   const PParams<T>& p = ap.p;
   const DevTables<T>& t = tables<T>(h);
-  cplx<T>*A = (cplx<T>*)h->A, *Ax = (cplx<T>*)h->Ax, *As = (cplx<T>*)h->As, *By = (cplx<T>*)h->By;
-  cplx<T>*S1 = (cplx<T>*)h->S1, *S2 = (cplx<T>*)h->S2, *R = (cplx<T>*)h->R;
+  const int64_t np = h->npencils;
   int slot = 0;
   hipError_t e = hipSuccess;
-  SlotTimer tm(ev);
-  const PassSched yi = h->sched[SLOT_YINV], yf = h->sched[SLOT_YFWD];  // tuned y passes
-#define SMX_REC()                                             \
-  do {                                                        \
+#define SMX_REC()                                                                                          \
+  do {                                                                                                     \
     if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e)); \
-    if (++slot < 15) tm.arm(slot); /* 15 kernels per step */ \
+    if (tm) tm->arm(++slot);                                                                               \
   } while (0)
-  tm.arm(0);
-  e = Ops::ypass(A, By, p.ny, h->npencils, +1, t.twy, h->st, t.ky, yi);  // d_y psi (pf3d.cpp:405,411)
-  SMX_REC();
-  e = Ops::ypass(A, A, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
-  SMX_REC();
-  e = Ops::ypass(Ax, Ax, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
-  SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
-  SMX_REC();
-  e = Adv::zforce(A, Ax, By, As, As, S1, S2, ap, h->st);  // S = f grad psi (:417-425)
-  SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
-  SMX_REC();
-  e = Ops::ypass(S1, S1, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
-  SMX_REC();
-  e = Ops::ypass(S2, S2, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
-  SMX_REC();
-  e = Adv::stokes(As, S1, S2, ap, h->st);  // (:433-447)
-  SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
-  SMX_REC();
-  e = Ops::ypass(S1, S1, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
-  SMX_REC();
-  e = Ops::ypass(S2, S2, p.ny, h->npencils, +1, t.twy, h->st, nullptr, yi);
-  SMX_REC();
-  e = Adv::zadvnl(As, S1, S2, A, Ax, By, As, ap, h->st);  // N = N(psi) - U.grad psi (:455-465)
-  SMX_REC();
-  e = Ops::ypass(As, As, p.ny, h->npencils, -1, t.twy, h->st, nullptr, yf);
-  SMX_REC();
-  e = Adv::xpass_adv(X_STEP, As, R, nullptr, A, Ax, As, p, h->st);  // CNAB2 (:471-481); projects R
-  SMX_REC();
+  if (tm) tm->arm(0);
+  if (!h->neumann) {
+    using Ops = PeriodicOps<T>;
+    using Adv = AdvectOps<T>;
+    cplx<T>*A = (cplx<T>*)vA, *Ax = (cplx<T>*)vAx, *As = (cplx<T>*)vAs, *By = (cplx<T>*)vBy;
+    cplx<T>*S1 = (cplx<T>*)vS1, *S2 = (cplx<T>*)vS2;
+    e = Ops::ypass(A, By, p.ny, np, +1, t.twy, h->st, t.ky, yi);  // d_y psi (pf3d.cpp:405,411)
+    SMX_REC();
+    e = Ops::ypass(A, A, p.ny, np, +1, t.twy, h->st, nullptr, yi);
+    SMX_REC();
+    e = Ops::ypass(Ax, Ax, p.ny, np, +1, t.twy, h->st, nullptr, yi);
+    SMX_REC();
+    e = Ops::ypass(As, As, p.ny, np, +1, t.twy, h->st, nullptr, yi);
+    SMX_REC();
+    e = Adv::zforce(A, Ax, By, As, As, S1, S2, ap, h->st);  // S = f grad psi (:417-425)
+    SMX_REC();
+    e = Ops::ypass(As, As, p.ny, np, -1, t.twy, h->st, nullptr, yf);
+    SMX_REC();
+    e = Ops::ypass(S1, S1, p.ny, np, -1, t.twy, h->st, nullptr, yf);
+    SMX_REC();
+    e = Ops::ypass(S2, S2, p.ny, np, -1, t.twy, h->st, nullptr, yf);
+    SMX_REC();
+    e = Adv::stokes(As, S1, S2, ap, h->st);  // (:433-447)
+    SMX_REC();
+    e = Ops::ypass(As, As, p.ny, np, +1, t.twy, h->st, nullptr, yi);
+    SMX_REC();
+    e = Ops::ypass(S1, S1, p.ny, np, +1, t.twy, h->st, nullptr, yi);
+    SMX_REC();
+    e = Ops::ypass(S2, S2, p.ny, np, +1, t.twy, h->st, nullptr, yi);
+    SMX_REC();
+    e = Adv::zadvnl(As, S1, S2, A, Ax, By, As, ap, h->st);  // N = N(psi) - U.grad psi (:455-465)
+    SMX_REC();
+    e = Ops::ypass(As, As, p.ny, np, -1, t.twy, h->st, nullptr, yf);
+    SMX_REC();
+  } else {
+    using NO = NeumannOps<T>;
+    using NA = NeuAdvOps<T>;
+    T *A = (T*)vA, *Ax = (T*)vAx, *As = (T*)vAs, *By = (T*)vBy, *S1 = (T*)vS1, *S2 = (T*)vS2;
+    const T qnx = (T)(M_PI / h->dx), qny = (T)(M_PI / h->dy), qnz = (T)(M_PI / h->dz);
+    e = NO::ypass(A, By, p.ny, np, +1, t.twy, t.qtwy, h->st, DP_DSIN, t.ky);  // d_y psi (sine along y)
+    SMX_REC();
+    e = NO::ypass(A, A, p.ny, np, +1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+    e = NO::ypass(Ax, Ax, p.ny, np, +1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+    e = NO::ypass(As, As, p.ny, np, +1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+    e = NA::zforce(A, Ax, By, As, As, S1, S2, ap, t.qtwz, h->st);  // S = f grad psi
+    SMX_REC();
+    e = NO::ypass(As, As, p.ny, np, -1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+    e = NO::ypass(S1, S1, p.ny, np, -1, t.twy, t.qtwy, h->st, DP_SIN);
+    SMX_REC();
+    e = NO::ypass(S2, S2, p.ny, np, -1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+    e = NA::stokes(As, S1, S2, ap, t.qtwx, qnx, qny, qnz, h->st);
+    SMX_REC();
+    e = NO::ypass(As, As, p.ny, np, +1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+    e = NO::ypass(S1, S1, p.ny, np, +1, t.twy, t.qtwy, h->st, DP_SIN);
+    SMX_REC();
+    e = NO::ypass(S2, S2, p.ny, np, +1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+    e = NA::zadvnl(As, S1, S2, A, Ax, By, As, ap, t.qtwz, h->st);  // N(psi) - U.grad psi
+    SMX_REC();
+    e = NO::ypass(As, As, p.ny, np, -1, t.twy, t.qtwy, h->st);
+    SMX_REC();
+  }
 #undef SMX_REC
   return SMX_OK;
 }
 
-// One advected Neumann step: the 15 launches of step_advect on the DCT grid, with the sine
-// parities of d_a psi, S_a and U_a (smx_neuadv.h).
+// One advected step (pf3d.cpp:395-485; on the DCT grid, the Neumann form): the chain up to N,
+// then the x pass with the CNAB2 update (periodic: it projects R onto Hermitian symmetry), 15
+// launches in place per tile.
 template <typename T>
-static int step_advect_neumann(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
-  using NO = NeumannOps<T>;
-  using NA = NeuAdvOps<T>;
-  const PParams<T>& p = ap.p;
-  const DevTables<T>& t = tables<T>(h);
-  T *A = (T*)h->A, *Ax = (T*)h->Ax, *As = (T*)h->As, *By = (T*)h->By;
-  T *S1 = (T*)h->S1, *S2 = (T*)h->S2, *R = (T*)h->R;
-  const int64_t np = h->npencils;
-  const T qnx = (T)(M_PI / h->dx), qny = (T)(M_PI / h->dy), qnz = (T)(M_PI / h->dz);
-  int slot = 0;
-  hipError_t e = hipSuccess;
+static int step_advect(smx_handle* h, const AParams<T>& ap, hipEvent_t* ev) {
   SlotTimer tm(ev);
-#define SMX_REC()                                                                                          \
-  do {                                                                                                     \
-    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e)); \
-    if (++slot < 15) tm.arm(slot);                                                                         \
-  } while (0)
-  tm.arm(0);
-  e = NO::ypass(A, By, p.ny, np, +1, t.twy, t.qtwy, h->st, DP_DSIN, t.ky);  // d_y psi (sine along y)
-  SMX_REC();
-  e = NO::ypass(A, A, p.ny, np, +1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NO::ypass(Ax, Ax, p.ny, np, +1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NO::ypass(As, As, p.ny, np, +1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NA::zforce(A, Ax, By, As, As, S1, S2, ap, t.qtwz, h->st);  // S = f grad psi (pf3d.cpp:417-425)
-  SMX_REC();
-  e = NO::ypass(As, As, p.ny, np, -1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NO::ypass(S1, S1, p.ny, np, -1, t.twy, t.qtwy, h->st, DP_SIN);
-  SMX_REC();
-  e = NO::ypass(S2, S2, p.ny, np, -1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NA::stokes(As, S1, S2, ap, t.qtwx, qnx, qny, qnz, h->st);  // (pf3d.cpp:433-447)
-  SMX_REC();
-  e = NO::ypass(As, As, p.ny, np, +1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NO::ypass(S1, S1, p.ny, np, +1, t.twy, t.qtwy, h->st, DP_SIN);
-  SMX_REC();
-  e = NO::ypass(S2, S2, p.ny, np, +1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NA::zadvnl(As, S1, S2, A, Ax, By, As, ap, t.qtwz, h->st);  // N(psi) - U.grad psi (:455-465)
-  SMX_REC();
-  e = NO::ypass(As, As, p.ny, np, -1, t.twy, t.qtwy, h->st);
-  SMX_REC();
-  e = NA::xpass_adv(X_STEP, As, R, nullptr, A, Ax, As, p, t.qtwx, h->st);  // CNAB2 (cosNoAdvMorph.cpp:524-525)
-  SMX_REC();
-#undef SMX_REC
+  CHK(advect_to_n<T>(h, ap, h->A, h->Ax, h->As, h->By, h->S1, h->S2, ev ? &tm : nullptr, h->sched[SLOT_YINV],
+                     h->sched[SLOT_YFWD]));
+  hipError_t e = h->neumann
+                     ? NeuAdvOps<T>::xpass_adv(X_STEP, (const T*)h->As, (T*)h->R, nullptr, (T*)h->A, (T*)h->Ax,
+                                               (T*)h->As, ap.p, tables<T>(h).qtwx, h->st)
+                     : AdvectOps<T>::xpass_adv(X_STEP, (const cplx<T>*)h->As, (cplx<T>*)h->R, nullptr,
+                                               (cplx<T>*)h->A, (cplx<T>*)h->Ax, (cplx<T>*)h->As, ap.p, h->st);
+  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected step: ") + hipGetErrorString(e));
   return SMX_OK;
 }
 
 template <typename T>
 static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
-  if (h->cfg.advect && h->neumann) {
-    AParams<T> ap = aparams<T>(h, h->fp32);
-    ap.p.npartial = NeuAdvOps<T>::zpass_blocks(ap.p);  // FP64 partials of the advected z pass
-    return step_advect_neumann<T>(h, ap, ev);
-  }
   if (h->cfg.advect) {
     AParams<T> ap = aparams<T>(h, h->fp32);
-    ap.p.npartial = AdvectOps<T>::zpass_blocks(ap.p);  // FP64 partials of the advected z pass
+    // FP64 partials of the advected z pass
+    ap.p.npartial = h->neumann ? NeuAdvOps<T>::zpass_blocks(ap.p) : AdvectOps<T>::zpass_blocks(ap.p);
     return step_advect<T>(h, ap, ev);
   }
   SlotTimer tm(ev);
@@ -602,8 +598,38 @@ static int spectral_sums(smx_handle* h, std::vector<double>& s) {
 }
 
 // dpsi/dt of the current state into `out` (FP64 real, dense [x][y][z]); uses real_io as scratch.
+// Advected N_hat of the current state (normalised, FP64) into D1, psi_hat stays in D2: the x
+// inverse of psi_hat to psi, d_x psi and psiSH (the advected x pass in its bootstrap form, whose
+// R output goes to D1 and is discarded), the advected chain up to N on FP64 work arrays, then
+// the forward x pass.  surfTrack.cpp:585-606.
+static int advected_n_spectrum(smx_handle* h) {
+  const size_t bytes = (h->neumann ? 8 : 16) * (size_t)h->nspec;
+  for (void*& w : h->advw)
+    if (!w) CHK(dalloc(h, &w, bytes));
+  AParams<double> ap = aparams<double>(h, false);
+  PParams<double>& p = ap.p;
+  const DevTables<double>& t = h->t64;
+  hipError_t e = h->neumann
+                     ? NeuAdvOps<double>::xpass_adv(X_BOOT, (const double*)h->D2, (double*)h->D1, (const double*)h->D2,
+                                                    (double*)h->advw[0], (double*)h->advw[1], (double*)h->advw[2], p,
+                                                    t.qtwx, h->st)
+                     : AdvectOps<double>::xpass_adv(X_BOOT, (const cplx<double>*)h->D2, (cplx<double>*)h->D1,
+                                                    (const cplx<double>*)h->D2, (cplx<double>*)h->advw[0],
+                                                    (cplx<double>*)h->advw[1], (cplx<double>*)h->advw[2], p, h->st);
+  if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("advected dpsi/dt: ") + hipGetErrorString(e));
+  CHK(advect_to_n<double>(h, ap, h->advw[0], h->advw[1], h->advw[2], h->advw[3], h->advw[4], h->advw[5], nullptr,
+                          PassSched{}, PassSched{}));
+  CHK(xpass<double>(h, p, X_FWD_SCALED, h->advw[2], h->D1, nullptr, nullptr));
+  return SMX_OK;
+}
+
 static int dtpsi_field(smx_handle* h, double* out) {
   CHK(spectrum_state(h));  // D2 = psi_hat
+  if (h->cfg.advect) {
+    CHK(advected_n_spectrum(h));           // D1 = N_hat with the advective term
+    CHK(kmul(h, KM_DTPSI, h->D2, h->D1));  // D1 = aLin psi_hat + N_hat
+    return c2r_spectrum_d1(h, out);
+  }
   CHK(c2r_state(h, h->real_io));
   CHK(nl_spectrum_of_real_io(h));        // D1 = N_hat
   CHK(kmul(h, KM_DTPSI, h->D2, h->D1));  // D1 = aLin psi_hat + N_hat
@@ -1405,7 +1431,6 @@ int smx_dominant_wavenumber(smx_handle* h, double* q) {
 int smx_dtpsi(smx_handle* h, double* out, int64_t row_stride) {
   if (!h || !out) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
-  if (h->cfg.advect) return fail(SMX_ERR_UNSUPPORTED, "dpsi/dt of an advected state is not built");
   const smx_config& c = h->cfg;
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   HIPCHK(hipSetDevice(c.device));
@@ -1470,7 +1495,6 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
                   int64_t* found) {
   if (!h || !opts || !surfz || !vel || !H || !K) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
-  if (h->cfg.advect) return fail(SMX_ERR_UNSUPPORTED, "surfTrack of an advected state needs dpsi/dt with advection (not built)");
   HIPCHK(hipSetDevice(h->cfg.device));
   const int64_t N = h->N;
   if (!h->sfield) CHK(dalloc(h, (void**)&h->sfield, sizeof(double) * 8 * N));

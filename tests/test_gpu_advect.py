@@ -98,12 +98,32 @@ def test_advected_workload_small(prec):
         assert e < (1e-10 if prec == A.SMX_FP64 else 1e-4)
 
 
-def test_advect_rejects_dtpsi_until_built():
-    cfg = cfg_reference(16, 16, 16, advect=1)
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+@pytest.mark.parametrize("force", [A.SMX_FORCE_SH, A.SMX_FORCE_MU])
+def test_advect_dtpsi_and_surftrack(prec, force):
+    """dpsi/dt of an advected state includes -U.grad psi (surfTrack.cpp:585-606), and the periodic
+    surfTrack of the advected loop (surfTrack.cpp is an advected driver) uses it."""
+    shape = (32, 16, 48)
+    say("advect dtpsi", prec, force)
+    cfg = cfg_reference(*shape, dt=2e-3, advect=1, force_form=force, precision=prec)
+    psi = structured_ic(shape)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(3)
     with Solver(cfg) as s:
-        s.set_psi(structured_ic((16, 16, 16)))
-        with pytest.raises(SmxError):
-            s.dtpsi()
+        s.set_psi(psi)
+        s.step(3)
+        e = rel_l2(s.dtpsi(), orc.dtpsi())
+        say("  dtpsi rel L2", e)
+        if prec != A.SMX_FP64:  # FP32 states can move a marginal crossing by one cell
+            assert e < 1e-5
+            return
+        (gz, gv, gh, gk), gf = s.surftrack(0.1)
+    (oz, ov, oh, ok), of = orc.surftrack(0.1)
+    say("  found", gf, of)
+    assert e < 1e-10
+    assert gf == of and gf > 0 and np.array_equal(gz, oz)
+    np.testing.assert_allclose(gv, ov, rtol=0, atol=1e-9 * np.abs(ov).max())
 
 
 @pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])

@@ -143,3 +143,29 @@ def test_neumann_adv_profile_slots():
     names = [p[0] for p in prof]
     assert len(names) == 15 and names[4] == "k_dctz_force" and names[8] == "k_dctx_stokes" and names[-1] == "k_dctx_adv"
     assert all(p[1] > 0 for p in prof)
+
+
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+def test_neumann_adv_dtpsi_and_surftrack(prec):
+    """dpsi/dt with the Neumann advective term, and the DCT surfTrack on the advected state."""
+    shape = (32, 16, 48)
+    say("neumann advected dtpsi", prec)
+    cfg = cfg_na(shape, prec, force_form=A.SMX_FORCE_MU)
+    psi = structured_ic(shape)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(3)
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        s.step(3)
+        e = rel_l2(s.dtpsi(), orc.dtpsi())
+        say("  dtpsi rel L2", e)
+        if prec != A.SMX_FP64:  # FP32 states can move a marginal crossing by one cell
+            assert e < 1e-5
+            return
+        (gz, gv, gh, gk), gf = s.surftrack(0.7)
+    (oz, ov, oh, ok), of = orc.surftrack(0.7)
+    say("  found", gf, of)
+    assert e < 1e-10
+    assert gf == of and gf > 0 and np.array_equal(gz, oz)
+    np.testing.assert_allclose(gv, ov, rtol=0, atol=1e-9 * np.abs(ov).max())

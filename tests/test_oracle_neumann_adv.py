@@ -175,3 +175,31 @@ def test_velocity_is_divergence_free_stokes_flow():
             cross = R[a] * Kq[b] - R[b] * Kq[a]
             assert np.abs(cross).max() < 1e-11 * max(np.abs(s).max() for s in Sx) * K2.max()
     assert max(np.abs(u).max() for u in Ux) > 1e-3
+
+
+@pytest.mark.parametrize("bc", [A.SMX_BC_PERIODIC, A.SMX_BC_NEUMANN])
+def test_advected_dtpsi_is_the_step_rate(bc):
+    """dpsi/dt = F^-1[s (L psi_hat + N_hat)] with the advective term in N (surfTrack.cpp:585-606):
+    after a first step, (psi^2 - psi^1)/dt = [L psi^1 + N^1 + (N^1 - N^0)/2] / C2 approaches it at
+    first order in dt.  The advective term is 5-10% of dpsi/dt here, so leaving it out would stall
+    the convergence at that level."""
+    shape = (16, 16, 24)
+    x = (np.arange(16) + 0.5) / 16 * 2 * np.pi
+    z = (np.arange(24) + 0.5) / 24 * 2 * np.pi
+    psi = (0.7 * np.cos(2 * z)[None, None, :] * (1 + 0.3 * np.cos(x)[:, None, None])
+           + 0.05 * np.sin(x)[:, None, None] * np.cos(3 * z)[None, None, :])
+    psi = np.broadcast_to(psi, shape).copy()
+    errs = []
+    for dt in (4e-5, 2e-5, 1e-5):
+        cfg = A.make_config(*shape, lx=8.0, ly=7.0, lz=11.0, bc=bc, eps=0.3, beta=0.0, gamma=1.0,
+                            nl_model=A.SMX_NL_CUBIC, dt=dt, advect=1, force_form=A.SMX_FORCE_MU)
+        o = O.Oracle(cfg)
+        o.set_psi(psi)
+        o.step(1)
+        p1 = o.get_psi()
+        d = o.dtpsi()
+        o.step(1)
+        errs.append(np.linalg.norm((o.get_psi() - p1) / dt - d) / np.linalg.norm(d))
+    assert errs[-1] < 1e-2
+    for a, b in zip(errs, errs[1:]):
+        assert 1.8 < a / b < 2.2, errs
