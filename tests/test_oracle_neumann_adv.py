@@ -177,8 +177,9 @@ def test_velocity_is_divergence_free_stokes_flow():
     assert max(np.abs(u).max() for u in Ux) > 1e-3
 
 
+@pytest.mark.parametrize("advect", [0, 1])
 @pytest.mark.parametrize("bc", [A.SMX_BC_PERIODIC, A.SMX_BC_NEUMANN])
-def test_advected_dtpsi_is_the_step_rate(bc):
+def test_dtpsi_is_the_step_rate(bc, advect):
     """dpsi/dt = F^-1[s (L psi_hat + N_hat)] with the advective term in N (surfTrack.cpp:585-606):
     after a first step, (psi^2 - psi^1)/dt = [L psi^1 + N^1 + (N^1 - N^0)/2] / C2 approaches it at
     first order in dt.  The advective term is 5-10% of dpsi/dt here, so leaving it out would stall
@@ -192,7 +193,7 @@ def test_advected_dtpsi_is_the_step_rate(bc):
     errs = []
     for dt in (4e-5, 2e-5, 1e-5):
         cfg = A.make_config(*shape, lx=8.0, ly=7.0, lz=11.0, bc=bc, eps=0.3, beta=0.0, gamma=1.0,
-                            nl_model=A.SMX_NL_CUBIC, dt=dt, advect=1, force_form=A.SMX_FORCE_MU)
+                            nl_model=A.SMX_NL_CUBIC, dt=dt, advect=advect, force_form=A.SMX_FORCE_MU)
         o = O.Oracle(cfg)
         o.set_psi(psi)
         o.step(1)
