@@ -251,3 +251,50 @@ def test_surftrack_periodic_plane():
 def test_splitmix_known_answer():
     # first SplitMix64 output for seed 0 is 0xE220A8397B1DCDAF
     assert O.splitmix_uniform(0, 0, 0) == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
+
+
+def test_surftrack_periodic_curved_surface_closed_form():
+    """Periodic surfTrack on psi = cos(q (z - h(x, y))), h = a cos(2 pi x/Lx) cos(2 pi y/Ly): the
+    crossing layer z = h + pi/(2q) is curved.  At the reported cell the oracle's
+    H = (psi_xx + psi_yy)/|grad psi| and K = (psi_xx psi_yy - psi_xy^2)/|grad psi|^2
+    (surfTrack.cpp:612-617,789-797) must equal the same expressions of the analytic derivatives
+    there (the field is band-limited to 1e-20, so spectral derivatives are exact to rounding)."""
+    nx, ny, nz = 32, 24, 48
+    Lx, Ly, Lz = 10.0, 8.0, 12.0
+    q = 2 * math.pi * 2 / Lz
+    a = 0.25
+    x = np.arange(nx) * Lx / nx
+    y = np.arange(ny) * Ly / ny
+    z = np.arange(nz) * Lz / nz
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    kx, ky = 2 * math.pi / Lx, 2 * math.pi / Ly
+    h = a * np.cos(kx * X) * np.cos(ky * Y)
+    hx = -a * kx * np.sin(kx * X) * np.cos(ky * Y)
+    hy = -a * ky * np.cos(kx * X) * np.sin(ky * Y)
+    hxx, hyy = -kx * kx * h, -ky * ky * h
+    hxy = a * kx * ky * np.sin(kx * X) * np.sin(ky * Y)
+    th = q * (Z - h)
+    psi = np.cos(th)
+    s, c = np.sin(th), np.cos(th)
+    pxx = q * hxx * s - q * q * hx * hx * c
+    pyy = q * hyy * s - q * q * hy * hy * c
+    pxy = q * hxy * s - q * q * hx * hy * c
+    g = np.sqrt((q * hx * s) ** 2 + (q * hy * s) ** 2 + (q * s) ** 2)
+    cfg = A.make_config(nx, ny, nz, lx=Lx, ly=Ly, lz=Lz, eps=0.3, beta=0.0, gamma=1.0, nl_model=A.SMX_NL_CUBIC)
+    o = O.Oracle(cfg)
+    o.set_psi(psi)
+    (sz, v, H, K), found = o.surftrack(0.1)
+    assert found == nx * ny
+    Hs = H.reshape(nx, ny)
+    Ks = K.reshape(nx, ny)
+    checked = 0
+    for i in range(nx):
+        for j in range(ny):
+            k = int(sz[i * ny + j])
+            kk = k - 1 if abs(psi[i, j, k]) > abs(psi[i, j, k - 1]) else k  # the cell the reference evaluates
+            He = (pxx[i, j, kk] + pyy[i, j, kk]) / g[i, j, kk]
+            Ke = (pxx[i, j, kk] * pyy[i, j, kk] - pxy[i, j, kk] ** 2) / g[i, j, kk] ** 2
+            assert abs(Hs[i, j] - He) < 1e-10 * (1 + abs(He))
+            assert abs(Ks[i, j] - Ke) < 1e-10 * (1 + abs(Ke))
+            checked += 1
+    assert checked == nx * ny and np.ptp(Hs) > 1e-2  # the surface is curved
