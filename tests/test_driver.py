@@ -94,3 +94,19 @@ def test_driver_psi_avg_file(driver, tmp_path):
     d = 2 * math.pi / 8  # plain preset: 8 points per wavelength, q0 = 1
     want = O.psi_avg(psi, d, d, d, nw)
     np.testing.assert_allclose(got, want, rtol=1e-4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bc", ["periodic", "neumann"])
+def test_driver_advected_run_with_surfaces(driver, tmp_path, bc):
+    """The advected loop with surface outputs (surfTrack.cpp is an advected driver): the surf
+    files get one record per surface output and velSurf.dat carries dpsi/dt with advection."""
+    nx, ny, nz = 16, 16, 32
+    extra = ["--neumann", "--advect", "1", "--arm", "0.7"] if bc == "neumann" else []
+    subprocess.run([driver, "--preset", "advected", "--nx", str(nx), "--ny", str(ny), "--nz", str(nz), *extra,
+                    "--l1-every", "10", "--surf-every", "2", "--steps", "40", "--save-every", "1000",
+                    "--out", str(tmp_path)], check=True)
+    for name in ("surfPsi.dat", "velSurf.dat", "curvH.dat", "curvK.dat"):
+        assert (tmp_path / name).exists()
+    vel = np.array([float(v) for v in (tmp_path / "velSurf.dat").read_text().split()])
+    assert vel.size > 0 and np.all(np.isfinite(vel))
