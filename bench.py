@@ -149,8 +149,10 @@ def run_advected(steps):
 
 def run_neumann_advected(steps):
     """Advection-coupled Neumann box 256x256x384: the DCT grid of cosNoAdvMorph with the
-    sine/cosine Stokes projection, mu grad psi force, eta=1, eps=0.5, q0=1, gamma=1, dt=0.02,
-    8 points per wavelength, hyperboloid IC; FP32-mixed."""
+    sine/cosine Stokes projection and the advected workload's parameters (mu grad psi force,
+    eta=1, eps=0.5, q0=1, gamma=1, dt=0.02, psi0 ~ U[-0.05, 0.05]), 8 points per wavelength;
+    FP32-mixed.  (A 0.4-amplitude layered start is unstable at this dt under explicit advection,
+    in the FP64 oracle as on the GPU.)"""
     from smectic3d_amd import Solver
     from smectic3d_amd import _abi as A
     nx, ny, nz = 256, 256, 384
@@ -158,11 +160,9 @@ def run_neumann_advected(steps):
     cfg = A.make_config(nx, ny, nz, lx=nx * dx, ly=ny * dx, lz=nz * dx, bc=A.SMX_BC_NEUMANN, eps=0.5, q0=1.0,
                         beta=0.0, gamma=1.0, dt=0.02, nl_model=A.SMX_NL_CUBIC, advect=1, eta=1.0, rho=1.0,
                         force_form=A.SMX_FORCE_MU, precision=A.SMX_FP32_MIXED)
-    c = (np.arange(nx) - nx // 2) * dx
-    cz = (np.arange(nz) - nz // 2) * dx
-    r2 = np.maximum(c[:, None, None] ** 2 + c[None, :, None] ** 2 - cz[None, None, :] ** 2, 0.0)
+    rng = np.random.default_rng(5)
     with Solver(cfg) as s:
-        s.set_psi(0.4 * np.cos(np.sqrt(r2 + 4.0)))
+        s.set_psi(rng.uniform(-0.05, 0.05, (nx, ny, nz)))
         s.step(5)
         s.sync()
         t0 = time.perf_counter()

@@ -83,10 +83,10 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
   const int tid = threadIdx.x, tc = tid % TPC, q = tid / TPC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, tw, tid);
-  // LDS copy of the quarter-wave twiddles for the DCT-II outputs, read after the transform's
-  // barriers (the DCT-III inputs read qtw before any barrier, from the global table)
-  __shared__ cplx<T> QW[DIR < 0 ? L : 1];
-  if constexpr (DIR < 0) tw_lds<L, NT>(QW, qtw, tid);
+  // LDS copy of the quarter-wave twiddles, read after a barrier (the transform's, or the one
+  // that stages the DCT-III input rows)
+  __shared__ cplx<T> QW[L];
+  tw_lds<L, NT>(QW, qtw, tid);
   const int64_t pa = (int64_t)blockIdx.x * NC + 2 * q;  // pencils pa (column a), pa + 1 (column b)
   const T* a = in + pa * L;
   const T* b = a + L;
@@ -110,16 +110,19 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
       oa[(2 * qq + 1) * L + o] = y.y;
     }
   } else {
+    // each entry is loaded once: a thread's own entries k stay in registers, the partners L-k
+    // come from the tile
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = TL::template pos_in<inv_first<L>()>(tc, e);
-      if constexpr (PAR == DP_COS) {
-        const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : mk(a[L - k], b[L - k]);
-        v[e] = dct3_in(mk(a[k], b[k]), pn, qtw[k]);
-      } else {
-        const int m = (L - k) % L;
-        v[e] = dct3_par<PAR>(mk(a[k], b[k]), mk(a[m], b[m]), k, L, kq, qtw[k]);
-      }
+      v[e] = mk(a[k], b[k]);
+    }
+    TL::template store<inv_first<L>()>(v, S, q, tc);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = TL::template pos_in<inv_first<L>()>(tc, e);
+      v[e] = dct3_par<PAR>(v[e], S[AD::at((L - k) % L, q)], k, L, kq, QW[k]);
     }
     InvFFTA<T, L, AD>::run(v, S, q, tc, twl);
     T* oa = out + pa * L;
@@ -300,20 +303,28 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twz, tid);
   // LDS copy of qtw for the DCT-II outputs of the split (read after barriers): a global
   // load there would wait behind the previous row's store in the wave's one counter
-  __shared__ cplx<T> QW[(OUT & Z_OUT_SPEC) ? L : 1];
-  if constexpr ((OUT & Z_OUT_SPEC) != 0) tw_lds<L, NT>(QW, qtw, tid);
+  constexpr bool kQW = (OUT & Z_OUT_SPEC) != 0 || IN == Z_IN_SPEC;
+  __shared__ cplx<T> QW[kQW ? L : 1];
+  if constexpr (kQW) tw_lds<L, NT>(QW, qtw, tid);
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
   static_assert(fwd_first<L>() == inv_last<L>(), "plan symmetry");
   constexpr int RP = fwd_first<L>();
   cplx<T> v[E];  // (psi_a, psi_b) at z = src(m), m = pos_in<RP>
   if constexpr (IN == Z_IN_SPEC) {
+    // each row is loaded once: its own rows k stay in registers, the partner rows L-k come from
+    // the tile (one global load per entry instead of two)
     const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in + f0);
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      v[e] = ld_off(src, (uint32_t)Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e) * rowb + q * CB);
+    Tile<T, L, E, FP>::template store<inv_first<L>()>(v, F, q, tc);
+    __syncthreads();  // also orders the QW fill before its reads
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
-      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : ld_off(src, (uint32_t)(L - k) * rowb + q * CB);
-      v[e] = dct3_in(ld_off(src, (uint32_t)k * rowb + q * CB), pn, qtw[k]);
+      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : F[(L - k) * FP + q];
+      v[e] = dct3_in(v[e], pn, QW[k]);
     }
     InvFFT<T, L, FP>::run(v, F, q, tc, twl);
   } else {
