@@ -87,6 +87,9 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
   // that stages the DCT-III input rows)
   __shared__ cplx<T> QW[L];
   tw_lds<L, NT>(QW, qtw, tid);
+  __shared__ T KQ[PAR == DP_DSIN ? L : 1];  // k_y for the derivative, read after the staging barrier
+  if constexpr (PAR == DP_DSIN)
+    for (int i = tid; i < L; i += NT) KQ[i] = kq[i];
   const int64_t pa = (int64_t)blockIdx.x * NC + 2 * q;  // pencils pa (column a), pa + 1 (column b)
   const T* a = in + pa * L;
   const T* b = a + L;
@@ -122,7 +125,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = TL::template pos_in<inv_first<L>()>(tc, e);
-      v[e] = dct3_par<PAR>(v[e], S[AD::at((L - k) % L, q)], k, L, kq, QW[k]);
+      v[e] = dct3_par<PAR>(v[e], S[AD::at((L - k) % L, q)], k, L, KQ, QW[k]);
     }
     InvFFTA<T, L, AD>::run(v, S, q, tc, twl);
     T* oa = out + pa * L;
