@@ -76,3 +76,29 @@ def test_growth_sweep_batch_matches_oracle():
     mq = dmq * np.arange(1, 11)
     sigma = -0.7 - (mq ** 2 - 1.0) ** 2
     np.testing.assert_allclose(g, sigma / (1 - 1e-3 * sigma / 2), rtol=1e-6)
+
+
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+def test_batch_of_every_step_kind(prec):
+    """Periodic and Neumann boxes, plain and advected, in one captured graph: each equals the
+    same box advanced alone, bit for bit."""
+    cfgs = []
+    for bc in (A.SMX_BC_PERIODIC, A.SMX_BC_NEUMANN):
+        for adv in (0, 1):
+            n = 32 if adv else 48
+            L = n / 8 * 2 * math.pi
+            cfgs.append(A.make_config(n, n, n + 16, lx=L, ly=L, lz=L * (n + 16) / n, bc=bc, eps=0.5, q0=1.0,
+                                      beta=0.0, gamma=1.0, dt=0.02, nl_model=A.SMX_NL_CUBIC, advect=adv,
+                                      force_form=A.SMX_FORCE_MU, precision=prec))
+    psis = [O.ic_noise(c.nx, c.ny, c.nz, 300 + i, 0.05) for i, c in enumerate(cfgs)]
+    say("batch of kinds", prec)
+    with Batch(cfgs) as b:
+        for s, p in zip(b.boxes, psis):
+            s.set_psi(p)
+        b.run(12, graph_steps=4)
+        got = [s.get_psi() for s in b.boxes]
+    for c, p, g in zip(cfgs, psis, got):
+        with Solver(c) as s:
+            s.set_psi(p)
+            s.step(12)
+            np.testing.assert_array_equal(g, s.get_psi())
