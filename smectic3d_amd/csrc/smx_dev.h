@@ -294,16 +294,6 @@ __device__ __forceinline__ const cplx<T>* tw_lds(cplx<T>* lds, const cplx<T>* __
   return lds;
 }
 
-// An offset of zero that the compiler cannot see through.  Added to an LDS table's address
-// inside a loop that runs one transform instance over several fields, it keeps the compiler
-// from hoisting the table's loads out of the loop (invariant across iterations, they would sit
-// in registers for the whole loop and cut the waves per SIMD).
-__device__ __forceinline__ int opaque_zero() {
-  int z = 0;
-  asm volatile("" : "+s"(z));
-  return z;
-}
-
 // ---------------------------------------------------------------------------
 // FP64 block reduction helpers (wavefront shuffles, then LDS across waves).
 __device__ __forceinline__ double wave_sum(double v) {

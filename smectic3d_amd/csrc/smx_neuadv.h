@@ -87,11 +87,11 @@ struct DZ {
 
 // LDS tables of a z kernel: twiddles, quarter-wave twiddles and k_z, filled before any use.
 #define SMX_DZ_TABLES(NT_)                                  \
-  __shared__ cplx<T> TW[L], QWl[L];                         \
-  __shared__ T KZl[L];                                      \
+  __shared__ cplx<T> TW[L], QW[L];                         \
+  __shared__ T KZ[L];                                      \
   const cplx<T>* twl = tw_lds<L, NT_>(TW, p.twz, tid);      \
-  tw_lds<L, NT_>(QWl, qtw, tid);                            \
-  for (int i = tid; i < L; i += NT_) KZl[i] = p.kz[i];      \
+  tw_lds<L, NT_>(QW, qtw, tid);                            \
+  for (int i = tid; i < L; i += NT_) KZ[i] = p.kz[i];      \
   __syncthreads();
 
 // Force S = f grad psi (pf3d.cpp:417-425), f = psiSH or mu = psiSH - eps psi - N(psi).
@@ -110,8 +110,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_force(cons
   SMX_DZ_TABLES(Z::NT)
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
-  // Fields in turn through one inverse and one forward transform instance (a straight-line
-  // kernel with one instance per field holds more registers and code):
+  // Fields in turn through one inverse and one forward transform instance (instead of one
+  // inlined transform per field):
   //   0 psiSH -> f;  1 psi -> f = mu (F_MU only; also forms d_z psi's inputs);  2 d_z psi -> S2;
   //   3 d_x psi -> S0;  4 d_y psi -> S1.
   const bool mu = (ap.force == F_MU);
@@ -119,10 +119,6 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_force(cons
 #pragma unroll 1
   for (int i = 0; i < 5; ++i) {
     if (i == 1 && !mu) continue;
-    const int z = opaque_zero();  // tables reloaded per field (see opaque_zero)
-    const cplx<T>* tw = twl + z;
-    const cplx<T>* QW = QWl + z;
-    const T* KZ = KZl + z;
     if (i == 0) Z::stage_form(bs, f0, rowb, q, tc, DP_COS, KZ, QW, F, w);
     else if (i == 1) Z::template stage_form<true>(bpsi, f0, rowb, q, tc, DP_COS, KZ, QW, F, w, g);
     else if (i == 2 && mu) {
@@ -130,7 +126,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_force(cons
       for (int e = 0; e < E; ++e) w[e] = g[e];
     } else Z::stage_form(i == 2 ? bpsi : (i == 3 ? bx : by), f0, rowb, q, tc, i == 2 ? DP_DSIN : DP_COS, KZ, QW, F,
                          w);
-    InvFFT<T, L, Z::FP>::run(w, F, q, tc, tw);
+    InvFFT<T, L, Z::FP>::run(w, F, q, tc, twl);
     if (i == 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) f[e] = w[e];
@@ -141,7 +137,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_force(cons
     } else {
 #pragma unroll
       for (int e = 0; e < E; ++e) w[e] = mk(f[e].x * w[e].x, f[e].y * w[e].y);
-      Z::r2c_store(w, i == 2 ? o2 : (i == 3 ? o0 : o1), i == 2 ? DP_SIN : DP_COS, f0, rowb, q, tc, tid, QW, F, tw);
+      Z::r2c_store(w, i == 2 ? o2 : (i == 3 ? o0 : o1), i == 2 ? DP_SIN : DP_COS, f0, rowb, q, tc, tid, QW, F, twl);
     }
   }
 }
@@ -169,10 +165,6 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
   cplx<T> w[E], aux[E], nacc[E];
 #pragma unroll 1
   for (int i = 0; i < 7; ++i) {
-    const int z = opaque_zero();  // tables reloaded per field (see opaque_zero)
-    const cplx<T>* tw = twl + z;
-    const cplx<T>* QW = QWl + z;
-    const T* KZ = KZl + z;
     if (i == 0) Z::template stage_form<true>(bpsi, f0, rowb, q, tc, DP_COS, KZ, QW, F, w, aux);
     else if (i == 1) {
 #pragma unroll
@@ -180,7 +172,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
     } else
       Z::stage_form(i == 2 ? u2 : (i == 3 ? u0 : (i == 4 ? bx : (i == 5 ? u1 : by))), f0, rowb, q, tc,
                     i == 2 ? DP_SIN : DP_COS, KZ, QW, F, w);
-    InvFFT<T, L, Z::FP>::run(w, F, q, tc, tw);
+    InvFFT<T, L, Z::FP>::run(w, F, q, tc, twl);
     if (i == 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) nacc[e] = mk(nonlin(p, w[e].x), nonlin(p, w[e].y));
@@ -193,7 +185,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
     }
   }
   if (p.partial != nullptr) wave_partial<E, Z::NT>(nacc, red, tid);
-  Z::r2c_store(nacc, nout, DP_COS, f0, rowb, q, tc, tid, QWl, F, twl);  // its barriers order red[]
+  Z::r2c_store(nacc, nout, DP_COS, f0, rowb, q, tc, tid, QW, F, twl);  // its barriers order red[]
   if (p.partial != nullptr) store_partial<Z::NT>(red, p.partial + blockIdx.x, tid);
 }
 
@@ -214,10 +206,10 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_dctx_stokes
   __shared__ cplx<T> F[L * FP];
   const PParams<T>& p = ap.p;
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
-  __shared__ cplx<T> TW[L], QWl[L];
+  __shared__ cplx<T> TW[L], QW[L];
   __shared__ T KX[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
-  tw_lds<L, NT>(QWl, qtw, tid);
+  tw_lds<L, NT>(QW, qtw, tid);
   for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy + y0;
@@ -238,17 +230,13 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_dctx_stokes
     }
   };
   // spectra at slots k = tc + it*TPC and L-k, per component; the components go through one
-  // transform instance each way (loops over a; one instance per component would hold more
-  // registers and code)
+  // transform instance each way (loops over a, instead of one inlined transform per component)
   cplx<T> v[E];
   cplx<T> fk0[NIT], fm0[NIT], fk1[NIT], fm1[NIT], fk2[NIT], fm2[NIT];
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    const int z = opaque_zero();  // tables reloaded per component (see opaque_zero)
-    const cplx<T>* tw = twl + z;
-    const cplx<T>* QW = QWl + z;
     load(a, v);
-    FwdFFT<T, L, FP, V>::run(v, F, q, tc, tw);
+    FwdFFT<T, L, FP, V>::run(v, F, q, tc, twl);
     __syncthreads();  // the previous component's reads of the tile are done
     Tile<T, L, E, FP>::template store<fwd_last<L, V>()>(v, F, q, tc);
     __syncthreads();
@@ -304,9 +292,6 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_dctx_stokes
   }
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    const int z = opaque_zero();
-    const cplx<T>* tw = twl + z;
-    const cplx<T>* QW = QWl + z;
     __syncthreads();  // the last reads of the tile (items or the previous inverse) are done
     // DCT-III inputs: cosine components from slots k, L-k; the sine one from L-k, k.  One call
     // per component, so that each reads its own arrays with constant indexes.
@@ -332,7 +317,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_dctx_stokes
     else inputs(fk2, fm2, false);
     __syncthreads();
     Tile<T, L, E, FP>::template load<inv_first<L, V>()>(v, F, q, tc);
-    InvFFT<T, L, FP, V>::run(v, F, q, tc, tw);
+    InvFFT<T, L, FP, V>::run(v, F, q, tc, twl);
     cplx<T>* dst = comp(a);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -453,7 +438,6 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctx_adv(const 
   T* outs[3] = {outA + plane, outX + plane, outS + plane};
 #pragma unroll 1
   for (int o = 0; o < 3; ++o) {
-    const cplx<T>* tw = twl + opaque_zero();  // twiddles reloaded per output (see opaque_zero)
     if (o > 0) {
       __syncthreads();  // the previous inverse transform's LDS reads are done
 #pragma unroll
@@ -484,7 +468,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctx_adv(const 
     }
     __syncthreads();
     Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F, q, tc);
-    InvFFT<T, L, FP>::run(v, F, q, tc, tw);
+    InvFFT<T, L, FP>::run(v, F, q, tc, twl);
     cplx<T>* dst = reinterpret_cast<cplx<T>*>(outs[o]);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
