@@ -104,7 +104,9 @@ int smx_get_psi(smx_handle *h, double *psi, int64_t row_stride);
 /* Advance nsteps CNAB2 steps.  Replaces the body of the time loop:
  * pf3d.cpp:390-485,549 (periodic, with advection when cfg.advect),
  * pf3dMorph.cpp:501-610 (mid-plane penalty),
- * cosFFT/mesabi/cosNoAdvMorph.cpp:483-530,619 (Neumann).
+ * cosFFT/mesabi/cosNoAdvMorph.cpp:483-530,619 (Neumann; with cfg.advect the
+ * same advected step on the cosine grid, velocity components as sine series
+ * along their own axis).
  * Asynchronous on the handle's stream. */
 int smx_step(smx_handle *h, int64_t nsteps);
 
