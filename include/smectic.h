@@ -238,8 +238,8 @@ int smx_set_schedule(smx_handle *h, const smx_schedule *s);
  * pf3d.cpp:226-291 (which times candidate FFT algorithms at set-up).  Periodic handles
  * only; not callable during stream capture. */
 int smx_autotune(smx_handle *h, int reps, smx_schedule *chosen);
-/* smx_autotune on every box (boxes of one shape are tuned once); the next smx_batch_run
- * captures its graph again with the new schedules. */
+/* smx_autotune on every periodic box (boxes of one shape are tuned once; Neumann boxes keep
+ * their fixed tiles); the next smx_batch_run captures its graph again with the new schedules. */
 int smx_batch_autotune(smx_batch *b, int reps);
 
 /* Block until the handle's stream is idle. */

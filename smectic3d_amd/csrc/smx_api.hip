@@ -1700,7 +1700,8 @@ int smx_batch_run(smx_batch* b, int64_t nsteps, int64_t graph_steps) {
 
 int smx_batch_autotune(smx_batch* b, int reps) {
   if (!b) return fail(SMX_ERR_INVALID, "null batch");
-  for (smx_handle* h : b->boxes) CHK(smx_autotune(h, reps, nullptr));
+  for (smx_handle* h : b->boxes)  // Neumann boxes keep their fixed tiles
+    if (!h->neumann) CHK(smx_autotune(h, reps, nullptr));
   // the captured graph holds the old schedules: capture again on the next run
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
