@@ -21,9 +21,20 @@ def say(*a):
     sys.stdout.flush()
 
 
-def torch_steps(cfg, psi0, nsteps):
-    import torch
+def torch_gpu():
+    """torch with its own HIP runtime on the GPU, or a skip: that runtime does not start in a
+    process where libsmectic's has already run (the whole GPU suite), so the check runs on its
+    own (pytest tests/test_gpu_torch_crosscheck.py)."""
+    torch = pytest.importorskip("torch")
+    try:
+        torch.zeros(1, device="cuda:0")
+    except RuntimeError as e:
+        pytest.skip(f"torch cannot use the GPU in this process ({e})")
+    return torch
 
+
+def torch_steps(cfg, psi0, nsteps):
+    torch = torch_gpu()
     dev = torch.device("cuda:0")
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     kx = 2 * math.pi * torch.fft.fftfreq(nx, d=cfg.lx / nx, dtype=torch.float64, device=dev)
