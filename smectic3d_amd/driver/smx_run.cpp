@@ -8,7 +8,9 @@
 //           [--fp32] [--advect 0|1] [--force sh|mu] [--wmid W] [--ic noise|hyperboloid|pf3d|morph|dct|file:PATH]
 //           [--amp A] [--seed S] [--steps N] [--l1-limit L] [--l1-every K] [--save-every K]
 //           [--surf-every K] [--arm A] [--vsign S] [--out DIR] [--max-steps N] [--ic-only]
-//           [--resume CKPT] [--checkpoint CKPT] [--psi-avg NW]
+//           [--raw-ic] [--resume CKPT] [--checkpoint CKPT] [--psi-avg NW]
+// --raw-ic writes the initial condition as psiIC.raw (native float64, dump order) instead of
+// the text psiIC.dat
 // --psi-avg writes psiAvg.dat, the local amplitude field of the final state over windows
 // of NW points (cosFFT/comet/morphSurfAmp.cpp:1005-1134), in the psi dump format
 // --resume continues bit for bit from an exact binary checkpoint written by --checkpoint
@@ -47,6 +49,7 @@ struct Opts {
   long surf_every = 0;     // in L1 checks (0: off)
   long max_steps = 1000000;
   bool ic_only = false;  // write psiIC.dat and stop (no device needed)
+  bool raw_ic = false;   // write the IC as psiIC.raw (native float64, same order) instead of text
   int psi_avg_nw = 0;    // > 0: write psiAvg.dat of the final state
   std::string resume, checkpoint;  // exact binary checkpoint to continue from / to write at the end
   double arm = -1, vsign = 1.0;
@@ -89,16 +92,19 @@ std::vector<double> make_ic(const Opts& o) {
             at(i, j, k) = Amp * std::cos(c.q0 * k * dz) +
                           Amp * 0.1 * std::sin(c.q0 * k * dz) * (std::cos(Qi * i * dx) + std::cos(Qi * j * dy));
   } else if (o.ic == "morph" || o.ic == "dct") {
-    // pf3dMorph.cpp:331-383 (layers about the mid plane) or the bottom-anchored
-    // Neumann variant cosNoAdvMorph.cpp:289-332 (zs = k); aE, bE scaled from 512.
+    // pf3dMorph.cpp:331-383 (layers about the mid plane; aE = 245, bE = 180 at 512,
+    // pf3dMorph.cpp:94-96) or the bottom-anchored Neumann variant
+    // cosFFT/mesabi/cosNoAdvMorph.cpp:289-332 (zs = k; aE = 312, bE = 376 at 512,
+    // cosNoAdvMorph.cpp:81-83); both scaled by nz/512 on other grids.
     const double Amp = 1.328, mid = nz / 2;
-    const double aE = 245.0 * nz / 512, bE = 180.0 * nz / 512;
+    const bool morph = o.ic == "morph";
+    const double aE = (morph ? 245.0 : 312.0) * nz / 512, bE = (morph ? 180.0 : 376.0) * nz / 512;
     for (int i = 0; i < nx; ++i)
       for (int j = 0; j < ny; ++j)
         for (int k = 0; k < nz; ++k) {
-          const bool band = (o.ic == "morph") ? (k > 62 * nz / 512 && k < 450 * nz / 512) : (k < bE + 1);
+          const bool band = morph ? (k > 62 * nz / 512 && k < 450 * nz / 512) : (k < bE + 1);
           if (!band) continue;
-          const double xs = i - mid, ys = j - mid, zs = (o.ic == "morph") ? k - mid : (double)k;
+          const double xs = i - mid, ys = j - mid, zs = morph ? k - mid : (double)k;
           const double ds = std::sqrt(xs * xs + ys * ys);
           double v = 0.0;
           if (ds < mid) {
@@ -222,6 +228,7 @@ int run(int argc, char** argv) {
     else if (k == "--vsign" && has) o.vsign = num(a);
     else if (k == "--out" && has) o.out = argv[++a];
     else if (k == "--ic-only") o.ic_only = true;
+    else if (k == "--raw-ic") o.raw_ic = true;
     else if (k == "--resume" && has) o.resume = argv[++a];
     else if (k == "--checkpoint" && has) o.checkpoint = argv[++a];
     else if (k == "--psi-avg" && has) o.psi_avg_nw = (int)num(a);
@@ -241,7 +248,13 @@ int run(int argc, char** argv) {
 
   const std::string dir = o.out + "/";
   std::vector<double> psi = make_ic(o);
-  write_field(dir + "psiIC.dat", psi);
+  if (o.raw_ic) {
+    std::ofstream os(dir + "psiIC.raw", std::ios::binary);
+    os.write(reinterpret_cast<const char*>(psi.data()), (std::streamsize)(psi.size() * sizeof(double)));
+    if (!os) throw std::runtime_error("cannot write " + dir + "psiIC.raw");
+  } else {
+    write_field(dir + "psiIC.dat", psi);
+  }
   if (o.ic_only) return 0;
   smx_handle* h = nullptr;
   if (smx_create(&o.c, &h) != SMX_OK) throw std::runtime_error(std::string("smx_create: ") + smx_last_error());
