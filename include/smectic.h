@@ -107,13 +107,27 @@ int smx_get_psi(smx_handle *h, double *psi, int64_t row_stride);
  * cosFFT/mesabi/cosNoAdvMorph.cpp:483-530,619 (Neumann; with cfg.advect the
  * same advected step on the cosine grid, velocity components as sine series
  * along their own axis).
- * Asynchronous on the handle's stream. */
+ * Asynchronous on the handle's stream.  Every finite_every-th step (default 100, the
+ * reference's L1 cadence pf3d.cpp:489) the carried state is checked for NaN/Inf without
+ * waiting for the device; a check that found one makes the next smx_step and smx_sync
+ * return SMX_ERR_NONFINITE (the reference's only guard is its L1 test, which a NaN state
+ * turns into a NaN without stopping the loop).  With L1 tracking on (the default) the
+ * call keeps one device copy of the state from before its last step for smx_l1. */
 int smx_step(smx_handle *h, int64_t nsteps);
 
 /* L1 = sum|psi^{n+1}-psi^n| / (dt * sum|psi^{n+1}|) over the last step taken
  * by the most recent smx_step call, FP64 compensated.  Replaces the L1 block
- * pf3d.cpp:489-519 (Reduce x2 + Bcast). */
+ * pf3d.cpp:489-519 (Reduce x2 + Bcast).  SMX_ERR_STATE when L1 tracking is off or no
+ * step was taken since the state was set; SMX_ERR_NONFINITE for a NaN/Inf state. */
 int smx_l1(smx_handle *h, double *l1);
+
+/* L1 tracking (on = 1, the default): smx_step snapshots the state before its last step
+ * (one device copy of the spectral stage array per call) so that smx_l1 can compare the
+ * two.  Off: no snapshot, smx_l1 returns SMX_ERR_STATE.  No counterpart in the reference
+ * (it keeps psi and psiNew, pf3d.cpp:494-509). */
+int smx_set_l1_tracking(smx_handle *h, int on);
+/* Steps between the non-finite checks of smx_step (0 turns them off; default 100). */
+int smx_set_finite_check(smx_handle *h, int64_t every);
 
 /* Lyapunov energy F of the current state, FP64 compensated, spectral
  * (Parseval) quadrature of -eps/2 psi^2 + alpha/2((lap+q0^2)psi)^2 plus the
@@ -180,10 +194,12 @@ int smx_adjoint_grad(smx_handle *h, const double *psi0, int64_t row_stride, int6
                      double target_amp, double *loss, double *dpsi0, double *deps, double *dq0);
 /* Device memory the adjoint may spend on keeping recorded segments resident (N'(psi_n) and
  * psi_hat_n of whole checkpoint segments, recorded during the forward sweep and never
- * recomputed).  bytes < 0: 80% of the free device memory at the time of the call (default);
- * 0: checkpoints only (every segment recomputed once).  The gradient does not depend
- * on it.  The record sets stay allocated in the handle between smx_adjoint_grad calls;
- * this call and smx_destroy release them.  No counterpart in the reference. */
+ * recomputed).  bytes < 0 (default): up to 80% of the free device memory, for the duration
+ * of one smx_adjoint_grad call -- the workspace is released when the call returns;
+ * bytes >= 0: at most that many bytes, and the workspace stays allocated in the handle so
+ * that repeated calls of the same length reuse it (0: checkpoints only, every segment
+ * recomputed once).  The gradient does not depend on it.  smx_destroy and the next call of
+ * this function release a kept workspace.  No counterpart in the reference. */
 int smx_set_adjoint_budget(smx_handle *h, int64_t bytes);
 /* Bookkeeping of the last smx_adjoint_grad: steps recorded in the forward sweep, steps
  * recomputed from checkpoints, device bytes allocated for the sweep. */
@@ -206,6 +222,24 @@ int smx_batch_handle(smx_batch *b, int box, smx_handle **h);
 int smx_batch_run(smx_batch *b, int64_t nsteps, int64_t graph_steps);
 /* Number of kernel nodes in the captured graph (0 before the first run). */
 int smx_batch_graph_nodes(smx_batch *b, int64_t *nodes);
+
+/*
+ * Gradients of every box of a batch (periodic, non-advected, penalty-free boxes): for box i,
+ * J_i = 1/2 sum_x (psi_T(x) - target_amp[i] cos(q0_i z))^2 after nsteps CNAB2 steps from
+ * psi0[i] (host, reference layout, row stride >= that box's nz), and dJ_i/dpsi0 (into
+ * dpsi0[i]), dJ_i/deps, dJ_i/dq0 -- smx_adjoint_grad of each box, with checkpoints every
+ * ckpt_every steps, but the adjoint sweeps of all boxes are the parallel branches of one
+ * captured hipGraph, launched once.  Record budgets: a box with an explicit
+ * smx_set_adjoint_budget uses it and keeps its workspace (the batch then keeps the captured
+ * graph for the next call of the same shape when every box does); the other boxes share
+ * 80% of the free device memory in proportion to the records they need and release their
+ * workspaces on return.  Each box is left at its psi0.  No counterpart in the reference
+ * (it has no adjoint and runs one box per MPI job; the step is pf3d.cpp:471-481). */
+int smx_batch_adjoint_grad(smx_batch *b, const double *const *psi0, int64_t row_stride, int64_t nsteps,
+                           int64_t ckpt_every, const double *target_amp, double *loss, double *const *dpsi0,
+                           double *deps, double *dq0);
+/* Kernel and copy nodes of the last captured adjoint batch graph (0 before the first call). */
+int smx_batch_adjoint_nodes(smx_batch *b, int64_t *nodes);
 
 /*
  * Tile schedules of the four kernels of the periodic step, in launch order (slot 0: inverse
@@ -242,7 +276,8 @@ int smx_autotune(smx_handle *h, int reps, smx_schedule *chosen);
  * their fixed tiles); the next smx_batch_run captures its graph again with the new schedules. */
 int smx_batch_autotune(smx_batch *b, int reps);
 
-/* Block until the handle's stream is idle. */
+/* Block until the handle's stream is idle.  SMX_ERR_NONFINITE if a step check of an
+ * earlier smx_step found NaN/Inf in the state. */
 int smx_sync(smx_handle *h);
 
 /*

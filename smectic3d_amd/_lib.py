@@ -19,6 +19,7 @@ EXPORTS = [
     "smx_batch_handle", "smx_batch_run", "smx_batch_graph_nodes", "smx_get_schedule", "smx_set_schedule",
     "smx_autotune", "smx_batch_autotune", "smx_set_adjoint_budget", "smx_adjoint_info",
     "smx_save_checkpoint", "smx_load_checkpoint", "smx_psi_avg", "smx_eckhaus_scan",
+    "smx_set_l1_tracking", "smx_set_finite_check", "smx_batch_adjoint_grad", "smx_batch_adjoint_nodes",
 ]
 
 _lib = None
@@ -76,6 +77,11 @@ def lib():
     L.smx_save_checkpoint.argtypes = [vp, C.c_char_p]
     L.smx_load_checkpoint.argtypes = [vp, C.c_char_p]
     L.smx_adjoint_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.smx_set_l1_tracking.argtypes = [vp, C.c_int]
+    L.smx_set_finite_check.argtypes = [vp, C.c_int64]
+    L.smx_batch_adjoint_grad.argtypes = [vp, C.POINTER(_dp), C.c_int64, C.c_int64, C.c_int64, _dp, _dp,
+                                         C.POINTER(_dp), _dp, _dp]
+    L.smx_batch_adjoint_nodes.argtypes = [vp, C.POINTER(C.c_int64)]
     for n in EXPORTS:
         f = getattr(L, n)
         if n not in ("smx_config_default", "smx_destroy", "smx_last_error", "smx_batch_destroy"):

@@ -79,11 +79,13 @@ struct smx_handle {
   void* D1 = nullptr;  // FP64 diagnostic spectra
   void* D2 = nullptr;
   double* real_io = nullptr;
-  double* real_prev = nullptr;
+  double* real_prev = nullptr;  // FP64 scratch: psi^n of the L1 criterion, psi0 across the adjoint
+  void* Aprev = nullptr;        // stage buffer before the last step of an smx_step call (L1; on first use)
   double* partial = nullptr;  // z-pass FP64 partial sums (mixed path mean mode)
   double* mean_state = nullptr;
   double* red = nullptr;  // reduction partials
-  unsigned int* flag = nullptr;
+  unsigned int* flag = nullptr;   // [0]: scratch counter of check_finite; [1]: sticky count of the step checks
+  unsigned int* hflag = nullptr;  // pinned host copy of flag[1], refreshed after every step check
   void* Ax = nullptr;  // advected step: x-inverse of i kx psi_hat (then d_x psi at (x,y,kz))
   void* As = nullptr;  // x-inverse of opSH psi_hat; reused for S_x, U_x and N
   void* By = nullptr;  // d_y psi at (x,y,kz)
@@ -96,26 +98,35 @@ struct smx_handle {
   DevTables<double> t64;
   std::vector<void*> allocs;
   bool has_state = false, has_prev = false;
+  bool l1_track = true;         // snapshot the stage buffer before the last step of each smx_step call
+  int64_t finite_every = 100;   // steps between the non-finite checks of smx_step (0: off)
   int64_t steps = 0;
   PassSched sched[NSLOTS];  // tile schedules of the periodic step kernels
-  int64_t adj_budget = -1;  // device bytes the adjoint may keep resident (-1: 80% of free memory)
+  // device bytes the adjoint may spend on resident record sets: -1 (default) = a share of
+  // the free memory for the duration of one call; >= 0 = this many, kept between calls
+  int64_t adj_budget = -1;
   int64_t adj_info[3] = {0, 0, 0};  // last adjoint: resident steps, recomputed steps, device bytes
-  // adjoint record sets kept between calls (N'(psi_n) and psi_hat_n of one segment each)
-  std::vector<void*> adj_np, adj_psi;
-  size_t adj_np_bytes = 0, adj_psi_bytes = 0;
-  void* adj_stage = nullptr;  // adjoint stage and carry alpha, kept with the record sets
-  void* adj_carry = nullptr;
+  // workspace of one adjoint sweep (record sets, checkpoints, stage, carry, accumulators)
+  struct AdjPlan {
+    int64_t T = 0, K = 0, nseg = 0, nres = 0, sres = 0;
+    size_t sbytes = 0, npbytes = 0;
+    int nblk = 0;
+    std::vector<void*> setnp, setpsi;  // N'(psi_n) (K real fields) and psi_hat_n (K+1 spectra) per set
+    char* ck = nullptr;                // checkpoints (A, R) of the recomputed segments
+    double* ckmean = nullptr;          // their FP64 mean state
+    double* acc = nullptr;             // per-block parameter sums of the adjoint x pass
+    void* stage = nullptr;             // adjoint stage and carry alpha
+    void* carry = nullptr;
+    std::vector<void*> allocs;
+    uint64_t gen = 0;                  // bumped whenever the workspace is (re)allocated
+  } adj;
+  uint64_t adj_gen = 0;
 };
 
 static void adj_release(smx_handle* h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
-  for (void* q : h->adj_np) (void)hipFree(q);
-  for (void* q : h->adj_psi) (void)hipFree(q);
-  h->adj_np.clear();
-  h->adj_psi.clear();
-  if (h->adj_stage) (void)hipFree(h->adj_stage);
-  if (h->adj_carry) (void)hipFree(h->adj_carry);
-  h->adj_stage = h->adj_carry = nullptr;
+  for (void* q : h->adj.allocs) (void)hipFree(q);
+  h->adj = smx_handle::AdjPlan();
 }
 
 // FP64 partial sums of one z pass: one per tile at the narrowest tile (8 columns), plus
@@ -473,26 +484,29 @@ static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
 }
 
 // ---- FP64 diagnostics on a copy of the state ----
-static int state_to_d1(smx_handle* h) {
+static int stage_to_d1(smx_handle* h, const void* stage) {
   const int64_t ncomp = h->neumann ? h->nspec : 2 * h->nspec;  // real components
   if (h->fp32) {
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)h->A,
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)stage,
                        (double*)h->D1, ncomp);
     HIPCHK(hipGetLastError());
   } else {
-    HIPCHK(hipMemcpyAsync(h->D1, h->A, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->D1, stage, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
   }
   return SMX_OK;
 }
+static int state_to_d1(smx_handle* h) { return stage_to_d1(h, h->A); }
 
-// Real psi of the current state into `out` (dense [x][y][z], FP64).
-static int c2r_state(smx_handle* h, double* out) {
+// Real psi of a stage buffer (the x-inverse of psi_hat) into `out` (dense [x][y][z], FP64).
+static int c2r_stage(smx_handle* h, const void* stage, double* out) {
   PParams<double> p = params<double>(h, false);
-  CHK(state_to_d1(h));
+  CHK(stage_to_d1(h, stage));
   CHK(ypass<double>(h, h->D1, h->D1, +1));
   CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out));
   return SMX_OK;
 }
+// Real psi of the current state into `out`.
+static int c2r_state(smx_handle* h, double* out) { return c2r_stage(h, h->A, out); }
 
 // Inverse chain of a normalised full spectrum in D1 (destroyed) into a real field.
 static int c2r_spectrum_d1(smx_handle* h, double* out) {
@@ -636,12 +650,48 @@ static int dtpsi_field(smx_handle* h, double* out) {
   return c2r_spectrum_d1(h, out);
 }
 
+// Non-finite check of the carry R inside smx_step: adds the count to the sticky device
+// counter flag[1] and copies it to the pinned host word hflag, without waiting; smx_step
+// reads hflag on entry (a check that has completed by then stops the run) and smx_sync
+// reads it after the stream is idle.  The reference's only guard is its L1 test every 100
+// steps (pf3d.cpp:489-519), where a NaN state makes L1 a NaN and the loop keeps running.
+static int enqueue_finite_check(smx_handle* h) {
+  const int64_t ncomp = h->neumann ? h->nspec : 2 * h->nspec;
+  if (h->fp32)
+    hipLaunchKernelGGL(k_count_nonfinite<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)h->R,
+                       ncomp, h->flag + 1);
+  else
+    hipLaunchKernelGGL(k_count_nonfinite<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const double*)h->R,
+                       ncomp, h->flag + 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h->hflag, h->flag + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, h->st));
+  return SMX_OK;
+}
+
+static unsigned int nonfinite_seen(const smx_handle* h) {
+  return __atomic_load_n(h->hflag, __ATOMIC_ACQUIRE);
+}
+
+// Clears the step checks' counters and the L1 snapshot (a new state was installed).
+static int reset_checks(smx_handle* h) {
+  HIPCHK(hipMemsetAsync(h->flag, 0, sizeof(unsigned int) * 4, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  __atomic_store_n(h->hflag, 0u, __ATOMIC_RELEASE);
+  h->has_prev = false;
+  return SMX_OK;
+}
+
 template <typename T>
 static int run_steps(smx_handle* h, int64_t nsteps) {
   PParams<T> p = params<T>(h, h->fp32);
+  const size_t sbytes = h->wbytes * (size_t)h->nspec;
+  if (h->l1_track && !h->Aprev) CHK(dalloc(h, &h->Aprev, sbytes));
   for (int64_t n = 0; n < nsteps; ++n) {
-    if (n == nsteps - 1) CHK(c2r_state(h, h->real_prev));  // psi^n for the L1 criterion of the last step
+    // stage buffer of psi^n for the L1 criterion of the last step (smx_l1 transforms it)
+    if (n == nsteps - 1 && h->l1_track)
+      HIPCHK(hipMemcpyAsync(h->Aprev, h->A, sbytes, hipMemcpyDeviceToDevice, h->st));
     CHK(step_once<T>(h, p, nullptr));
+    if (h->finite_every > 0 && (h->steps + n + 1) % h->finite_every == 0) CHK(enqueue_finite_check(h));
   }
   return SMX_OK;
 }
@@ -674,95 +724,108 @@ static int profile_T(smx_handle* h, int64_t nsteps, std::vector<double>& acc) {
 
 
 // ---- adjoint (periodic, pure SH) ----
-template <typename T>
-static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* loss, double* deps, double* dq0) {
-  const smx_config& c = h->cfg;
+// One record set holds a segment's N'(psi_n) (K real fields in the working precision, layout
+// [z][x][y]) and its Psi_n = psi_hat_n (K+1 spectra).
+static size_t adj_setbytes(const smx_handle* h, int64_t K) {
+  const size_t sbytes = h->wbytes * (size_t)h->nspec;
+  const size_t npbytes = (h->fp32 ? 4 : 8) * (size_t)h->N;
+  return npbytes * (size_t)K + sbytes * (size_t)(K + 1);
+}
+
+static size_t adj_held(const smx_handle* h) {
+  const auto& a = h->adj;
+  if (!a.stage) return 0;
+  return adj_setbytes(h, a.K) * a.setnp.size() + 2 * a.sbytes * (size_t)(a.sres + 1);
+}
+
+// Default record budget of one handle: 80% of the device memory that is free or already
+// held by the handle's adjoint workspace.
+static int adj_default_budget(smx_handle* h, int64_t* budget) {
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  *budget = (int64_t)(0.8 * ((double)fr + (double)adj_held(h)));
+  return SMX_OK;
+}
+
+// Workspace of a T-step adjoint in segments of K steps with `budget` device bytes for
+// resident record sets.  Segments [sres, nseg) are recorded during the forward sweep and
+// never recomputed; the head [0, sres) is checkpointed (A, R and the FP64 mean mode) and
+// recomputed into set 0 during the reverse sweep.  A workspace of the same shape is reused.
+static int adj_prepare(smx_handle* h, int64_t T_, int64_t K, int64_t budget) {
+  auto& a = h->adj;
   const int64_t nseg = (T_ + K - 1) / K;
   const size_t sbytes = h->wbytes * (size_t)h->nspec;
-  const size_t npbytes = sizeof(T) * (size_t)h->N;
-  // One record set holds a segment's N'(psi_n) (K real fields, layout [z][x][y]) and its
-  // Psi_n = psi_hat_n (K+1 spectra).  Segments whose sets stay resident in HBM are recorded
-  // during the forward sweep and never recomputed; the others are checkpointed (A, R and
-  // the FP64 mean mode) and recomputed into set 0 during the reverse sweep.
-  const size_t setbytes = npbytes * (size_t)K + sbytes * (size_t)(K + 1);
-  std::vector<void*> tmp;
-  auto talloc = [&](void** pp, size_t bytes) -> int {
-    hipError_t e = hipMalloc(pp, bytes);
-    if (e != hipSuccess) {
-      *pp = nullptr;
-      return fail(SMX_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + ") for the adjoint: " + hipGetErrorString(e));
-    }
-    tmp.push_back(*pp);
+  const size_t npbytes = (h->fp32 ? 4 : 8) * (size_t)h->N;
+  const size_t setbytes = adj_setbytes(h, K);
+  const int64_t want = std::max<int64_t>(0, std::min<int64_t>(nseg, budget / (int64_t)setbytes));
+  if (a.stage && a.T == T_ && a.K == K && (a.nres == want || (int64_t)a.setnp.size() == std::max<int64_t>(want, 1)))
     return SMX_OK;
-  };
-  auto cleanup = [&]() {
-    (void)hipStreamSynchronize(h->st);
-    for (void* q : tmp) (void)hipFree(q);
-  };
-  int64_t budget = h->adj_budget;
-  if (budget < 0) {  // 80% of the device memory that is free or already held by the record sets
-    size_t fr = 0, tot = 0;
-    HIPCHK(hipMemGetInfo(&fr, &tot));
-    const double held = (double)h->adj_np.size() * (double)(h->adj_np_bytes + h->adj_psi_bytes);
-    budget = (int64_t)(0.8 * ((double)fr + held));
-  }
-  const int64_t nres_want = std::min<int64_t>(nseg, budget / (int64_t)setbytes);
-  void *AS = nullptr, *AC = nullptr;
-  double *ckmean = nullptr, *acc = nullptr;
-  const int nxb = c.ny / kNC, nblk = nxb * h->nplanes;
-  int rc = SMX_OK;
-  if ((rc = talloc((void**)&acc, sizeof(double) * 2 * nblk)) ||
-      (rc = talloc((void**)&ckmean, sizeof(double) * 2 * nseg))) {
-    cleanup();
-    return rc;
-  }
-  // record sets, as many as the budget allows (at least one for the recomputed segments);
-  // kept in the handle for the next call
-  if (h->adj_np_bytes != npbytes * (size_t)K || h->adj_psi_bytes != sbytes * (size_t)(K + 1)) {
-    adj_release(h);
-    h->adj_np_bytes = npbytes * (size_t)K;
-    h->adj_psi_bytes = sbytes * (size_t)(K + 1);
-  }
-  while ((int64_t)h->adj_np.size() < std::max<int64_t>(nres_want, 1)) {
-    void *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, h->adj_np_bytes) != hipSuccess || hipMalloc(&b, h->adj_psi_bytes) != hipSuccess) {
+  adj_release(h);
+  size_t bytes = 0;
+  auto al = [&](void** pp, size_t n) -> bool {
+    if (hipMalloc(pp, n) != hipSuccess) {
       (void)hipGetLastError();
-      if (a) (void)hipFree(a);
-      if (h->adj_np.empty()) {
-        cleanup();
+      *pp = nullptr;
+      return false;
+    }
+    a.allocs.push_back(*pp);
+    bytes += n;
+    return true;
+  };
+  a.T = T_;
+  a.K = K;
+  a.nseg = nseg;
+  a.sbytes = sbytes;
+  a.npbytes = npbytes;
+  a.nblk = (h->cfg.ny / kNC) * h->nplanes;
+  if (!al(&a.stage, sbytes) || !al(&a.carry, sbytes) || !al((void**)&a.acc, sizeof(double) * 2 * a.nblk) ||
+      !al((void**)&a.ckmean, sizeof(double) * 2 * nseg)) {
+    adj_release(h);
+    return fail(SMX_ERR_HIP, "no device memory for the adjoint stage, carry and accumulators");
+  }
+  while ((int64_t)a.setnp.size() < std::max<int64_t>(want, 1)) {
+    void *x = nullptr, *y = nullptr;
+    if (!al(&x, npbytes * (size_t)K) || !al(&y, sbytes * (size_t)(K + 1))) {
+      if (a.setnp.empty()) {
+        adj_release(h);
         return fail(SMX_ERR_HIP, "no device memory for one adjoint record set (" + std::to_string(setbytes) + " B)");
       }
       break;
     }
-    h->adj_np.push_back(a);
-    h->adj_psi.push_back(b);
+    a.setnp.push_back(x);
+    a.setpsi.push_back(y);
   }
-  // stage and carry after the record sets: a change of segment length releases the
-  // whole workspace above
-  if ((!h->adj_stage && hipMalloc(&h->adj_stage, sbytes) != hipSuccess) ||
-      (!h->adj_carry && hipMalloc(&h->adj_carry, sbytes) != hipSuccess)) {
-    cleanup();
-    return fail(SMX_ERR_HIP, "hipMalloc of the adjoint stage and carry failed");
+  a.nres = std::min<int64_t>(want, (int64_t)a.setnp.size());
+  a.sres = nseg - a.nres;
+  if (a.sres > 0 && !al((void**)&a.ck, 2 * sbytes * (size_t)a.sres)) {
+    adj_release(h);
+    return fail(SMX_ERR_HIP, "no device memory for the adjoint checkpoints");
   }
-  AS = h->adj_stage;
-  AC = h->adj_carry;
-  std::vector<char*> setnp, setpsi;
-  for (size_t i = 0; i < h->adj_np.size(); ++i) {
-    setnp.push_back((char*)h->adj_np[i]);
-    setpsi.push_back((char*)h->adj_psi[i]);
-  }
-  const int64_t nres = std::min<int64_t>(nres_want, (int64_t)setnp.size());
-  const int64_t sres = nseg - nres;  // segments [sres, nseg) are resident
-  char* ck = nullptr;
-  if (sres > 0 && (rc = talloc((void**)&ck, 2 * sbytes * (size_t)sres))) {
-    cleanup();
-    return rc;
-  }
-  h->adj_info[1] = std::min<int64_t>(T_, sres * K);  // recomputed steps
-  h->adj_info[0] = T_ - h->adj_info[1];               // resident steps
-  h->adj_info[2] = (int64_t)(2 * sbytes + sizeof(double) * (2 * nblk + 2 * nseg) + setbytes * setnp.size() +
-                             (ck ? 2 * sbytes * (size_t)sres : 0));
+  a.gen = ++h->adj_gen;
+  h->adj_info[1] = std::min<int64_t>(T_, a.sres * K);  // recomputed steps
+  h->adj_info[0] = T_ - h->adj_info[1];                // resident steps
+  h->adj_info[2] = (int64_t)bytes;
+  return SMX_OK;
+}
+
+// Device work of the adjoint sweep from the handle's current state (psi0, bootstrapped):
+// forward sweep, target residual, reverse sweep, dJ/dpsi0 into real_io.  Only kernel
+// launches and device-to-device copies on the handle's stream (no host waits, no
+// allocation), so a batch can capture it into a graph.  The loss partials stay in red, the
+// parameter sums in the workspace's accumulators (adj_finish reads both).
+template <typename T>
+static int adj_enqueue(smx_handle* h, double amp) {
+  const smx_config& c = h->cfg;
+  auto& a = h->adj;
+  const int64_t T_ = a.T, K = a.K, nseg = a.nseg, sres = a.sres;
+  const size_t sbytes = a.sbytes, npbytes = a.npbytes;
+  void* AS = a.stage;
+  void* AC = a.carry;
+  char* ck = a.ck;
+  double* ckmean = a.ckmean;
   PParams<T> p = params<T>(h, h->fp32);
+  auto setnp = [&](int64_t set) { return (char*)a.setnp[set]; };
+  auto setpsi = [&](int64_t set) { return (char*)a.setpsi[set]; };
   auto save = [&](int64_t seg) -> int {
     HIPCHK(hipMemcpyAsync(ck + 2 * sbytes * seg, h->A, sbytes, hipMemcpyDeviceToDevice, h->st));
     HIPCHK(hipMemcpyAsync(ck + 2 * sbytes * seg + sbytes, h->R, sbytes, hipMemcpyDeviceToDevice, h->st));
@@ -775,97 +838,106 @@ static int adjoint_T(smx_handle* h, int64_t T_, int64_t K, double amp, double* l
     HIPCHK(hipMemcpyAsync(h->mean_state, ckmean + 2 * seg, 2 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
     return SMX_OK;
   };
-  // steps [s0, s0+len) from the current state, recording N'(psi_n) and Psi_n into set `set`
+  // `len` steps from the current state, recording N'(psi_n) and Psi_n into set `set`
   auto record = [&](int64_t set, int64_t len) -> int {
     PParams<T> px = params<T>(h, false);
     px.scale = (T)(1.0 / (double)c.nx);  // psi_hat_{s0} from the stage buffer
-    CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, setpsi[set], nullptr, nullptr));
+    CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, setpsi(set), nullptr, nullptr));
     for (int64_t j = 0; j < len; ++j) {
       PParams<T> pr = p;
-      pr.rec_np = (T*)(setnp[set] + npbytes * j);
-      pr.rec_psi = (cplx<T>*)(setpsi[set] + sbytes * (j + 1));
+      pr.rec_np = (T*)(setnp(set) + npbytes * j);
+      pr.rec_psi = (cplx<T>*)(setpsi(set) + sbytes * (j + 1));
       CHK(step_once<T>(h, pr, nullptr));
     }
     return SMX_OK;
   };
-  auto run = [&]() -> int {
-    // forward sweep: checkpoints for the recomputed head, records for the resident tail
-    for (int64_t seg = 0; seg < nseg; ++seg) {
-      const int64_t len = std::min(K, T_ - seg * K);
-      if (seg < sres) {
-        CHK(save(seg));
-        for (int64_t j = 0; j < len; ++j) CHK(step_once<T>(h, p, nullptr));
-      } else {
-        CHK(record(seg - sres, len));
-      }
-    }
-    // a_T = psi_T - g, the loss and the explicit q0 term of the target
-    CHK(c2r_state(h, h->real_io));
-    hipLaunchKernelGGL(k_adj_target, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->real_io, h->N, c.nz, h->dz,
-                       c.q0, amp, h->red);
-    HIPCHK(hipGetLastError());
-    std::vector<double> tsum;
-    CHK(host_sum(h, 2, kRedBlocks, tsum));
-    // alpha_T = F a_T; first adjoint stage w_{T-1} = dt/2/C2 c_{T-1} alpha_T
-    PParams<T> pa = p;
-    pa.partial = nullptr;
-    CHK(zpass<T>(h, pa, Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, AS, h->real_io, nullptr));
-    CHK(ypass<T>(h, AS, AS, -1));
-    HIPCHK(hipMemsetAsync(acc, 0, sizeof(double) * 2 * nblk, h->st));
-    {  // Psi_T = psi_hat_T (from the stage buffer) for the Psi_T half of step T-1
-      PParams<T> px = params<T>(h, false);
-      px.scale = (T)(1.0 / (double)c.nx);
-      CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, h->P, nullptr, nullptr));
-    }
-    HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, (const cplx<T>*)h->P, pa, (T)(T_ - 1 >= 1 ? 3 : 2),
-                                    1, 1, c.q0, c.alpha, acc, h->st));
-    // reverse sweep over the segments
-    for (int64_t seg = nseg - 1; seg >= 0; --seg) {
-      const int64_t s0 = seg * K, len = std::min(K, T_ - s0);
-      int64_t set = seg - sres;
-      if (seg < sres) {
-        set = 0;
-        CHK(restore(seg));
-        CHK(record(0, len));
-      }
-      for (int64_t j = len - 1; j >= 0; --j) {
-        const int64_t n = s0 + j;
-        PParams<T> pm = pa;
-        pm.mul_np = (const T*)(setnp[set] + npbytes * j);
-        CHK(ypass<T>(h, AS, AS, +1, h->sched[SLOT_YINV]));
-        CHK(zpass<T>(h, pm, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_MUL, AS, AS, nullptr, nullptr, h->sched[SLOT_Z]));
-        CHK(ypass<T>(h, AS, AS, -1, h->sched[SLOT_YFWD]));
-        const cplx<T>* psin = (const cplx<T>*)(setpsi[set] + sbytes * j);
-        HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, psin, pa, (T)(n - 1 >= 1 ? 3 : 2), 0,
-                                        n >= 1 ? 1 : 0, c.q0, c.alpha, acc, h->st));
-      }
-    }
-    // dJ/dpsi0 = F^-1 alpha_0 (FP64 inverse chain)
-    const int64_t ncomp = 2 * h->nspec;
-    if (h->fp32) {
-      hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)AC, (double*)h->D1,
-                         ncomp);
-      HIPCHK(hipGetLastError());
+  // forward sweep: checkpoints for the recomputed head, records for the resident tail
+  for (int64_t seg = 0; seg < nseg; ++seg) {
+    const int64_t len = std::min(K, T_ - seg * K);
+    if (seg < sres) {
+      CHK(save(seg));
+      for (int64_t j = 0; j < len; ++j) CHK(step_once<T>(h, p, nullptr));
     } else {
-      HIPCHK(hipMemcpyAsync(h->D1, AC, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
+      CHK(record(seg - sres, len));
     }
-    CHK(c2r_spectrum_d1(h, h->real_io));
-    std::vector<double> ha(2 * (size_t)nblk);
-    HIPCHK(hipMemcpyAsync(ha.data(), acc, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, h->st));
-    HIPCHK(hipStreamSynchronize(h->st));
-    double se = 0, sq = 0;
-    for (int b = 0; b < nblk; ++b) {
-      se += ha[2 * b];
-      sq += ha[2 * b + 1];
+  }
+  // a_T = psi_T - g, the loss and the explicit q0 term of the target (partials in red)
+  CHK(c2r_state(h, h->real_io));
+  hipLaunchKernelGGL(k_adj_target, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->real_io, h->N, c.nz, h->dz, c.q0,
+                     amp, h->red);
+  HIPCHK(hipGetLastError());
+  // alpha_T = F a_T; first adjoint stage w_{T-1} = dt/2/C2 c_{T-1} alpha_T
+  PParams<T> pa = p;
+  pa.partial = nullptr;
+  CHK(zpass<T>(h, pa, Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, AS, h->real_io, nullptr));
+  CHK(ypass<T>(h, AS, AS, -1));
+  hipLaunchKernelGGL(k_zero_f64, dim3((2 * a.nblk + 255) / 256), dim3(256), 0, h->st, a.acc, (int64_t)(2 * a.nblk));
+  HIPCHK(hipGetLastError());
+  {  // Psi_T = psi_hat_T (from the stage buffer) for the Psi_T half of step T-1
+    PParams<T> px = params<T>(h, false);
+    px.scale = (T)(1.0 / (double)c.nx);
+    CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, h->P, nullptr, nullptr));
+  }
+  HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, (const cplx<T>*)h->P, pa, (T)(T_ - 1 >= 1 ? 3 : 2), 1, 1,
+                                  c.q0, c.alpha, a.acc, h->st));
+  // reverse sweep over the segments
+  for (int64_t seg = nseg - 1; seg >= 0; --seg) {
+    const int64_t s0 = seg * K, len = std::min(K, T_ - s0);
+    int64_t set = seg - sres;
+    if (seg < sres) {
+      set = 0;
+      CHK(restore(seg));
+      CHK(record(0, len));
     }
-    *loss = tsum[0];
-    *deps = (double)h->N * se;  // Parseval: sum_x a b = N sum_k w Re(conj alpha beta)
-    *dq0 = (double)h->N * sq + tsum[1];
-    return SMX_OK;
-  };
-  rc = run();
-  cleanup();
-  return rc;
+    for (int64_t j = len - 1; j >= 0; --j) {
+      const int64_t n = s0 + j;
+      PParams<T> pm = pa;
+      pm.mul_np = (const T*)(setnp(set) + npbytes * j);
+      CHK(ypass<T>(h, AS, AS, +1, h->sched[SLOT_YINV]));
+      CHK(zpass<T>(h, pm, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_MUL, AS, AS, nullptr, nullptr, h->sched[SLOT_Z]));
+      CHK(ypass<T>(h, AS, AS, -1, h->sched[SLOT_YFWD]));
+      const cplx<T>* psin = (const cplx<T>*)(setpsi(set) + sbytes * j);
+      HIPCHK(AdjointOps<T>::xpass_adj((cplx<T>*)AS, (cplx<T>*)AC, psin, pa, (T)(n - 1 >= 1 ? 3 : 2), 0, n >= 1 ? 1 : 0,
+                                      c.q0, c.alpha, a.acc, h->st));
+    }
+  }
+  // dJ/dpsi0 = F^-1 alpha_0 (FP64 inverse chain)
+  const int64_t ncomp = 2 * h->nspec;
+  if (h->fp32) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)AC, (double*)h->D1,
+                       ncomp);
+    HIPCHK(hipGetLastError());
+  } else {
+    HIPCHK(hipMemcpyAsync(h->D1, AC, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
+  }
+  return c2r_spectrum_d1(h, h->real_io);
+}
+
+// Host side of the adjoint's results after adj_enqueue: waits for the stream, sums the loss
+// partials and the parameter sums in a fixed order.
+static int adj_finish(smx_handle* h, double* loss, double* deps, double* dq0) {
+  std::vector<double> tsum;
+  CHK(host_sum(h, 2, kRedBlocks, tsum));
+  const int nblk = h->adj.nblk;
+  std::vector<double> ha(2 * (size_t)nblk);
+  HIPCHK(hipMemcpyAsync(ha.data(), h->adj.acc, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  double se = 0, sq = 0;
+  for (int b = 0; b < nblk; ++b) {
+    se += ha[2 * b];
+    sq += ha[2 * b + 1];
+  }
+  *loss = tsum[0];
+  *deps = (double)h->N * se;  // Parseval: sum_x a b = N sum_k w Re(conj alpha beta)
+  *dq0 = (double)h->N * sq + tsum[1];
+  return SMX_OK;
+}
+
+static int adj_check(const smx_handle* h) {
+  const smx_config& c = h->cfg;
+  if (h->neumann || c.advect || c.w_mid != 0.0)
+    return fail(SMX_ERR_UNSUPPORTED, "the adjoint covers the periodic, non-advected, penalty-free step");
+  return SMX_OK;
 }
 
 
@@ -878,7 +950,20 @@ struct smx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t graph_steps = 0;
+  // captured adjoint sweeps of every box, kept while every box keeps its workspace
+  hipGraph_t adj_graph = nullptr;
+  hipGraphExec_t adj_exec = nullptr;
+  std::vector<double> adj_key;  // workspace generations, step counts, amplitudes and schedules
+  int64_t adj_nodes = 0;
 };
+
+static void batch_drop_adj_graph(smx_batch* b) {
+  if (b->adj_exec) (void)hipGraphExecDestroy(b->adj_exec);
+  if (b->adj_graph) (void)hipGraphDestroy(b->adj_graph);
+  b->adj_exec = nullptr;
+  b->adj_graph = nullptr;
+  b->adj_key.clear();
+}
 
 static int box_steps(smx_handle* h, int64_t n) {
   for (int64_t i = 0; i < n; ++i) {
@@ -1106,9 +1191,8 @@ static int field_to_host(smx_handle* h, const double* dev, double* out, int64_t 
 static int state_from_real_io(smx_handle* h) {
   HIPCHK(hipMemsetAsync(h->mean_state, 0, sizeof(double) * 8, h->st));
   CHK(h->fp32 ? bootstrap<float>(h) : bootstrap<double>(h));
-  HIPCHK(hipStreamSynchronize(h->st));
+  CHK(reset_checks(h));
   h->has_state = true;
-  h->has_prev = false;
   h->steps = 0;
   return SMX_OK;
 }
@@ -1242,8 +1326,12 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
   if ((rc = upload_tables<double>(h, h->t64))) return bail(rc);
   if (h->fp32 && (rc = upload_tables<float>(h, h->t32))) return bail(rc);
   if ((e = hipMemset(h->mean_state, 0, sizeof(double) * 8)) != hipSuccess ||
-      (e = hipMemset(h->surf, 0, sizeof(double) * 4 * h->nxy)) != hipSuccess)
+      (e = hipMemset(h->surf, 0, sizeof(double) * 4 * h->nxy)) != hipSuccess ||
+      (e = hipMemset(h->flag, 0, sizeof(unsigned int) * 4)) != hipSuccess)
     return bail(fail(SMX_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e)));
+  if ((e = hipHostMalloc((void**)&h->hflag, sizeof(unsigned int) * 4, hipHostMallocDefault)) != hipSuccess)
+    return bail(fail(SMX_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e)));
+  std::memset(h->hflag, 0, sizeof(unsigned int) * 4);
   *out = h;
   return SMX_OK;
 }
@@ -1253,6 +1341,7 @@ void smx_destroy(smx_handle* h) {
   adj_release(h);
   if (h->st) (void)hipStreamSynchronize(h->st);
   for (void* p : h->allocs) (void)hipFree(p);
+  if (h->hflag) (void)hipHostFree(h->hflag);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
 }
@@ -1286,17 +1375,34 @@ int smx_step(smx_handle* h, int64_t nsteps) {
   if (!h) return fail(SMX_ERR_INVALID, "null handle");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
   if (nsteps < 0) return fail(SMX_ERR_INVALID, "nsteps < 0");
+  if (const unsigned int bad = nonfinite_seen(h))
+    return fail(SMX_ERR_NONFINITE, std::to_string(bad) + " non-finite values in the state at an earlier step check");
   if (nsteps == 0) return SMX_OK;
   HIPCHK(hipSetDevice(h->cfg.device));
   CHK(h->fp32 ? run_steps<float>(h, nsteps) : run_steps<double>(h, nsteps));
   h->steps += nsteps;
-  h->has_prev = true;
+  h->has_prev = h->l1_track;
   return SMX_OK;
 }
 
 int smx_sync(smx_handle* h) {
   if (!h) return fail(SMX_ERR_INVALID, "null handle");
   HIPCHK(hipStreamSynchronize(h->st));
+  if (const unsigned int bad = nonfinite_seen(h))
+    return fail(SMX_ERR_NONFINITE, std::to_string(bad) + " non-finite values in the state at a step check");
+  return SMX_OK;
+}
+
+int smx_set_l1_tracking(smx_handle* h, int on) {
+  if (!h) return fail(SMX_ERR_INVALID, "null handle");
+  h->l1_track = on != 0;
+  if (!h->l1_track) h->has_prev = false;
+  return SMX_OK;
+}
+
+int smx_set_finite_check(smx_handle* h, int64_t every) {
+  if (!h || every < 0) return fail(SMX_ERR_INVALID, "bad argument");
+  h->finite_every = every;
   return SMX_OK;
 }
 
@@ -1370,12 +1476,15 @@ int smx_load_checkpoint(smx_handle* h, const char* path) {
   hsh = fnv1a(hsh, mean, sizeof(mean));
   std::fclose(f);
   if (hsh != stored) return fail(SMX_ERR_INVALID, std::string(path) + ": checksum mismatch");
+  // steps still queued on the handle's stream must not overwrite the loaded state: the copies
+  // are ordered behind them on the same stream
   for (size_t i = 0; i < arrs.size(); ++i)
-    HIPCHK(hipMemcpy(arrs[i], data[i].data(), abytes, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->mean_state, mean, sizeof(mean), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(arrs[i], data[i].data(), abytes, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->mean_state, mean, sizeof(mean), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));  // the host buffers are freed on return
+  CHK(reset_checks(h));
   h->steps = hd.steps;
   h->has_state = true;
-  h->has_prev = false;
   return SMX_OK;
 }
 
@@ -1387,9 +1496,11 @@ int smx_step_count(const smx_handle* h, int64_t* n) {
 
 int smx_l1(smx_handle* h, double* l1) {
   if (!h || !l1) return fail(SMX_ERR_INVALID, "null argument");
-  if (!h->has_prev) return fail(SMX_ERR_STATE, "no step taken since smx_set_psi");
+  if (!h->has_prev)
+    return fail(SMX_ERR_STATE, "no L1 snapshot: no smx_step since the state was set, or L1 tracking is off");
   HIPCHK(hipSetDevice(h->cfg.device));
   CHK(check_finite(h));
+  CHK(c2r_stage(h, h->Aprev, h->real_prev));
   CHK(c2r_state(h, h->real_io));
   hipLaunchKernelGGL(k_reduce_l1, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, h->real_io, h->real_prev, h->N,
                      h->red);
@@ -1553,18 +1664,22 @@ int smx_adjoint_grad(smx_handle* h, const double* psi0, int64_t row_stride, int6
                      double target_amp, double* loss, double* dpsi0, double* deps, double* dq0) {
   if (!h || !psi0 || !loss || !dpsi0 || !deps || !dq0) return fail(SMX_ERR_INVALID, "null argument");
   const smx_config& c = h->cfg;
-  if (h->neumann || c.advect || c.w_mid != 0.0)
-    return fail(SMX_ERR_UNSUPPORTED, "the adjoint covers the periodic, non-advected, penalty-free step");
+  CHK(adj_check(h));
   if (nsteps < 1 || ckpt_every < 1) return fail(SMX_ERR_INVALID, "nsteps and ckpt_every must be >= 1");
   if (row_stride < c.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
   CHK(smx_set_psi(h, psi0, row_stride));
-  // psi0 stays on the device (the L1 snapshot buffer) to leave the handle at psi0 afterwards
+  // psi0 stays on the device (FP64 scratch) to leave the handle at psi0 afterwards
   const size_t rbytes = sizeof(double) * (size_t)h->N;
   HIPCHK(hipMemcpyAsync(h->real_prev, h->real_io, rbytes, hipMemcpyDeviceToDevice, h->st));
   const int64_t K = std::min(ckpt_every, nsteps);
-  CHK(h->fp32 ? adjoint_T<float>(h, nsteps, K, target_amp, loss, deps, dq0)
-              : adjoint_T<double>(h, nsteps, K, target_amp, loss, deps, dq0));
-  CHK(field_to_host(h, h->real_io, dpsi0, row_stride));
+  int64_t budget = h->adj_budget;
+  if (budget < 0) CHK(adj_default_budget(h, &budget));
+  CHK(adj_prepare(h, nsteps, K, budget));
+  int rc = h->fp32 ? adj_enqueue<float>(h, target_amp) : adj_enqueue<double>(h, target_amp);
+  if (rc == SMX_OK) rc = adj_finish(h, loss, deps, dq0);
+  if (rc == SMX_OK) rc = field_to_host(h, h->real_io, dpsi0, row_stride);
+  if (h->adj_budget < 0) adj_release(h);  // the default budget lends memory for this call only
+  CHK(rc);
   HIPCHK(hipMemcpyAsync(h->real_io, h->real_prev, rbytes, hipMemcpyDeviceToDevice, h->st));
   return state_from_real_io(h);
 }
@@ -1662,6 +1777,7 @@ int smx_batch_create(int nbox, const smx_config* cfgs, smx_batch** out) {
 void smx_batch_destroy(smx_batch* b) {
   if (!b) return;
   if (b->st) (void)hipStreamSynchronize(b->st);
+  batch_drop_adj_graph(b);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto e : b->joins)
@@ -1702,7 +1818,8 @@ int smx_batch_autotune(smx_batch* b, int reps) {
   if (!b) return fail(SMX_ERR_INVALID, "null batch");
   for (smx_handle* h : b->boxes)  // Neumann boxes keep their fixed tiles
     if (!h->neumann) CHK(smx_autotune(h, reps, nullptr));
-  // the captured graph holds the old schedules: capture again on the next run
+  // the captured graphs hold the old schedules: capture again on the next run
+  batch_drop_adj_graph(b);
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
@@ -1719,6 +1836,116 @@ int smx_batch_graph_nodes(smx_batch* b, int64_t* nodes) {
   size_t n = 0;
   if (b->graph) HIPCHK(hipGraphGetNodes(b->graph, nullptr, &n));
   *nodes = (int64_t)n;
+  return SMX_OK;
+}
+
+// Batched gradients: every box's adjoint sweep (adj_enqueue) is a parallel branch of one
+// captured graph, launched once.  Record budgets: a box with an explicit budget uses it
+// and keeps its workspace (and the batch keeps the graph when every box does); the other
+// boxes share 80% of the free device memory in proportion to what their full records need,
+// and release their workspaces at the end of the call.
+int smx_batch_adjoint_grad(smx_batch* b, const double* const* psi0, int64_t row_stride, int64_t nsteps,
+                           int64_t ckpt_every, const double* target_amp, double* loss, double* const* dpsi0,
+                           double* deps, double* dq0) {
+  if (!b || !psi0 || !target_amp || !loss || !dpsi0 || !deps || !dq0) return fail(SMX_ERR_INVALID, "null argument");
+  if (nsteps < 1 || ckpt_every < 1) return fail(SMX_ERR_INVALID, "nsteps and ckpt_every must be >= 1");
+  const size_t nb = b->boxes.size();
+  for (size_t i = 0; i < nb; ++i) {
+    CHK(adj_check(b->boxes[i]));
+    if (!psi0[i] || !dpsi0[i]) return fail(SMX_ERR_INVALID, "null field pointer for box " + std::to_string(i));
+    if (row_stride < b->boxes[i]->cfg.nz) return fail(SMX_ERR_INVALID, "row_stride < nz");
+  }
+  HIPCHK(hipSetDevice(b->boxes[0]->cfg.device));
+  const int64_t K = std::min(ckpt_every, nsteps), nseg = (nsteps + K - 1) / K;
+  for (size_t i = 0; i < nb; ++i) {
+    smx_handle* h = b->boxes[i];
+    CHK(smx_set_psi(h, psi0[i], row_stride));
+    HIPCHK(hipMemcpyAsync(h->real_prev, h->real_io, sizeof(double) * (size_t)h->N, hipMemcpyDeviceToDevice, h->st));
+  }
+  // record budgets
+  double need_shared = 0.0, held = 0.0;
+  for (smx_handle* h : b->boxes)
+    if (h->adj_budget < 0) {
+      need_shared += (double)adj_setbytes(h, K) * (double)nseg;
+      held += (double)adj_held(h);
+    }
+  double avail = 0.0;
+  if (need_shared > 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    avail = 0.8 * ((double)fr + held);
+  }
+  bool keep_all = true;
+  for (smx_handle* h : b->boxes) {
+    int64_t budget = h->adj_budget;
+    if (budget < 0) {
+      const double need = (double)adj_setbytes(h, K) * (double)nseg;
+      budget = (int64_t)std::min(need, avail * need / need_shared);
+      keep_all = false;
+    }
+    CHK(adj_prepare(h, nsteps, K, budget));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  std::vector<double> key;
+  key.push_back((double)nsteps);
+  key.push_back((double)K);
+  for (size_t i = 0; i < nb; ++i) {
+    const smx_handle* h = b->boxes[i];
+    key.push_back((double)h->adj.gen);
+    key.push_back(target_amp[i]);
+    for (int s = 0; s < NSLOTS; ++s) {
+      key.push_back((double)h->sched[s].nc);
+      key.push_back((double)h->sched[s].v);
+    }
+  }
+  if (!b->adj_exec || b->adj_key != key) {
+    batch_drop_adj_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    int rc = SMX_OK;
+    hipError_t e = hipEventRecord(b->fork, b->st);
+    for (size_t i = 0; i < nb && rc == SMX_OK && e == hipSuccess; ++i) {
+      smx_handle* h = b->boxes[i];
+      e = hipStreamWaitEvent(h->st, b->fork, 0);
+      if (e == hipSuccess) rc = h->fp32 ? adj_enqueue<float>(h, target_amp[i]) : adj_enqueue<double>(h, target_amp[i]);
+      if (rc == SMX_OK && e == hipSuccess) e = hipEventRecord(b->joins[i], h->st);
+      if (rc == SMX_OK && e == hipSuccess) e = hipStreamWaitEvent(b->st, b->joins[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->st, &g);
+    if (rc != SMX_OK || e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc != SMX_OK) return rc;
+      return fail(SMX_ERR_HIP, std::string("adjoint batch capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->adj_graph = g;
+    size_t n = 0;
+    HIPCHK(hipGraphGetNodes(g, nullptr, &n));
+    b->adj_nodes = (int64_t)n;
+    HIPCHK(hipGraphInstantiate(&b->adj_exec, g, nullptr, nullptr, 0));
+    b->adj_key = key;
+  }
+  HIPCHK(hipGraphLaunch(b->adj_exec, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  int rc = SMX_OK;
+  for (size_t i = 0; i < nb; ++i) {
+    smx_handle* h = b->boxes[i];
+    if (rc == SMX_OK) rc = adj_finish(h, &loss[i], &deps[i], &dq0[i]);
+    if (rc == SMX_OK) rc = field_to_host(h, h->real_io, dpsi0[i], row_stride);
+  }
+  if (!keep_all) batch_drop_adj_graph(b);  // the graph refers to workspaces released below
+  for (smx_handle* h : b->boxes) {
+    if (h->adj_budget < 0) adj_release(h);
+    if (rc == SMX_OK) {
+      HIPCHK(hipMemcpyAsync(h->real_io, h->real_prev, sizeof(double) * (size_t)h->N, hipMemcpyDeviceToDevice, h->st));
+      rc = state_from_real_io(h);
+    }
+  }
+  return rc;
+}
+
+int smx_batch_adjoint_nodes(smx_batch* b, int64_t* nodes) {
+  if (!b || !nodes) return fail(SMX_ERR_INVALID, "null argument");
+  *nodes = b->adj_nodes;
   return SMX_OK;
 }
 

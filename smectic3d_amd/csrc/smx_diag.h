@@ -189,6 +189,12 @@ __global__ void k_count_nonfinite(const T* __restrict__ a, int64_t n, unsigned i
   if (c) atomicAdd(out, c);
 }
 
+// Zero fill (a kernel rather than a memset node, so that captured graphs replay it reliably).
+__global__ void k_zero_f64(double* __restrict__ a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.0;
+}
+
 __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = (double)a[i];

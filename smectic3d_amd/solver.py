@@ -66,7 +66,16 @@ class Solver:
         check(lib().smx_step(self._h, int(n)))
 
     def sync(self):
+        """Wait for the handle's stream; raises SMX_ERR_NONFINITE if a step check found NaN/Inf."""
         check(lib().smx_sync(self._h))
+
+    def set_l1_tracking(self, on):
+        """on: smx_step keeps the state before its last step for l1() (one device copy per call)."""
+        check(lib().smx_set_l1_tracking(self._h, 1 if on else 0))
+
+    def set_finite_check(self, every):
+        """Steps between the non-finite checks of step() (0: off; default 100)."""
+        check(lib().smx_set_finite_check(self._h, int(every)))
 
     def step_count(self):
         n = C.c_int64()
@@ -136,7 +145,9 @@ class Solver:
         return J.value, d, de.value, dq.value
 
     def set_adjoint_budget(self, nbytes):
-        """Device bytes the adjoint may keep resident (-1: 80% of free memory; 0: checkpoints only)."""
+        """Device bytes of resident adjoint records: -1 (default) lends up to 80% of the free
+        memory for one call; >= 0 keeps a workspace of at most that many bytes between calls
+        (0: checkpoints only)."""
         check(lib().smx_set_adjoint_budget(self._h, int(nbytes)))
 
     def adjoint_info(self):
@@ -204,6 +215,38 @@ class Batch:
     def graph_nodes(self):
         n = C.c_int64()
         check(lib().smx_batch_graph_nodes(self._b, C.byref(n)))
+        return n.value
+
+    def adjoint_grad(self, psi0s, nsteps, ckpt_every=50, target_amps=0.5):
+        """Gradients of every box in one captured graph: lists (J, dJ/dpsi0, dJ/deps, dJ/dq0),
+        J_i = 1/2 sum (psi_T - target_amps[i] cos(q0_i z))^2.  Every box is left at its psi0."""
+        nb = len(self.boxes)
+        psi0s = [np.ascontiguousarray(p, dtype=np.float64).reshape(b.shape) for p, b in zip(psi0s, self.boxes)]
+        if len(psi0s) != nb:
+            raise ValueError("one psi0 per box")
+        nz = {b.cfg.nz for b in self.boxes}
+        stride = max(nz)
+        if len(nz) > 1:  # one row stride for every box: pad the rows of the shorter boxes
+            ins = []
+            for p in psi0s:
+                q = np.zeros(p.shape[:2] + (stride,))
+                q[..., :p.shape[2]] = p
+                ins.append(q)
+        else:
+            ins = psi0s
+        outs = [np.zeros(p.shape[:2] + (stride,)) for p in psi0s]
+        amps = np.ascontiguousarray(np.broadcast_to(np.asarray(target_amps, dtype=np.float64), (nb,)))
+        J, de, dq = np.zeros(nb), np.zeros(nb), np.zeros(nb)
+        pin = (_dp * nb)(*[_ptr(p) for p in ins])
+        pout = (_dp * nb)(*[_ptr(o) for o in outs])
+        check(lib().smx_batch_adjoint_grad(self._b, pin, stride, int(nsteps), int(ckpt_every), _ptr(amps), _ptr(J),
+                                           pout, _ptr(de), _ptr(dq)))
+        grads = [o[..., :p.shape[2]].copy() for o, p in zip(outs, psi0s)]
+        return list(J), grads, list(de), list(dq)
+
+    def adjoint_nodes(self):
+        n = C.c_int64()
+        check(lib().smx_batch_adjoint_nodes(self._b, C.byref(n)))
         return n.value
 
     def close(self):
