@@ -115,6 +115,19 @@ int smx_get_psi(smx_handle *h, double *psi, int64_t row_stride);
  * call keeps one device copy of the state from before its last step for smx_l1. */
 int smx_step(smx_handle *h, int64_t nsteps);
 
+/* smx_step for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  The same
+ * steps, kernels and results as smx_step, with two differences: the FP64 partial sums of the
+ * z passes (FP32-mixed mean mode) go to a static device array of 512*512/8 + 64 doubles
+ * (262,656 bytes, one per device, held until the process ends) instead of the handle's buffer,
+ * and no L1 snapshot is kept (smx_l1 then returns SMX_ERR_STATE).  Allocates nothing and calls
+ * no allocating routine.  SMX_ERR_INVALID when a grid plane needs more partial sums than the
+ * array's 32,832 entries (nx*ny/8 + 64 of them; every supported plane fits).  Every call on one
+ * device shares that device's array
+ * and leaves its contents behind: calls that overlap on one device (handles on different
+ * streams stepping at once) race. */
+int smx_step_noalloc(smx_handle *h, int64_t nsteps);
+
 /* L1 = sum|psi^{n+1}-psi^n| / (dt * sum|psi^{n+1}|) over the last step taken
  * by the most recent smx_step call, FP64 compensated.  Replaces the L1 block
  * pf3d.cpp:489-519 (Reduce x2 + Bcast).  SMX_ERR_STATE when L1 tracking is off or no

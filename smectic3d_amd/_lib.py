@@ -20,6 +20,7 @@ EXPORTS = [
     "smx_autotune", "smx_batch_autotune", "smx_set_adjoint_budget", "smx_adjoint_info",
     "smx_save_checkpoint", "smx_load_checkpoint", "smx_psi_avg", "smx_eckhaus_scan",
     "smx_set_l1_tracking", "smx_set_finite_check", "smx_batch_adjoint_grad", "smx_batch_adjoint_nodes",
+    "smx_step_noalloc",
 ]
 
 _lib = None
@@ -52,6 +53,7 @@ def lib():
     for n in ("smx_set_psi", "smx_get_psi", "smx_dtpsi"):
         getattr(L, n).argtypes = [vp, _dp, C.c_int64]
     L.smx_step.argtypes = [vp, C.c_int64]
+    L.smx_step_noalloc.argtypes = [vp, C.c_int64]
     for n in ("smx_l1", "smx_energy", "smx_dominant_wavenumber"):
         getattr(L, n).argtypes = [vp, _dp]
     L.smx_surftrack.argtypes = [vp, C.POINTER(SmxSurfOpts), _dp, _dp, _dp, _dp, C.POINTER(C.c_int64)]

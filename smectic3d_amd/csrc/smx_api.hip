@@ -133,6 +133,37 @@ static void adj_release(smx_handle* h) {
 // slack.  The handle holds two such regions: the step's and the autotuner's scratch.
 static int64_t partial_len(const smx_handle* h) { return h->nxy / 8 + 64; }
 
+// ---- device scratch of smx_step_noalloc ----
+// FP64 partial sums of the z passes (one per z tile) for the entry point that allocates
+// nothing: a file-scope device array, one copy per device, shared by every handle on that
+// device (so calls that overlap on one device race).  Sized for the largest supported
+// plane, 512 x 512 points at the narrowest z tile (8 columns).  Host code uses the address
+// hipGetSymbolAddress returns on each device, never the array's name.  The array has external
+// linkage: a file-static device array is a local symbol that the code object's dynamic symbol
+// table does not list, so the runtime cannot resolve its address.
+constexpr int64_t kScratchDoubles = 512 * 512 / 8 + 64;
+namespace smx {
+__device__ __align__(256) double scratch[kScratchDoubles];
+}
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch_ptr;  // device ordinal -> that device's scratch
+
+// Address of `device`'s copy of the array, looked up once per device on the first
+// smx_step_noalloc call there (the caller has made that device current).
+static int scratch_ptr(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SMX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = g_scratch_ptr.emplace(device, (double*)p).first;
+  }
+  *out = it->second;
+  return SMX_OK;
+}
+
 static int dalloc(smx_handle* h, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
@@ -452,6 +483,7 @@ template <typename T>
 static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
   if (h->cfg.advect) {
     AParams<T> ap = aparams<T>(h, h->fp32);
+    ap.p.partial = p.partial;  // the caller's partial-sum region (the handle's, or the device scratch)
     // FP64 partials of the advected z pass
     ap.p.npartial = h->neumann ? NeuAdvOps<T>::zpass_blocks(ap.p) : AdvectOps<T>::zpass_blocks(ap.p);
     return step_advect<T>(h, ap, ev);
@@ -681,14 +713,18 @@ static int reset_checks(smx_handle* h) {
   return SMX_OK;
 }
 
+// partial: FP64 partial-sum region of the z passes (nullptr: the handle's); snapshot: keep the
+// stage buffer before the last step for smx_l1.
 template <typename T>
-static int run_steps(smx_handle* h, int64_t nsteps) {
+static int run_steps(smx_handle* h, int64_t nsteps, double* partial = nullptr, bool snapshot = true) {
   PParams<T> p = params<T>(h, h->fp32);
+  if (partial && p.partial) p.partial = partial;
   const size_t sbytes = h->wbytes * (size_t)h->nspec;
-  if (h->l1_track && !h->Aprev) CHK(dalloc(h, &h->Aprev, sbytes));
+  snapshot = snapshot && h->l1_track;
+  if (snapshot && !h->Aprev) CHK(dalloc(h, &h->Aprev, sbytes));
   for (int64_t n = 0; n < nsteps; ++n) {
     // stage buffer of psi^n for the L1 criterion of the last step (smx_l1 transforms it)
-    if (n == nsteps - 1 && h->l1_track)
+    if (n == nsteps - 1 && snapshot)
       HIPCHK(hipMemcpyAsync(h->Aprev, h->A, sbytes, hipMemcpyDeviceToDevice, h->st));
     CHK(step_once<T>(h, p, nullptr));
     if (h->finite_every > 0 && (h->steps + n + 1) % h->finite_every == 0) CHK(enqueue_finite_check(h));
@@ -1382,6 +1418,26 @@ int smx_step(smx_handle* h, int64_t nsteps) {
   CHK(h->fp32 ? run_steps<float>(h, nsteps) : run_steps<double>(h, nsteps));
   h->steps += nsteps;
   h->has_prev = h->l1_track;
+  return SMX_OK;
+}
+
+int smx_step_noalloc(smx_handle* h, int64_t nsteps) {
+  if (!h) return fail(SMX_ERR_INVALID, "null handle");
+  if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  if (nsteps < 0) return fail(SMX_ERR_INVALID, "nsteps < 0");
+  if (partial_len(h) > kScratchDoubles)
+    return fail(SMX_ERR_INVALID, "grid plane " + std::to_string(h->cfg.nx) + " x " + std::to_string(h->cfg.ny) +
+                                     " needs " + std::to_string(partial_len(h)) + " partial sums; the device scratch holds " +
+                                     std::to_string(kScratchDoubles));
+  if (const unsigned int bad = nonfinite_seen(h))
+    return fail(SMX_ERR_NONFINITE, std::to_string(bad) + " non-finite values in the state at an earlier step check");
+  if (nsteps == 0) return SMX_OK;
+  HIPCHK(hipSetDevice(h->cfg.device));
+  double* sp = nullptr;
+  CHK(scratch_ptr(h->cfg.device, &sp));
+  CHK(h->fp32 ? run_steps<float>(h, nsteps, sp, false) : run_steps<double>(h, nsteps, sp, false));
+  h->steps += nsteps;
+  h->has_prev = false;  // no L1 snapshot: that would need a buffer
   return SMX_OK;
 }
 
