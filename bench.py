@@ -1,12 +1,14 @@
 """Benchmark of the periodic Swift-Hohenberg step on one MI355X.
 
-Workload (pf3dMorph-style periodic run): 256^3 grid, L = 32*2*pi per axis
-(8 points per wavelength), q0 = 1, eps = 0.3, beta = 0, gamma = 1, dt = 0.1,
-cubic nonlinearity, CNAB2; psi0 = 0.4 cos(q0 sqrt(max(x^2+y^2-z^2, 0) + 4))
-about the box centre plus uniform noise of amplitude 1e-3 (SplitMix64 seed 0).
-Both the FP64 path and the FP32-mixed path are run from the same initial
-state; `value` is the FP32-mixed throughput, the FP64 figures and the error of
-the mixed path against the FP64 path are reported beside it.
+Headline workload (pf3dMorph-style periodic run): 256^3 grid, L = 32*2*pi per axis
+(8 points per wavelength), q0 = 1, eps = 0.3, beta = 0, gamma = 1, dt = 0.1, cubic
+nonlinearity, CNAB2; psi0 = 0.4 cos(q0 sqrt(max(x^2+y^2-z^2, 0) + 4)) about the box centre
+plus uniform noise of amplitude 1e-3 (SplitMix64 seed 0).  `value` is the FP64 throughput
+(the reference's arithmetic); the FP32-mixed path runs beside it from the same state, and
+its error against the FP64 path after the workload's full 2000 steps is stated and checked
+against the 1e-6 / 1e-9 bounds.  Secondary legs: the uneven 24-box batch (forward graph and
+batched gradients), the Neumann box with surfTrack histograms, the advected boxes (with
+their FP32-vs-FP64 error) and the 256^3 adjoint.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -57,18 +59,35 @@ def hyperboloid_ic(n, dx, q0=1.0, a=0.4, noise=1e-3, seed=0):
     return psi + noise * (2.0 * u.reshape(n, n, n) - 1.0)
 
 
+SCHED_FILE = os.path.join(ROOT, "profiles", "periodic_256_schedules.json")
+
+
+def pinned_schedules():
+    """Tile schedules of the 256^3 step kernels recorded by an autotuned run and used by the
+    profiled runs under profiles/ (so the kernels timed here are the ones profiled)."""
+    try:
+        with open(SCHED_FILE) as f:
+            d = json.load(f)
+        return d["fp32"], d["fp64"]
+    except (OSError, ValueError, KeyError):
+        return None, None
+
+
 def run_path(precision, psi0, steps, warmup, prof_steps, tune=True, fixed=None):
+    """Timed region: `steps` steps in one smx_step call on the resident state, L1 tracking off
+    (no snapshot copy inside the region), bracketed by stream synchronisations."""
     from smectic3d_amd import Solver
     cfg = workload_cfg(precision)
     s = Solver(cfg)
     t0 = time.perf_counter()
-    if fixed is not None:  # schedules of an earlier run (profiled runs use the same kernels)
+    if fixed is not None:  # pinned schedules (the profiled instantiations)
         s.set_schedule([tuple(x) for x in fixed])
         sched = s.schedule()
     else:
         sched = s.autotune(20) if tune else s.schedule()
     tune_s = time.perf_counter() - t0
     s.set_psi(psi0)
+    s.set_l1_tracking(False)
     s.step(warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -77,11 +96,7 @@ def run_path(precision, psi0, steps, warmup, prof_steps, tune=True, fixed=None):
     t1 = time.perf_counter()
     out = {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps,
            "schedule": sched, "tune_s": tune_s}
-    out["psi"] = s.get_psi()
-    out["energy"] = s.energy()
-    out["qdom"] = s.dominant_wavenumber()
-    out["l1"] = s.l1()
-    out["kernels"] = s.profile_step(prof_steps)  # advances the state; after the final readouts
+    out["kernels"] = s.profile_step(prof_steps)
     if precision == 1:
         # plain float4 copy of one stage array: the achievable-bandwidth yardstick
         n_spec_bytes = cfg.nx * cfg.ny * (cfg.nz // 2 + 1) * 8
@@ -90,61 +105,157 @@ def run_path(precision, psi0, steps, warmup, prof_steps, tune=True, fixed=None):
     return out
 
 
-def run_neumann(steps, surf_every=20):
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def run_accuracy(psi0, nsteps=2000, f32=None, f64=None):
+    """FP32-mixed against FP64 over the workload's full 2000 steps (independent of --steps):
+    relative L2 of psi, relative energy error and relative error of the dominant wavenumber,
+    checked against the 1e-6 (psi) and 1e-9 (energy) bounds."""
+    from smectic3d_amd import Solver
+    from smectic3d_amd import _abi as A
+    res = {}
+    for prec, sch in ((A.SMX_FP64, f64), (A.SMX_FP32_MIXED, f32)):
+        with Solver(workload_cfg(prec)) as s:
+            if sch is not None:
+                s.set_schedule([tuple(x) for x in sch])
+            s.set_psi(psi0)
+            s.step(nsteps)
+            res[prec] = (s.get_psi(), s.energy(), s.dominant_wavenumber(), s.l1())
+    p64, e64, q64, l64 = res[A.SMX_FP64]
+    p32, e32, q32, l32 = res[A.SMX_FP32_MIXED]
+    out = {"steps": nsteps, "psi_rel_l2": rel(p32, p64), "energy_rel": abs(e32 - e64) / abs(e64),
+           "qdom_rel": abs(q32 - q64) / abs(q64), "energy_fp64": e64, "qdom_fp64": q64, "l1_fp64": l64,
+           "l1_fp32_mixed": l32}
+    out["meets_psi_1e-6"] = out["psi_rel_l2"] <= 1e-6
+    out["meets_energy_1e-9"] = out["energy_rel"] <= 1e-9
+    return out
+
+
+def neumann_cfg(precision):
+    from smectic3d_amd import _abi as A
+    nx, ny, nz = 256, 256, 384
+    dx = 2 * math.pi / 8
+    return A.make_config(nx, ny, nz, lx=nx * dx, ly=ny * dx, lz=nz * dx, bc=A.SMX_BC_NEUMANN, eps=0.5, q0=1.0,
+                         beta=0.0, gamma=1.0, dt=0.05, nl_model=A.SMX_NL_CUBIC, precision=precision)
+
+
+def histogram_compare(a, b, bins=64):
+    """Histograms of the FP32 values `a` and the FP64 values `b` on common bins: 64 equal bins
+    between the 0.5th and 99.5th percentiles of the FP64 values (values outside go to the end
+    bins).  Returns the total-variation distance of the two normalised histograms and the
+    relative differences of mean and standard deviation."""
+    lo, hi = np.percentile(b, [0.5, 99.5])
+    edges = np.linspace(lo, hi, bins + 1)
+    ha = np.histogram(np.clip(a, lo, hi), edges)[0] / max(1, len(a))
+    hb = np.histogram(np.clip(b, lo, hi), edges)[0] / max(1, len(b))
+    return {"bins": bins, "range_fp64_p0.5_p99.5": [float(lo), float(hi)], "samples": [int(len(a)), int(len(b))],
+            "total_variation": float(0.5 * np.abs(ha - hb).sum()),
+            "mean_rel": float(abs(a.mean() - b.mean()) / abs(b.mean())),
+            "std_rel": float(abs(a.std() - b.std()) / b.std())}
+
+
+def run_neumann(steps, surf_every=20, compare=True):
     """cosNoAdvMorph DCT/Neumann + surfTrack: 256x256x384, FP32 bulk with FP64 reductions,
-    eps=0.5, q0=1, gamma=1, dt=0.05, hyperboloid IC; surfTrack every 20 steps."""
+    eps=0.5, q0=1, gamma=1, dt=0.05, hyperboloid IC; surfTrack every 20 steps (the reference's
+    column scan from the top, arm 0.7, v = -dpsi/dt / |grad psi|).  The same run in FP64 gives
+    the reference histograms of v and H over every call's crossing columns."""
     from smectic3d_amd import Solver
     from smectic3d_amd import _abi as A
     nx, ny, nz = 256, 256, 384
     dx = 2 * math.pi / 8
-    cfg = A.make_config(nx, ny, nz, lx=nx * dx, ly=ny * dx, lz=nz * dx, bc=A.SMX_BC_NEUMANN, eps=0.5, q0=1.0,
-                        beta=0.0, gamma=1.0, dt=0.05, nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED)
     c = (np.arange(nx) - nx // 2) * dx
     cz = (np.arange(nz) - nz // 2) * dx
     r2 = np.maximum(c[:, None, None] ** 2 + c[None, :, None] ** 2 - cz[None, None, :] ** 2, 0.0)
     psi0 = 0.4 * np.cos(np.sqrt(r2 + 4.0))
-    with Solver(cfg) as s:
-        s.set_psi(psi0)
-        s.step(surf_every)
-        s.sync()
-        t0 = time.perf_counter()
-        found = 0
-        for _ in range(steps // surf_every):
-            s.step(surf_every)
-            _, f = s.surftrack(0.7, -1.0)
-            found += f
-        s.sync()
-        t1 = time.perf_counter()
-        ks = s.profile_step(20)
-    return {"steps": steps, "surftrack_every": surf_every, "steps_per_s_with_surftrack": steps / (t1 - t0),
-            "columns_found_per_call": found / max(1, steps // surf_every),
-            "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
+    out, samples = {}, {}
+    for prec in ((A.SMX_FP32_MIXED, A.SMX_FP64) if compare else (A.SMX_FP32_MIXED,)):
+        vs, hs = [], []
+        with Solver(neumann_cfg(prec)) as s:
+            s.set_psi(psi0)
+            s.set_l1_tracking(False)
+            s.sync()
+            t0 = time.perf_counter()
+            found = 0
+            for _ in range(steps // surf_every):
+                s.step(surf_every)
+                (sz, v, H, K), f = s.surftrack(0.7, -1.0)
+                found += f
+                if compare:  # every column reported on this call (the persistent arrays, as the reference's)
+                    vs.append(v.copy())
+                    hs.append(H.copy())
+            s.sync()
+            t1 = time.perf_counter()
+            psi_end = s.get_psi() if compare else None
+            if prec == A.SMX_FP32_MIXED:
+                ks = s.profile_step(20)
+                t_s0 = time.perf_counter()
+                for _ in range(20):
+                    s.surftrack(0.7, -1.0)
+                t_surf = (time.perf_counter() - t_s0) / 20
+        samples[prec] = (np.concatenate(vs) if vs else None, np.concatenate(hs) if hs else None, psi_end)
+        key = "fp32" if prec == A.SMX_FP32_MIXED else "fp64"
+        out[key] = {"steps_per_s_with_surftrack": steps / (t1 - t0),
+                    "columns_found_per_call": found / max(1, steps // surf_every)}
+    r = out["fp32"]
+    r.update({"steps": steps, "surftrack_every": surf_every, "surftrack_ms_per_call": 1e3 * t_surf,
+              "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]})
+    if compare:
+        v32, h32, p32 = samples[A.SMX_FP32_MIXED]
+        v64, h64, p64 = samples[A.SMX_FP64]
+        r["fp64"] = out["fp64"]
+        r["vs_fp64"] = {"psi_rel_l2_end": rel(p32, p64),
+                        "velocity_hist": histogram_compare(v32, v64), "meancurv_hist": histogram_compare(h32, h64),
+                        "note": "histograms over every column's value at each of the surfTrack calls "
+                                "(the persistent output arrays), FP32-mixed run against the FP64 run"}
+    return r
 
 
-def run_advected(steps):
-    """Advection-coupled periodic 256^3: mu grad psi force, spectral Stokes (eta=1), eps=0.5, q0=1,
-    gamma=1, dt=0.02, psi0 ~ U[-0.05, 0.05]; FP32-mixed."""
-    from smectic3d_amd import Solver
+def advected_cfg(precision):
     from smectic3d_amd import _abi as A
     n = 256
     L = n / 8 * 2 * math.pi
-    cfg = A.make_config(n, lx=L, ly=L, lz=L, eps=0.5, q0=1.0, beta=0.0, gamma=1.0, dt=0.02,
-                        nl_model=A.SMX_NL_CUBIC, advect=1, eta=1.0, rho=1.0, force_form=A.SMX_FORCE_MU,
-                        precision=A.SMX_FP32_MIXED)
-    rng = np.random.default_rng(4)
-    with Solver(cfg) as s:
+    return A.make_config(n, lx=L, ly=L, lz=L, eps=0.5, q0=1.0, beta=0.0, gamma=1.0, dt=0.02,
+                         nl_model=A.SMX_NL_CUBIC, advect=1, eta=1.0, rho=1.0, force_form=A.SMX_FORCE_MU,
+                         precision=precision)
+
+
+def run_advected(steps, compare=True):
+    """Advection-coupled periodic 256^3: mu grad psi force, spectral Stokes (eta=1), eps=0.5, q0=1,
+    gamma=1, dt=0.02, psi0 ~ U[-0.05, 0.05]; FP32-mixed, with the same run in FP64 for its error."""
+    from smectic3d_amd import Solver
+    from smectic3d_amd import _abi as A
+    n = 256
+    psi0 = np.random.default_rng(4).uniform(-0.05, 0.05, (n, n, n))
+    with Solver(advected_cfg(A.SMX_FP32_MIXED)) as s:
         s.autotune(10)
-        s.set_psi(rng.uniform(-0.05, 0.05, (n, n, n)))
-        s.step(5)
+        s.set_psi(psi0)
+        s.set_l1_tracking(False)
         s.sync()
         t0 = time.perf_counter()
         s.step(steps)
         s.sync()
         t1 = time.perf_counter()
         e = s.energy()
+        p32 = s.get_psi() if compare else None
         ks = s.profile_step(10)
-    return {"steps": steps, "steps_per_s": steps / (t1 - t0), "energy": e,
-            "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
+    out = {"steps": steps, "steps_per_s": steps / (t1 - t0), "energy": e,
+           "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]}
+    if compare:
+        with Solver(advected_cfg(A.SMX_FP64)) as s:
+            s.set_psi(psi0)
+            s.set_l1_tracking(False)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            t1 = time.perf_counter()
+            e64 = s.energy()
+            p64 = s.get_psi()
+        out["fp64_steps_per_s"] = steps / (t1 - t0)
+        out["vs_fp64"] = {"psi_rel_l2": rel(p32, p64), "energy_rel": abs(e - e64) / abs(e64), "energy_fp64": e64}
+    return out
 
 
 def run_neumann_advected(steps):
@@ -217,6 +328,51 @@ def run_batch(steps, tune=True):
             "schedules": {f"{c.nx}^3": sc for c, sc in zip(cfgs, scheds)}}
 
 
+def run_batch_adjoint(steps=500, ckpt=50, reps=3):
+    """Batched gradients of the 24 uneven boxes of batch_cfgs() (noise 0.05): loss
+    1/2 |psi_T - 0.5 cos(q0 z)|^2 accumulated in FP64, dJ/dpsi0, dJ/deps, dJ/dq0 by the
+    adjoint with checkpoints every 50 steps, every box's sweep a branch of one captured
+    hipGraph launched once per call.  Each box keeps its workspace (an explicit budget of
+    its full records), so repeated calls reuse the captured graph; the first call's time
+    includes capturing and instantiating it.  Compared with the same boxes' forward solve
+    as one graph batch (host to host: psi0 in, psi_T out)."""
+    from smectic3d_amd import Batch
+    cfgs = batch_cfgs()
+    psis = [np.random.default_rng(100 + i).uniform(-0.05, 0.05, (c.nx, c.ny, c.nz)) for i, c in enumerate(cfgs)]
+    with Batch(cfgs) as b:
+        b.autotune(10)
+        for s, c in zip(b.boxes, cfgs):
+            n = c.nx * c.ny * c.nz
+            sbytes = 8 * c.nx * c.ny * (c.nz // 2 + 1)
+            s.set_adjoint_budget(((steps + ckpt - 1) // ckpt) * (4 * n * ckpt + sbytes * (ckpt + 1)))
+        t0 = time.perf_counter()
+        J, d, de, dq = b.adjoint_grad(psis, steps, ckpt, 0.5)
+        tfirst = time.perf_counter() - t0
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            J2, d2, de2, dq2 = b.adjoint_grad(psis, steps, ckpt, 0.5)
+            ts.append(time.perf_counter() - t0)
+        same = all(np.array_equal(x, y) for x, y in zip(d, d2)) and de == de2 and dq == dq2
+        nodes = b.adjoint_nodes()
+        resident = [s.adjoint_info() for s in b.boxes]
+        # forward solve of the same boxes and length, one graph batch, host to host
+        t0 = time.perf_counter()
+        for s, p in zip(b.boxes, psis):
+            s.set_psi(p)
+        b.run(steps, graph_steps=50)
+        for s in b.boxes:
+            s.get_psi()
+        tfwd = time.perf_counter() - t0
+    tg = float(np.median(ts))
+    return {"boxes": len(cfgs), "sizes": sorted(c.nx for c in cfgs), "steps": steps, "ckpt_every": ckpt,
+            "grads_per_s": len(cfgs) / tg, "batch_gradient_s": tg, "first_call_s": tfirst,
+            "forward_batch_s": tfwd, "gradient_over_forward": tg / tfwd, "graph_nodes": nodes,
+            "repeat_bitwise_equal": bool(same), "resident_steps_per_box": sorted({r[0] for r in resident}),
+            "device_GB": sum(r[2] for r in resident) / 1e9,
+            "loss_sum": float(np.sum(J)), "dJ_deps": [float(x) for x in de], "dJ_dq0": [float(x) for x in dq]}
+
+
 def run_adjoint(steps, ckpt):
     """Gradient of 1/2|psi_T - 0.5 cos(q0 z)|^2 w.r.t. psi0, eps, q0 at 256^3 (checkpoint every 50),
     against the cost of a forward solve of the same length (FP32-mixed)."""
@@ -229,6 +385,7 @@ def run_adjoint(steps, ckpt):
     psi0 = np.random.default_rng(9).uniform(-0.05, 0.05, (n, n, n))
     with Solver(cfg) as s:
         s.autotune(10)
+        s.set_l1_tracking(False)
         s.set_psi(psi0)
         s.sync()
         t0 = time.perf_counter()
@@ -242,8 +399,11 @@ def run_adjoint(steps, ckpt):
         s.get_psi()
         tfio = time.perf_counter() - t0
         t0 = time.perf_counter()
-        s.adjoint_grad(psi0, steps, ckpt)  # first call: allocates and first-touches the record sets
+        s.adjoint_grad(psi0, steps, ckpt)  # default budget: the workspace is lent for this call only
         tcold = time.perf_counter() - t0
+        # the same budget, given explicitly, keeps the workspace in the handle between calls
+        s.set_adjoint_budget(s.adjoint_info()[2])
+        s.adjoint_grad(psi0, steps, ckpt)
         t0 = time.perf_counter()
         J, d, de, dq = s.adjoint_grad(psi0, steps, ckpt)
         tg = time.perf_counter() - t0
@@ -253,17 +413,18 @@ def run_adjoint(steps, ckpt):
             "gradient_over_forward": tg / tfio, "gradient_over_forward_steps_only": tg / tf,
             "first_call_s": tcold, "first_call_over_forward": tcold / tfio,
             "resident_steps": res, "recomputed_steps": rec, "device_GB": nbytes / 1e9,
-            "note": "gradient_s: a repeated call (record sets kept in the handle); first_call_s includes "
-                    "allocating and first-touching them; both include the host transfers of psi0 and "
-                    "dJ/dpsi0, as forward_with_io_s includes those of psi0 and psi_T (the forward solve "
-                    "host to host); forward_s times the steps alone",
+            "note": "gradient_s: a repeated call with an explicit budget (workspace kept in the handle); "
+                    "first_call_s: the default budget, which allocates the workspace and releases it on "
+                    "return; both include the host transfers of psi0 and dJ/dpsi0, as forward_with_io_s "
+                    "includes those of psi0 and psi_T (the forward solve host to host); forward_s times "
+                    "the steps alone",
             "loss": J, "dJ_deps": de, "dJ_dq0": dq}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "periodic_256_fp32_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "periodic_256_traffic.json")
 
 
-def measured_traffic(slot, schedule, n):
+def measured_traffic(slot, schedule, n, real="double"):
     """HBM bytes per launch of the kernel behind step slot `slot` under `schedule`, from the
     committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (each counter calibrated against the
     copy kernel of known size); None if that instantiation was not measured."""
@@ -276,34 +437,47 @@ def measured_traffic(slot, schedule, n):
     if slot not in slots:
         return None
     nc, v = schedule[slots.index(slot)]
-    inst = {"k_ypass_inv": f"smx::k_ypass<float, {n}, 1, {nc}, {v}>",
-            "k_zpass_nl": f"smx::k_zpass<float, {n}, {nc}, 0, 1, {v}>",
-            "k_ypass_fwd": f"smx::k_ypass<float, {n}, -1, {nc}, {v}>",
-            "k_xpass_step": f"smx::k_xpass<float, {n}, {nc}, 0, {v & 1}, {v >> 1}>"}[slot]
+    inst = {"k_ypass_inv": f"smx::k_ypass<{real}, {n}, 1, {nc}, {v}>",
+            "k_zpass_nl": f"smx::k_zpass<{real}, {n}, {nc}, 0, 1, {v}>",
+            "k_ypass_fwd": f"smx::k_ypass<{real}, {n}, -1, {nc}, {v}>",
+            "k_xpass_step": f"smx::k_xpass<{real}, {n}, {nc}, 0, {v & 1}, {v >> 1}>"}[slot]
     k = kernels.get(inst)
     if not k or "FETCH_SIZE_bytes" not in k or "WRITE_SIZE_bytes" not in k:
         return None
     return k["FETCH_SIZE_bytes"] + k["WRITE_SIZE_bytes"]
 
 
-def cpu_baseline(psi0, budget_s=12.0):
-    """The FP64 oracle (a C port of the reference loop, OpenMP threads) timed on the host,
-    on a bounded sample of the same workload: whole steps at 256^3 until ~budget_s."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(psi0, warmup=5, steps=20, runs=3):
+    """The FP64 oracle (a C port of the reference loop with the reference's dataflow: R2C of psi,
+    N, R2C of N, CNAB2, C2R every step; OpenMP threads, its own FFT, not FFTW; one process, no
+    MPI) timed on the host on the 256^3 workload: 5 warm-up steps, then 3 runs of 20 timed
+    steps, median."""
     from oracle import oracle as O
     from smectic3d_amd import _abi as A
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = O.Oracle(workload_cfg(A.SMX_FP64))
     o.set_psi(psi0)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        o.step(1)
-        n += 1
-        el = time.perf_counter() - t0
-        if n >= 2 and el > budget_s:
-            break
-    return {"value": n / el, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} FP64 steps of the 256^3 workload (oracle/smx_oracle.c, {threads} OpenMP threads, own FFT, not FFTW)"}
+    o.step(warmup)
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        o.step(steps)
+        rates.append(steps / (time.perf_counter() - t0))
+    return {"value": float(np.median(rates)), "unit": "steps/s", "cores": threads, "kind": "port",
+            "runs_steps_per_s": rates, "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "processes": 1,
+            "fft": "own mixed-radix FP64 (not FFTW)", "mpi": "none",
+            "sample": f"256^3 FP64 workload: {warmup} warm-up steps, then {runs} runs of {steps} timed steps, median "
+                      f"(oracle/smx_oracle.c, {threads} OpenMP threads, own FFT, not FFTW)"}
 
 
 def main():
@@ -313,14 +487,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extra", action="store_true", help="skip the Neumann/advected/batch/adjoint legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the accuracy and secondary legs")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the 256^3 step kernels instead of installing the pinned schedules")
     ap.add_argument("--no-tune", action="store_true", help="default tile schedules (no autotuning)")
-    ap.add_argument("--leg", choices=["neumann", "advected", "neumann_advected", "batch", "adjoint"],
+    ap.add_argument("--leg", choices=["neumann", "advected", "neumann_advected", "batch", "batch_adjoint",
+                                      "adjoint", "accuracy"],
                     help="run only this secondary workload (with --leg-steps steps) and print its result")
     ap.add_argument("--leg-steps", type=int, default=None)
     ap.add_argument("--schedules-from", default=None,
-                    help="install the tile schedules recorded in an earlier bench JSON line instead of "
-                         "autotuning (profiled runs then time the same kernels)")
+                    help="install the tile schedules recorded in a bench JSON line or a schedules file")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
@@ -328,81 +504,102 @@ def main():
         sys.exit(2)
     from smectic3d_amd import _abi as A
 
+    n = 256
+    dx = (n / 8 * 2 * math.pi) / n
+    f32 = f64 = None
+    sched_src = "autotuned"
+    if a.schedules_from:
+        prev = json.load(open(a.schedules_from))
+        if "fp32" in prev and "fp64" in prev:
+            f32, f64 = prev["fp32"], prev["fp64"]
+        else:
+            f32, f64 = prev["fp32_mixed"]["schedule"], prev["schedule_fp64"]
+        sched_src = "from " + a.schedules_from
+    elif not a.tune and not a.no_tune:
+        f32, f64 = pinned_schedules()
+        if f32 is not None:
+            sched_src = "pinned (profiles/periodic_256_schedules.json, the profiled instantiations)"
+    if a.no_tune:
+        sched_src = "default"
+
     if a.leg:  # one secondary workload alone (profiling runs)
         runs = {"neumann": (run_neumann, 4000), "advected": (run_advected, 1000),
                 "neumann_advected": (run_neumann_advected, 500),
-                "batch": (lambda k: run_batch(k, tune=not a.no_tune), 500), "adjoint": (lambda k: run_adjoint(k, 50), 2000)}
+                "batch": (lambda k: run_batch(k, tune=not a.no_tune), 500),
+                "batch_adjoint": (lambda k: run_batch_adjoint(k, 50), 500),
+                "adjoint": (lambda k: run_adjoint(k, 50), 2000),
+                "accuracy": (lambda k: run_accuracy(hyperboloid_ic(n, dx), k, f32, f64), 2000)}
         fn, default_steps = runs[a.leg]
         print(json.dumps({"leg": a.leg, "result": fn(a.leg_steps or default_steps)}, default=str))
         return
 
-    n = 256
-    dx = (n / 8 * 2 * math.pi) / n
     psi0 = hyperboloid_ic(n, dx)
-    log(f"workload 256^3, steps={a.steps} warmup={a.warmup}")
-    f32 = f64 = None
-    if a.schedules_from:
-        prev = json.load(open(a.schedules_from))
-        f32, f64 = prev["schedule_fp32"], prev["fp64"]["schedule"]
+    log(f"workload 256^3, steps={a.steps} warmup={a.warmup}, schedules {sched_src}")
+    r64 = run_path(A.SMX_FP64, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune, fixed=f64)
+    log(f"fp64: {r64['steps_per_s']:.1f} steps/s schedule {r64['schedule']}")
     r32 = run_path(A.SMX_FP32_MIXED, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune, fixed=f32)
     log(f"fp32-mixed: {r32['steps_per_s']:.1f} steps/s schedule {r32['schedule']}")
-    r64 = run_path(A.SMX_FP64, psi0, a.steps, a.warmup, a.prof_steps, tune=not a.no_tune, fixed=f64)
-    log(f"fp64: {r64['steps_per_s']:.1f} steps/s")
 
-    err_psi = float(np.linalg.norm(r32["psi"] - r64["psi"]) / np.linalg.norm(r64["psi"]))
-    err_e = abs(r32["energy"] - r64["energy"]) / abs(r64["energy"])
-    err_q = abs(r32["qdom"] - r64["qdom"]) / abs(r64["qdom"])
+    def kernel_table(ks):
+        return [{"name": k, "avg_ms": m, "alg_bytes": b, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]
 
-    # roofline of the dominant kernel (largest share of the step time) of the FP32-mixed path
-    ks = r32["kernels"]
-    name, ms, alg = max(ks, key=lambda k: k[1])
+    # roofline of the dominant kernel (largest share of the step time) of the FP64 path
+    name, ms, alg = max(r64["kernels"], key=lambda k: k[1])
     achieved = alg / (ms * 1e-3) / 1e9
     nzh = n // 2 + 1
-    s_r, s_c = n ** 3 * 4, n * n * nzh * 8
-    step_alg = 2 * s_r + 5 * s_c  # compulsory bytes of the pure-SH step DAG (FP32)
+    step_alg64 = 2 * n ** 3 * 8 + 5 * n * n * nzh * 16  # compulsory bytes of the pure-SH step DAG
+    step_alg32 = step_alg64 // 2
+    ksum64 = sum(k[1] for k in r64["kernels"])
+    ksum32 = sum(k[1] for k in r32["kernels"])
 
     line = {
         "metric": METRIC,
-        "value": r32["steps_per_s"],
+        "value": r64["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": r32["ms_per_step"],
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": "periodic SH 256^3, hyperboloid IC + 1e-3 noise, L=32*2pi, q0=1, eps=0.3, beta=0, "
-                        "gamma=1, dt=0.1, cubic NL, CNAB2 (FP32-mixed headline; FP64 beside it)",
-            "grid": [n, n, n], "precision": "fp32-mixed",
+                        "gamma=1, dt=0.1, cubic NL, CNAB2; FP64 (value) with the FP32-mixed path beside it",
+            "grid": [n, n, n], "precision": "fp64",
         },
-        "fp64": {"value": r64["steps_per_s"], "unit": "steps/s", "ms_per_step": r64["ms_per_step"],
-                 "schedule": r64["schedule"]},
-        "schedule_fp32": r32["schedule"],
-        "schedule_note": "(columns per tile, plan variant) of the y-inverse, z, y-forward and x kernels, "
-                         "chosen by smx_autotune before the timed region",
-        "autotune_s": {"fp32": r32["tune_s"], "fp64": r64["tune_s"]},
+        "schedule_fp64": r64["schedule"],
+        "schedule_source": sched_src,
+        "schedule_note": "(columns per tile, plan variant) of the y-inverse, z, y-forward and x kernels",
+        "kernel_sum_ms_fp64": ksum64,
+        "ms_per_step_over_kernel_sum_fp64": r64["ms_per_step"] / ksum64,
+        "kernels_fp64": kernel_table(r64["kernels"]),
+        "step_roofline_fp64": {"alg_bytes_per_step": step_alg64,
+                               "achieved_GBs": step_alg64 / (r64["ms_per_step"] * 1e-3) / 1e9,
+                               "frac": step_alg64 / (r64["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "fp32_mixed": {"value": r32["steps_per_s"], "unit": "steps/s", "ms_per_step": r32["ms_per_step"],
+                       "schedule": r32["schedule"], "kernel_sum_ms": ksum32,
+                       "speedup_vs_fp64": r32["steps_per_s"] / r64["steps_per_s"],
+                       "kernels": kernel_table(r32["kernels"]),
+                       "step_roofline": {"alg_bytes_per_step": step_alg32,
+                                         "achieved_GBs": step_alg32 / (r32["ms_per_step"] * 1e-3) / 1e9,
+                                         "frac": step_alg32 / (r32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS}},
         "copy_kernel_GBs": r32["copy_GBs"],
-        "speedup_fp32_mixed_vs_fp64": r32["steps_per_s"] / r64["steps_per_s"],
-        "error_fp32_mixed_vs_fp64": {"psi_rel_l2": err_psi, "energy_rel": err_e, "qdom_rel": err_q,
-                                     "steps": a.steps + a.warmup},
-        "step_roofline_fp32": {"alg_bytes_per_step": step_alg,
-                               "achieved_GBs": step_alg / (r32["ms_per_step"] * 1e-3) / 1e9,
-                               "frac": step_alg / (r32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "kernels_fp32": [{"name": k, "avg_ms": m, "alg_bytes": b, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks],
-        "kernels_fp64": [{"name": k, "avg_ms": m, "alg_bytes": b, "GBs": b / (m * 1e-3) / 1e9}
-                         for k, m, b in r64["kernels"]],
+        "autotune_s": {"fp32": r32["tune_s"], "fp64": r64["tune_s"]},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(name, r32["schedule"], n)},
+                     "traffic": measured_traffic(name, r64["schedule"], n, "double")},
         "cpu_baseline": None,
     }
     if not a.no_extra:
+        log("accuracy: FP32-mixed vs FP64 over 2000 steps ...")
+        line["fp32_mixed"]["error_vs_fp64"] = run_accuracy(psi0, 2000, r32["schedule"], r64["schedule"])
         log("batch ...")
         line["batch_24_boxes_fp32"] = run_batch(500, tune=not a.no_tune)
+        log("batched gradients ...")
+        line["batch_gradients_24_boxes_fp32"] = run_batch_adjoint(500, 50)
         log("neumann + surftrack ...")
         line["neumann_surftrack_256x256x384_fp32"] = run_neumann(4000)
         log("advected ...")
