@@ -516,17 +516,19 @@ static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
 }
 
 // ---- FP64 diagnostics on a copy of the state ----
-static int stage_to_d1(smx_handle* h, const void* stage) {
+// FP64 copy of a stage buffer of the working precision into dst.
+static int stage_to_f64(smx_handle* h, const void* stage, void* dst) {
   const int64_t ncomp = h->neumann ? h->nspec : 2 * h->nspec;  // real components
   if (h->fp32) {
     hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)stage,
-                       (double*)h->D1, ncomp);
+                       (double*)dst, ncomp);
     HIPCHK(hipGetLastError());
   } else {
-    HIPCHK(hipMemcpyAsync(h->D1, stage, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(dst, stage, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
   }
   return SMX_OK;
 }
+static int stage_to_d1(smx_handle* h, const void* stage) { return stage_to_f64(h, stage, h->D1); }
 static int state_to_d1(smx_handle* h) { return stage_to_d1(h, h->A); }
 
 // Real psi of a stage buffer (the x-inverse of psi_hat) into `out` (dense [x][y][z], FP64).
@@ -559,10 +561,10 @@ static int spectrum_state(smx_handle* h) {
   return SMX_OK;
 }
 
-// Normalised N_hat of the real field in h->real_io into D1.
-static int nl_spectrum_of_real_io(smx_handle* h) {
+// Normalised N_hat of a real field into D1.
+static int nl_spectrum_of(smx_handle* h, const double* real) {
   PParams<double> p = params<double>(h, false);
-  CHK(zpass<double>(h, p, Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, h->D1, h->real_io, nullptr));
+  CHK(zpass<double>(h, p, Z_IN_REAL, Z_OUT_SPEC, Z_OP_NL, nullptr, h->D1, real, nullptr));
   CHK(ypass<double>(h, h->D1, h->D1, -1));
   CHK(xpass<double>(h, p, X_FWD_SCALED, h->D1, h->D1, nullptr, nullptr));
   return SMX_OK;
@@ -588,12 +590,12 @@ static KParams kparams(smx_handle* h) {
 
 static int kmul(smx_handle* h, int mode, const void* in, void* out) {
   KParams k = kparams(h);
+  const unsigned rows = (unsigned)std::min<int64_t>((int64_t)h->nplanes * h->cfg.nx, 16384);  // grid-stride rows
   if (h->neumann)
-    hipLaunchKernelGGL(k_kmul_real, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const double*)in, (double*)out, k,
-                       mode);
+    hipLaunchKernelGGL(k_kmul_real, dim3(rows), dim3(256), 0, h->st, (const double*)in, (double*)out, k, mode);
   else
-    hipLaunchKernelGGL(k_kmul_cplx, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const cplx<double>*)in,
-                       (cplx<double>*)out, k, mode);
+    hipLaunchKernelGGL(k_kmul_cplx, dim3(rows), dim3(256), 0, h->st, (const cplx<double>*)in, (cplx<double>*)out, k,
+                       mode);
   HIPCHK(hipGetLastError());
   return SMX_OK;
 }
@@ -677,7 +679,7 @@ static int dtpsi_field(smx_handle* h, double* out) {
     return c2r_spectrum_d1(h, out);
   }
   CHK(c2r_state(h, h->real_io));
-  CHK(nl_spectrum_of_real_io(h));        // D1 = N_hat
+  CHK(nl_spectrum_of(h, h->real_io));    // D1 = N_hat
   CHK(kmul(h, KM_DTPSI, h->D2, h->D1));  // D1 = aLin psi_hat + N_hat
   return c2r_spectrum_d1(h, out);
 }
@@ -1662,30 +1664,71 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
                   int64_t* found) {
   if (!h || !opts || !surfz || !vel || !H || !K) return fail(SMX_ERR_INVALID, "null argument");
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
+  if (!(opts->arm >= 0.0)) return fail(SMX_ERR_INVALID, "arm must be >= 0 (the scan arms above it, then crosses 0)");
   HIPCHK(hipSetDevice(h->cfg.device));
   const int64_t N = h->N;
-  if (!h->sfield) CHK(dalloc(h, (void**)&h->sfield, sizeof(double) * 8 * N));
+  const size_t sdoubles = (h->neumann ? 1 : 2) * (size_t)h->nspec;  // one FP64 stage array
+  // real fields f[0..7], two FP64 stage arrays (W0, W1) and the crossing index of each column
+  if (!h->sfield)
+    CHK(dalloc(h, (void**)&h->sfield, sizeof(double) * (8 * (size_t)N + 2 * sdoubles) + sizeof(int) * (size_t)h->nxy));
   double* f = h->sfield;
-  // f[0] psi, f[1] dtpsi, then per boundary condition:
-  //  periodic: f[2..4] spectral gradient, f[5] psi_xx + psi_yy, f[6] psi_xx, f[7] psi_yy, psi_xy in real_io
-  //            (surfTrack.cpp:611-626)
-  //  Neumann:  f[2] psi_xx, f[3] psi_yy, f[4] psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750)
-  CHK(dtpsi_field(h, f + N));
-  CHK(c2r_state(h, f));
-  CHK(spectrum_state(h));
-  if (!h->neumann) {
-    const int modes[] = {KM_IKX, KM_IKY, KM_IKZ, KM_LAP_XY, KM_DXX, KM_DYY, KM_DXY};
-    double* dst[] = {f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N, f + 6 * N, f + 7 * N, h->real_io};
-    for (int i = 0; i < 7; ++i) {
-      CHK(kmul(h, modes[i], h->D2, h->D1));
-      CHK(c2r_spectrum_d1(h, dst[i]));
-    }
+  double* W0 = f + 8 * N;       // x stage of psi_hat: [kz][x][ky]
+  double* W1 = W0 + sdoubles;   // y inverse of it: [kz][x][y]
+  int* kcol = reinterpret_cast<int*>(W1 + sdoubles);
+  PParams<double> p = params<double>(h, false);
+  // W0 from the state (the FP64 state is its own x stage), psi_hat into D2, psi into f[0]
+  const void* x_stage = h->A;
+  if (h->fp32) {
+    CHK(stage_to_f64(h, h->A, W0));
+    x_stage = W0;
+  }
+  {
+    PParams<double> px = p;
+    px.scale = h->neumann ? 0.5 / (double)h->cfg.nx : 1.0 / (double)h->cfg.nx;  // undo the x inverse
+    CHK(xpass<double>(h, px, X_FWD_SCALED, x_stage, h->D2, nullptr, nullptr));
+  }
+  CHK(ypass<double>(h, x_stage, W1, +1));
+  CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, W1, nullptr, nullptr, f));
+  // dpsi/dt into f[1]: N_hat of psi (with the advective term when advection is on), then
+  // F^-1[aLin psi_hat + N_hat] (cosNoAdvMorphCurv.cpp:744,764; surfTrack.cpp:585-611)
+  if (h->cfg.advect) {
+    CHK(advected_n_spectrum(h));
   } else {
-    const int modes[] = {KM_DXX, KM_DYY, KM_DZZ};
-    for (int i = 0; i < 3; ++i) {
-      CHK(kmul(h, modes[i], h->D2, h->D1));
-      CHK(c2r_spectrum_d1(h, f + (2 + i) * N));
-    }
+    CHK(nl_spectrum_of(h, f));
+  }
+  CHK(kmul(h, KM_DTPSI, h->D2, h->D1));
+  CHK(c2r_spectrum_d1(h, f + N));
+  // Derivatives.  A multiplier that depends on one wavenumber commutes with the transforms
+  // of the other axes, so it is applied where those are already done: on W1 (x and y
+  // inverted) for kz, on W0 (x inverted) for ky, on the full spectrum D2 for kx.
+  auto along_z = [&](int mode, double* out) -> int {
+    CHK(kmul(h, mode, W1, h->D1));
+    return zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out);
+  };
+  auto along_y = [&](int mode, double* out) -> int {
+    CHK(kmul(h, mode, x_stage, h->D1));
+    CHK(ypass<double>(h, h->D1, h->D1, +1));
+    return zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out);
+  };
+  auto along_x = [&](int mode, double* out) -> int {
+    CHK(kmul(h, mode, h->D2, h->D1));
+    return c2r_spectrum_d1(h, out);
+  };
+  if (!h->neumann) {
+    // f[2..4] spectral gradient, f[5] psi_xx + psi_yy, f[6] psi_xx, f[7] psi_yy, psi_xy in
+    // real_io (surfTrack.cpp:611-626)
+    CHK(along_x(KM_IKX, f + 2 * N));
+    CHK(along_y(KM_IKY, f + 3 * N));
+    CHK(along_z(KM_IKZ, f + 4 * N));
+    CHK(along_x(KM_LAP_XY, f + 5 * N));
+    CHK(along_x(KM_DXX, f + 6 * N));
+    CHK(along_y(KM_DYY, f + 7 * N));
+    CHK(along_x(KM_DXY, h->real_io));
+  } else {
+    // f[2] psi_xx, f[3] psi_yy, f[4] psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750)
+    CHK(along_x(KM_DXX, f + 2 * N));
+    CHK(along_y(KM_DYY, f + 3 * N));
+    CHK(along_z(KM_DZZ, f + 4 * N));
   }
   HIPCHK(hipMemsetAsync(h->flag, 0, sizeof(unsigned int), h->st));
   SurfParams sp;
@@ -1698,13 +1741,19 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   sp.tdy = 2 * h->dy;
   sp.tdz = 2 * h->dz;
   const int64_t nxy = h->nxy;
-  const unsigned blocks = (unsigned)((nxy + 255) / 256);
+  // the crossing of every column (one wave per column), then each column's outputs (one thread)
+  const unsigned sblocks = (unsigned)((nxy + 3) / 4), blocks = (unsigned)((nxy + 255) / 256);
+  if (h->neumann)
+    hipLaunchKernelGGL(k_surf_scan<false>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol);
+  else
+    hipLaunchKernelGGL(k_surf_scan<true>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol);
+  HIPCHK(hipGetLastError());
   if (h->neumann)
     hipLaunchKernelGGL(k_surf_neumann, dim3(blocks), dim3(256), 0, h->st, f, f + N, f + 2 * N, f + 3 * N, f + 4 * N,
-                       sp, h->surf, h->flag);
+                       kcol, sp, h->surf, h->flag);
   else
     hipLaunchKernelGGL(k_surf_periodic, dim3(blocks), dim3(256), 0, h->st, f, f + N, f + 2 * N, f + 3 * N, f + 4 * N,
-                       f + 5 * N, f + 6 * N, f + 7 * N, h->real_io, sp, h->surf, h->flag);
+                       f + 5 * N, f + 6 * N, f + 7 * N, h->real_io, kcol, sp, h->surf, h->flag);
   HIPCHK(hipGetLastError());
   double* outs[4] = {surfz, vel, H, K};
   for (int i = 0; i < 4; ++i)

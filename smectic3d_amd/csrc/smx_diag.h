@@ -39,12 +39,15 @@ enum KMode {
   KM_LAP_XY = 8,  // out = -(kx^2 + ky^2) in       (surfTrack.cpp:612-613)
 };
 
+// (kz, x, ky) of entry i of a [kz][x][ky] array.  32-bit arithmetic: every supported grid has
+// fewer than 2^31 spectral entries (512^3 Neumann: 1.3e8), and 64-bit division costs as much
+// issue time as these grid-stride kernels spend on memory.
 __device__ __forceinline__ void kidx(const KParams& k, int64_t i, int& iz, int& ix, int& iy) {
-  const int64_t nxy = (int64_t)k.nx * k.ny;
-  iz = (int)(i / nxy);
-  const int64_t r = i - (int64_t)iz * nxy;
-  ix = (int)(r / k.ny);
-  iy = (int)(r - (int64_t)ix * k.ny);
+  const uint32_t u = (uint32_t)i, nxy = (uint32_t)k.nx * (uint32_t)k.ny;
+  const uint32_t z = u / nxy, r = u - z * nxy, x = r / (uint32_t)k.ny;
+  iz = (int)z;
+  ix = (int)x;
+  iy = (int)(r - x * (uint32_t)k.ny);
 }
 
 __device__ __forceinline__ double alin(const KParams& k, double k2) {
@@ -52,47 +55,62 @@ __device__ __forceinline__ double alin(const KParams& k, double k2) {
   return k.eps - k.alpha * d * d;
 }
 
-__global__ void k_kmul_cplx(const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out, KParams k, int mode) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k.n; i += (int64_t)gridDim.x * blockDim.x) {
-    int iz, ix, iy;
-    kidx(k, i, iz, ix, iy);
-    const double qx = k.kx[ix], qy = k.ky[iy], qz = k.kz[iz];
-    const cplx<double> a = in[i];
-    cplx<double> r;
-    switch (mode) {
-      case KM_DTPSI: {
-        const double m = alin(k, qx * qx + qy * qy + qz * qz);
-        r = mk(m * a.x + out[i].x, m * a.y + out[i].y);
-        break;
+// The k-space multipliers walk the [kz][x][ky] layout row by row: one row (kz, x) per block
+// iteration, so qx and qz are uniform over the block and only qy is read per entry.
+__global__ __launch_bounds__(256) void k_kmul_cplx(const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
+                                                   KParams k, int mode) {
+  const int nrows = k.nplanes * k.nx;
+  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int iz = row / k.nx, ix = row - iz * k.nx;
+    const double qx = k.kx[ix], qz = k.kz[iz];
+    const int64_t base = (int64_t)row * k.ny;
+    for (int iy = threadIdx.x; iy < k.ny; iy += blockDim.x) {
+      const int64_t i = base + iy;
+      const double qy = k.ky[iy];
+      const cplx<double> a = in[i];
+      cplx<double> r;
+      switch (mode) {
+        case KM_DTPSI: {
+          const double m = alin(k, qx * qx + qy * qy + qz * qz);
+          const cplx<double> o = out[i];
+          r = mk(m * a.x + o.x, m * a.y + o.y);
+          break;
+        }
+        case KM_IKX: r = mk(-qx * a.y, qx * a.x); break;
+        case KM_IKY: r = mk(-qy * a.y, qy * a.x); break;
+        case KM_IKZ: r = mk(-qz * a.y, qz * a.x); break;
+        case KM_DXX: r = (-qx * qx) * a; break;
+        case KM_DYY: r = (-qy * qy) * a; break;
+        case KM_DZZ: r = (-qz * qz) * a; break;
+        case KM_DXY: r = (-qx * qy) * a; break;
+        default: r = (-(qx * qx) - qy * qy) * a; break;
       }
-      case KM_IKX: r = mk(-qx * a.y, qx * a.x); break;
-      case KM_IKY: r = mk(-qy * a.y, qy * a.x); break;
-      case KM_IKZ: r = mk(-qz * a.y, qz * a.x); break;
-      case KM_DXX: r = (-qx * qx) * a; break;
-      case KM_DYY: r = (-qy * qy) * a; break;
-      case KM_DZZ: r = (-qz * qz) * a; break;
-      case KM_DXY: r = (-qx * qy) * a; break;
-      default: r = (-(qx * qx) - qy * qy) * a; break;
+      out[i] = r;
     }
-    out[i] = r;
   }
 }
 
-__global__ void k_kmul_real(const double* __restrict__ in, double* __restrict__ out, KParams k, int mode) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k.n; i += (int64_t)gridDim.x * blockDim.x) {
-    int iz, ix, iy;
-    kidx(k, i, iz, ix, iy);
-    const double qx = k.kx[ix], qy = k.ky[iy], qz = k.kz[iz];
-    const double a = in[i];
-    double r;
-    switch (mode) {
-      case KM_DTPSI: r = alin(k, qx * qx + qy * qy + qz * qz) * a + out[i]; break;
-      case KM_DXX: r = -qx * qx * a; break;
-      case KM_DYY: r = -qy * qy * a; break;
-      case KM_DZZ: r = -qz * qz * a; break;
-      default: r = (-(qx * qx) - qy * qy) * a; break;
+__global__ __launch_bounds__(256) void k_kmul_real(const double* __restrict__ in, double* __restrict__ out, KParams k,
+                                                   int mode) {
+  const int nrows = k.nplanes * k.nx;
+  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int iz = row / k.nx, ix = row - iz * k.nx;
+    const double qx = k.kx[ix], qz = k.kz[iz];
+    const int64_t base = (int64_t)row * k.ny;
+    for (int iy = threadIdx.x; iy < k.ny; iy += blockDim.x) {
+      const int64_t i = base + iy;
+      const double qy = k.ky[iy];
+      const double a = in[i];
+      double r;
+      switch (mode) {
+        case KM_DTPSI: r = alin(k, qx * qx + qy * qy + qz * qz) * a + out[i]; break;
+        case KM_DXX: r = -qx * qx * a; break;
+        case KM_DYY: r = -qy * qy * a; break;
+        case KM_DZZ: r = -qz * qz * a; break;
+        default: r = (-(qx * qx) - qy * qy) * a; break;
+      }
+      out[i] = r;
     }
-    out[i] = r;
   }
 }
 
@@ -366,34 +384,74 @@ struct SurfParams {
   double tdx, tdy, tdz;  // 2 dx, 2 dy, 2 dz
 };
 
-// surfTrack.cpp:767-806: scan k upward, arm at psi > arm, first psi < 0 after
-// arming; derivatives taken at k-1 when |psi(k)| > |psi(k-1)|; reports k.
-__global__ void k_surf_periodic(const double* __restrict__ psi, const double* __restrict__ dt,
+// Lanes from `a` on (0 <= a < 64): a crossing may be taken at the arming index itself, as in the
+// reference's loop (possible only for a negative threshold, which the host refuses).
+__device__ __forceinline__ uint64_t lanes_from(int a) { return ~((1ull << a) - 1ull); }
+
+// Column scan of the reference's sequential loop (arm at psi > arm, then the first psi < 0):
+// one wave per column, every 64-value chunk of the column (nz <= 512: at most 8) loaded
+// before the tests, which take the chunks in scan order with ballots (lane order = scan
+// order).  The crossing index (-1: none) goes to kc_out[column]; the output kernels below
+// then evaluate each column in its own lane.  blockDim = 256: 4 columns per block.
+template <bool UP>
+__global__ __launch_bounds__(256) void k_surf_scan(const double* __restrict__ psi, int64_t nxy, int nz, double arm,
+                                                   int* __restrict__ kc_out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // uniform across the wave
+  if (c >= nxy) return;
+  const double* colp = psi + c * nz;
+  const int nch = (nz + 63) >> 6;
+  double v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = UP ? q * 64 + lane : nz - 1 - q * 64 - lane;
+    v[q] = (q < nch && k >= 0 && k < nz) ? colp[k] : 0.0;
+  }
+  bool armed = false;
+  int kc = -1;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = UP ? q * 64 + lane : nz - 1 - q * 64 - lane;
+    const bool in = q < nch && k >= 0 && k < nz;
+    const uint64_t am = __ballot(in && v[q] > arm);
+    uint64_t neg = __ballot(in && v[q] < 0.0);
+    if (kc < 0) {
+      if (!armed && am) {
+        neg &= lanes_from(__ffsll((unsigned long long)am) - 1);
+        armed = true;
+      }
+      if (armed && neg) {
+        const int l = __ffsll((unsigned long long)neg) - 1;
+        kc = UP ? q * 64 + l : nz - 1 - q * 64 - l;
+      }
+    }
+  }
+  if (lane == 0) kc_out[c] = kc;
+}
+
+// surfTrack.cpp:767-806 at the crossing k of k_surf_scan<true> (scan k upward, arm at
+// psi > arm, first psi < 0 after arming): derivatives taken at k-1 when |psi(k)| > |psi(k-1)|;
+// reports k.  One thread per column.
+__global__ __launch_bounds__(256) void k_surf_periodic(const double* __restrict__ psi, const double* __restrict__ dt,
                                 const double* __restrict__ gx, const double* __restrict__ gy,
                                 const double* __restrict__ gz, const double* __restrict__ hnum,
                                 const double* __restrict__ dxx, const double* __restrict__ dyy,
-                                const double* __restrict__ dxy, SurfParams sp, double* __restrict__ out,
-                                unsigned int* __restrict__ nfound) {
+                                const double* __restrict__ dxy, const int* __restrict__ kc_in, SurfParams sp,
+                                double* __restrict__ out, unsigned int* __restrict__ nfound) {
   const int64_t nxy = (int64_t)sp.nx * sp.ny;
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
+  const int kc = kc_in[c];
+  if (kc < 0) return;
   const int64_t col = c * sp.nz;
-  int track = 0;
-  for (int k = 0; k < sp.nz; ++k) {
-    const double v = psi[col + k];
-    if (v > sp.arm && track == 0) track = 1;
-    if (v < 0.0 && track == 1) {
-      int64_t id = col + k;
-      if (fabs(v) > fabs(psi[id - 1])) id -= 1;
-      const double g = sqrt(gx[id] * gx[id] + gy[id] * gy[id] + gz[id] * gz[id]);
-      out[c] = (double)k;
-      out[nxy + c] = sp.vsign * dt[id] / g;
-      out[2 * nxy + c] = hnum[id] / g;
-      out[3 * nxy + c] = (dxx[id] * dyy[id] - dxy[id] * dxy[id]) / (g * g);
-      atomicAdd(nfound, 1u);
-      break;
-    }
-  }
+  int64_t id = col + kc;
+  if (kc > 0 && fabs(psi[id]) > fabs(psi[id - 1])) id -= 1;
+  const double g = sqrt(gx[id] * gx[id] + gy[id] * gy[id] + gz[id] * gz[id]);
+  out[c] = (double)kc;
+  out[nxy + c] = sp.vsign * dt[id] / g;
+  out[2 * nxy + c] = hnum[id] / g;
+  out[3 * nxy + c] = (dxx[id] * dyy[id] - dxy[id] * dxy[id]) / (g * g);
+  atomicAdd(nfound, 1u);
 }
 
 // Central differences with one-sided (x2) stencils at the walls,
@@ -417,62 +475,58 @@ __device__ __forceinline__ double fd_z(const double* p, const SurfParams& s, int
   return 2.0 * (P(k) - P(k - 1)) / s.tdz;
 }
 
-// cosNoAdvMorphCurv.cpp:877-966: scan k downward, arm at psi > arm, first psi < 0;
-// index moved to k+1 when |psi(k)| > |psi(k+1)|; Megrabov H = div(grad psi/|grad psi|)
-// and Gaussian curvature K from the FD gradient and the spectral second derivatives.
-__global__ void k_surf_neumann(const double* __restrict__ psi, const double* __restrict__ dt,
+// cosNoAdvMorphCurv.cpp:877-966 at the crossing k of k_surf_scan<false> (scan k downward,
+// arm at psi > arm, first psi < 0): index moved to k+1 when |psi(k)| > |psi(k+1)|; Megrabov
+// H = div(grad psi/|grad psi|) and Gaussian curvature K from the FD gradient and the spectral
+// second derivatives.  One thread per column.
+__global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__ psi, const double* __restrict__ dt,
                                const double* __restrict__ dxx, const double* __restrict__ dyy,
-                               const double* __restrict__ dzz, SurfParams sp, double* __restrict__ out,
-                               unsigned int* __restrict__ nfound) {
+                               const double* __restrict__ dzz, const int* __restrict__ kc_in, SurfParams sp,
+                               double* __restrict__ out, unsigned int* __restrict__ nfound) {
   const int64_t nxy = (int64_t)sp.nx * sp.ny;
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
+  const int kc = kc_in[c];
+  if (kc < 0) return;
   const int i = (int)(c / sp.ny), j = (int)(c - (int64_t)i * sp.ny);
   const int64_t col = c * sp.nz;
-  int track = 0;
-  for (int k = sp.nz - 1; k > -1; --k) {
-    const double v = psi[col + k];
-    if (v > sp.arm && track == 0) track = 1;
-    if (v < 0.0 && track == 1) {
-      int k2 = k;
-      if (fabs(v) > fabs(psi[col + k + 1])) k2 = k + 1;
-      const int64_t id = col + k2;
-      const double px = fd_x(psi, sp, i, j, k2), py = fd_y(psi, sp, i, j, k2), pz = fd_z(psi, sp, i, j, k2);
-      const double g = sqrt(px * px + py * py + pz * pz);
-      double d_xy;
-      if (j > 0 && j < sp.ny - 1) d_xy = (fd_x(psi, sp, i, j + 1, k2) - fd_x(psi, sp, i, j - 1, k2)) / sp.tdy;
-      else if (j == 0) d_xy = 2.0 * (fd_x(psi, sp, i, 1, k2) - fd_x(psi, sp, i, 0, k2)) / sp.tdy;
-      else d_xy = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j - 1, k2)) / sp.tdy;
-      // the reference picks the z stencil by k while centring it on k2 (:926-940); a
-      // centred stencil that would leave the column falls back to one-sided
-      double d_xz, d_yz;
-      if (k > 0 && k < sp.nz - 1 && k2 - 1 >= 0 && k2 + 1 <= sp.nz - 1) {
-        d_xz = (fd_x(psi, sp, i, j, k2 + 1) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
-        d_yz = (fd_y(psi, sp, i, j, k2 + 1) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
-      } else if (k == 0 || k2 - 1 < 0) {
-        d_xz = 2.0 * (fd_x(psi, sp, i, j, k2 + 1) - fd_x(psi, sp, i, j, k2)) / sp.tdz;
-        d_yz = 2.0 * (fd_y(psi, sp, i, j, k2 + 1) - fd_y(psi, sp, i, j, k2)) / sp.tdz;
-      } else {
-        d_xz = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
-        d_yz = 2.0 * (fd_y(psi, sp, i, j, k2) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
-      }
-      const double fxx = dxx[id], fyy = dyy[id], fzz = dzz[id];
-      const double Hc = ((py * py + pz * pz) * fxx + (px * px + pz * pz) * fyy + (px * px + py * py) * fzz -
-                         2.0 * (px * py * d_xy + px * pz * d_xz + py * pz * d_yz)) /
-                        (g * g * g);
-      const double Kc = (pz * pz * (fxx * fyy - d_xy * d_xy) + px * px * (fyy * fzz - d_yz * d_yz) +
-                         py * py * (fxx * fzz - d_xz * d_xz) +
-                         2.0 * (py * d_xy * (pz * d_xz - px * fzz) + px * d_xz * (py * d_yz - pz * fyy) +
-                                pz * d_yz * (px * d_xy - py * fxx))) /
-                        (g * g * g * g);
-      out[c] = (double)k2;
-      out[nxy + c] = sp.vsign * dt[id] / g;
-      out[2 * nxy + c] = Hc;
-      out[3 * nxy + c] = Kc;
-      atomicAdd(nfound, 1u);
-      break;
-    }
+  const int k = kc;
+  int k2 = k;
+  if (k + 1 < sp.nz && fabs(psi[col + k]) > fabs(psi[col + k + 1])) k2 = k + 1;
+  const int64_t id = col + k2;
+  const double px = fd_x(psi, sp, i, j, k2), py = fd_y(psi, sp, i, j, k2), pz = fd_z(psi, sp, i, j, k2);
+  const double g = sqrt(px * px + py * py + pz * pz);
+  double d_xy;
+  if (j > 0 && j < sp.ny - 1) d_xy = (fd_x(psi, sp, i, j + 1, k2) - fd_x(psi, sp, i, j - 1, k2)) / sp.tdy;
+  else if (j == 0) d_xy = 2.0 * (fd_x(psi, sp, i, 1, k2) - fd_x(psi, sp, i, 0, k2)) / sp.tdy;
+  else d_xy = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j - 1, k2)) / sp.tdy;
+  // the reference picks the z stencil by k while centring it on k2 (:926-940); a
+  // centred stencil that would leave the column falls back to one-sided
+  double d_xz, d_yz;
+  if (k > 0 && k < sp.nz - 1 && k2 - 1 >= 0 && k2 + 1 <= sp.nz - 1) {
+    d_xz = (fd_x(psi, sp, i, j, k2 + 1) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
+    d_yz = (fd_y(psi, sp, i, j, k2 + 1) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
+  } else if (k == 0 || k2 - 1 < 0) {
+    d_xz = 2.0 * (fd_x(psi, sp, i, j, k2 + 1) - fd_x(psi, sp, i, j, k2)) / sp.tdz;
+    d_yz = 2.0 * (fd_y(psi, sp, i, j, k2 + 1) - fd_y(psi, sp, i, j, k2)) / sp.tdz;
+  } else {
+    d_xz = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
+    d_yz = 2.0 * (fd_y(psi, sp, i, j, k2) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
   }
+  const double fxx = dxx[id], fyy = dyy[id], fzz = dzz[id];
+  const double Hc = ((py * py + pz * pz) * fxx + (px * px + pz * pz) * fyy + (px * px + py * py) * fzz -
+                     2.0 * (px * py * d_xy + px * pz * d_xz + py * pz * d_yz)) /
+                    (g * g * g);
+  const double Kc = (pz * pz * (fxx * fyy - d_xy * d_xy) + px * px * (fyy * fzz - d_yz * d_yz) +
+                     py * py * (fxx * fzz - d_xz * d_xz) +
+                     2.0 * (py * d_xy * (pz * d_xz - px * fzz) + px * d_xz * (py * d_yz - pz * fyy) +
+                            pz * d_yz * (px * d_xy - py * fxx))) /
+                    (g * g * g * g);
+  out[c] = (double)k2;
+  out[nxy + c] = sp.vsign * dt[id] / g;
+  out[2 * nxy + c] = Hc;
+  out[3 * nxy + c] = Kc;
+  atomicAdd(nfound, 1u);
 }
 
 }  // namespace smx

@@ -342,12 +342,17 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
     for (int e = 0; e < E; ++e) v[e] = F[mk_src(Tile<T, L, E, FP>::template pos_in<RP>(tc, e), L) * FP + q];
   }
   if constexpr ((OUT & Z_OUT_REAL) != 0) {
+    // staged through the tile so that each column's L values are written as one contiguous row
+    // (lanes of a wave write consecutive z of a column instead of z values L apart)
+    __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int z = mk_src(Tile<T, L, E, FP>::template pos_in<RP>(tc, e), L);
-      real_out[(f0 + 2 * q) * L + z] = (double)v[e].x;
-      real_out[(f0 + 2 * q + 1) * L + z] = (double)v[e].y;
+    for (int e = 0; e < E; ++e) F[mk_src(Tile<T, L, E, FP>::template pos_in<RP>(tc, e), L) * FP + q] = v[e];
+    __syncthreads();
+    for (int i = tid; i < NC * L; i += NT) {
+      const int c = i / L, z = i - c * L;
+      real_out[(f0 + c) * L + z] = (double)((c & 1) == 0 ? F[z * FP + (c >> 1)].x : F[z * FP + (c >> 1)].y);
     }
+    if constexpr ((OUT & Z_OUT_SPEC) != 0) __syncthreads();  // the forward transform reuses the tile
   }
   if constexpr ((OUT & Z_OUT_SPEC) != 0) {
     if (op == Z_OP_NL) {  // cosNoAdvMorph.cpp:502-503
