@@ -30,9 +30,10 @@ def box_cfgs(seed=5, nbox=6, sizes=(32, 48, 64), prec=A.SMX_FP64):
     return cfgs
 
 
-@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
-def test_batch_graph_equals_individual_runs(prec):
-    cfgs = box_cfgs(prec=prec)
+@pytest.mark.parametrize("prec,sizes", [(A.SMX_FP64, (32, 48, 64)), (A.SMX_FP32_MIXED, (32, 48, 64)),
+                                         (A.SMX_FP32_MIXED, (64, 96, 128))])
+def test_batch_graph_equals_individual_runs(prec, sizes):
+    cfgs = box_cfgs(prec=prec, sizes=sizes)
     psis = [O.ic_noise(c.nx, c.ny, c.nz, 100 + i, 0.05) for i, c in enumerate(cfgs)]
     say("batch sizes", [c.nx for c in cfgs])
     with Batch(cfgs) as b:
