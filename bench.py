@@ -79,15 +79,17 @@ def run_path(precision, psi0, steps, warmup, prof_steps, tune=True, fixed=None):
     from smectic3d_amd import Solver
     cfg = workload_cfg(precision)
     s = Solver(cfg)
+    s.set_psi(psi0)
+    s.set_l1_tracking(False)
+    # planning, as FFTW_MEASURE at set-up (pf3d.cpp:226-291): the autotuner times every tile
+    # schedule on a scratch copy of the loaded state and installs the fastest
     t0 = time.perf_counter()
-    if fixed is not None:  # pinned schedules (the profiled instantiations)
+    if fixed is not None:  # given schedules (profiled runs of fixed kernels)
         s.set_schedule([tuple(x) for x in fixed])
         sched = s.schedule()
     else:
         sched = s.autotune(20) if tune else s.schedule()
     tune_s = time.perf_counter() - t0
-    s.set_psi(psi0)
-    s.set_l1_tracking(False)
     s.step(warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -426,8 +428,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "periodic_256_traffic.json")
 
 def measured_traffic(slot, schedule, n, real="double"):
     """HBM bytes per launch of the kernel behind step slot `slot` under `schedule`, from the
-    committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (each counter calibrated against the
-    copy kernel of known size); None if that instantiation was not measured."""
+    committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary of an autotuning run (so every schedule
+    candidate was dispatched and measured; each counter calibrated against the copy kernel of
+    known size); None if that instantiation was not measured."""
     try:
         with open(TRAFFIC_FILE) as f:
             kernels = json.load(f)["kernels"]
@@ -488,8 +491,8 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the accuracy and secondary legs")
-    ap.add_argument("--tune", action="store_true",
-                    help="autotune the 256^3 step kernels instead of installing the pinned schedules")
+    ap.add_argument("--pinned", action="store_true",
+                    help="install the schedules of profiles/periodic_256_schedules.json instead of autotuning")
     ap.add_argument("--no-tune", action="store_true", help="default tile schedules (no autotuning)")
     ap.add_argument("--leg", choices=["neumann", "advected", "neumann_advected", "batch", "batch_adjoint",
                                       "adjoint", "accuracy"],
@@ -515,10 +518,10 @@ def main():
         else:
             f32, f64 = prev["fp32_mixed"]["schedule"], prev["schedule_fp64"]
         sched_src = "from " + a.schedules_from
-    elif not a.tune and not a.no_tune:
+    elif a.pinned:
         f32, f64 = pinned_schedules()
         if f32 is not None:
-            sched_src = "pinned (profiles/periodic_256_schedules.json, the profiled instantiations)"
+            sched_src = "pinned (profiles/periodic_256_schedules.json)"
     if a.no_tune:
         sched_src = "default"
 
