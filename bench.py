@@ -177,6 +177,7 @@ def run_neumann(steps, surf_every=20, compare=True):
         with Solver(neumann_cfg(prec)) as s:
             s.set_psi(psi0)
             s.set_l1_tracking(False)
+            sched = s.autotune(10)  # tile widths of the Neumann step kernels
             s.sync()
             t0 = time.perf_counter()
             found = 0
@@ -199,7 +200,7 @@ def run_neumann(steps, surf_every=20, compare=True):
         samples[prec] = (np.concatenate(vs) if vs else None, np.concatenate(hs) if hs else None, psi_end)
         key = "fp32" if prec == A.SMX_FP32_MIXED else "fp64"
         out[key] = {"steps_per_s_with_surftrack": steps / (t1 - t0),
-                    "columns_found_per_call": found / max(1, steps // surf_every)}
+                    "columns_found_per_call": found / max(1, steps // surf_every), "schedule": sched}
     r = out["fp32"]
     r.update({"steps": steps, "surftrack_every": surf_every, "surftrack_ms_per_call": 1e3 * t_surf,
               "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]})

@@ -255,16 +255,17 @@ int smx_batch_adjoint_grad(smx_batch *b, const double *const *psi0, int64_t row_
 int smx_batch_adjoint_nodes(smx_batch *b, int64_t *nodes);
 
 /*
- * Tile schedules of the four kernels of the periodic step, in launch order (slot 0: inverse
- * y pass, 1: z pass with N(psi), 2: forward y pass, 3: x pass with the CNAB2 update): the
+ * Tile schedules of the four kernels of the step, in launch order (slot 0: inverse y pass, 1: z
+ * pass with N(psi), 2: forward y pass, 3: x pass with the CNAB2 update).  Periodic handles: the
  * columns per tile (8, 16 or 32) and the radix-plan variant (0: default; 1: the alternative
- * plan of the power-of-two lengths).  Slot 3 also takes variants 2 to 7: plans 0 and 1
- * with the x pass's R stores issued after its inverse transform (2, 3), with all of R
- * read before any of it is rewritten (4, 5), or with the new R staged in LDS and written
- * after the update loop (6, 7), instead of R read and rewritten element by element
- * (bit-identical results).  The step's result depends on the schedule only
- * through rounding.  The default is nc = 16, variant = 0 in every slot.  The advected step
- * uses the schedules of slots 0 and 2 for its eleven y passes.
+ * plan of the power-of-two lengths); slot 3 also takes variants 2 to 7: plans 0 and 1 with the x
+ * pass's R stores issued after its inverse transform (2, 3), with all of R read before any of it
+ * is rewritten (4, 5), or with the new R staged in LDS and written after the update loop (6, 7),
+ * instead of R read and rewritten element by element (bit-identical results).  The default is
+ * nc = 16, variant = 0 in every slot.  The advected step uses the schedules of slots 0 and 2 for
+ * its eleven y passes.  Neumann handles: the tile width only (variant 0): 32 (default) or 16
+ * columns in FP32, 16 (default) or 8 in FP64; the advected Neumann step keeps the defaults.
+ * The step's result depends on the schedule only through rounding.
  */
 typedef struct smx_schedule {
   int nc[4];
@@ -273,7 +274,7 @@ typedef struct smx_schedule {
 int smx_get_schedule(const smx_handle *h, smx_schedule *s);
 /* Install a schedule; SMX_ERR_UNSUPPORTED if some slot has no kernel for this grid (too
  * many threads or too much LDS per tile, or ny / nx*ny not a multiple of nc), or if a
- * non-default schedule is given to a Neumann handle. */
+ * non-default schedule is given to an advected Neumann handle. */
 int smx_set_schedule(smx_handle *h, const smx_schedule *s);
 /* Autotuner: slot by slot, time every valid candidate (reps whole step chains each -- up to
  * 16x more on grids smaller than 256^3, whose kernels are short --, HIP
@@ -282,10 +283,10 @@ int smx_set_schedule(smx_handle *h, const smx_schedule *s);
  * (may be NULL); with SMX_TUNE_VERBOSE=1 in the environment each candidate's median is
  * printed to stderr.  Results are cached per (device,
  * precision, grid) for the life of the process.  Replaces the FFTW_MEASURE planning of
- * pf3d.cpp:226-291 (which times candidate FFT algorithms at set-up).  Periodic handles
- * only; not callable during stream capture. */
+ * pf3d.cpp:226-291 (which times candidate FFT algorithms at set-up).  An advected Neumann
+ * handle keeps its fixed tiles (the call returns them); not callable during stream capture. */
 int smx_autotune(smx_handle *h, int reps, smx_schedule *chosen);
-/* smx_autotune on every periodic box (boxes of one shape are tuned once; Neumann boxes keep
+/* smx_autotune on every box (boxes of one shape are tuned once; advected Neumann boxes keep
  * their fixed tiles); the next smx_batch_run captures its graph again with the new schedules. */
 int smx_batch_autotune(smx_batch *b, int reps);
 

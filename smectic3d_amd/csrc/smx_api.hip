@@ -101,7 +101,7 @@ struct smx_handle {
   bool l1_track = true;         // snapshot the stage buffer before the last step of each smx_step call
   int64_t finite_every = 100;   // steps between the non-finite checks of smx_step (0: off)
   int64_t steps = 0;
-  PassSched sched[NSLOTS];  // tile schedules of the periodic step kernels
+  PassSched sched[NSLOTS];  // tile schedules of the step kernels (Neumann: tile widths, variant 0)
   // device bytes the adjoint may spend on resident record sets: -1 (default) = a share of
   // the free memory for the duration of one call; >= 0 = this many, kept between calls
   int64_t adj_budget = -1;
@@ -283,11 +283,14 @@ static AParams<T> aparams(smx_handle* h, bool mixed_mean) {
 }
 
 // ---- pass facade over both boundary conditions ----
+// s: tile schedule of a periodic pass; on a Neumann handle s.nc is the pass's tile width when
+// neu is set (the Neumann step chain), else the Neumann default width is used.
 template <typename T>
-static int ypass(smx_handle* h, const void* in, void* out, int dir, PassSched s = PassSched{}) {
+static int ypass(smx_handle* h, const void* in, void* out, int dir, PassSched s = PassSched{}, bool neu = false) {
   const DevTables<T>& t = tables<T>(h);
   hipError_t e = h->neumann
-                     ? NeumannOps<T>::ypass((const T*)in, (T*)out, h->cfg.ny, h->npencils, dir, t.twy, t.qtwy, h->st)
+                     ? NeumannOps<T>::ypass((const T*)in, (T*)out, h->cfg.ny, h->npencils, dir, t.twy, t.qtwy, h->st,
+                                            DP_COS, nullptr, neu ? s.nc : 0)
                      : PeriodicOps<T>::ypass((const cplx<T>*)in, (cplx<T>*)out, h->cfg.ny, h->npencils, dir, t.twy,
                                              h->st, nullptr, s);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("y pass: ") + hipGetErrorString(e));
@@ -296,9 +299,9 @@ static int ypass(smx_handle* h, const void* in, void* out, int dir, PassSched s 
 
 template <typename T>
 static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, void* out, void* r, const void* psi0,
-                 PassSched s = PassSched{}) {
+                 PassSched s = PassSched{}, bool neu = false) {
   hipError_t e = h->neumann ? NeumannOps<T>::xpass(mode, (const T*)in, (T*)out, (T*)r, (const T*)psi0, p,
-                                                   tables<T>(h).qtwx, h->st)
+                                                   tables<T>(h).qtwx, h->st, neu ? s.nc : 0)
                             : PeriodicOps<T>::xpass(mode, (const cplx<T>*)in, (cplx<T>*)out, (cplx<T>*)r,
                                                     (const cplx<T>*)psi0, p, h->st, s);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("x pass: ") + hipGetErrorString(e));
@@ -307,9 +310,9 @@ static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, v
 
 template <typename T>
 static int zpass(smx_handle* h, const PParams<T>& p, int in_mode, int out_mode, int op, const void* in, void* out,
-                 const double* real_in, double* real_out, PassSched s = PassSched{}) {
+                 const double* real_in, double* real_out, PassSched s = PassSched{}, bool neu = false) {
   hipError_t e = h->neumann ? NeumannOps<T>::zpass(in_mode, out_mode, op, (const T*)in, (T*)out, real_in, real_out,
-                                                   p, tables<T>(h).qtwz, h->st)
+                                                   p, tables<T>(h).qtwz, h->st, neu ? s.nc : 0)
                             : PeriodicOps<T>::zpass(in_mode, out_mode, op, (const cplx<T>*)in, (cplx<T>*)out,
                                                     real_in, real_out, p, h->st, s);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("z pass: ") + hipGetErrorString(e));
@@ -489,15 +492,17 @@ static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
     return step_advect<T>(h, ap, ev);
   }
   SlotTimer tm(ev);
-  if (h->neumann) {
+  if (h->neumann) {  // the handle's tile widths; the x pass sums one FP64 partial per z tile
+    PParams<T> q = p;
+    q.npartial = NeumannOps<T>::zpass_blocks(p, h->sched[SLOT_Z].nc);
     tm.arm(0);
-    CHK(ypass<T>(h, h->A, h->A, +1));
+    CHK(ypass<T>(h, h->A, h->A, +1, h->sched[SLOT_YINV], true));
     tm.arm(1);
-    CHK(zpass<T>(h, p, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr));
+    CHK(zpass<T>(h, q, Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, h->A, h->A, nullptr, nullptr, h->sched[SLOT_Z], true));
     tm.arm(2);
-    CHK(ypass<T>(h, h->A, h->A, -1));
+    CHK(ypass<T>(h, h->A, h->A, -1, h->sched[SLOT_YFWD], true));
     tm.arm(3);
-    CHK(xpass<T>(h, p, X_STEP, h->A, h->A, h->R, nullptr));
+    CHK(xpass<T>(h, q, X_STEP, h->A, h->A, h->R, nullptr, h->sched[SLOT_X], true));
     return SMX_OK;
   }
   // periodic chain with the handle's tile schedules; the x pass sums the z pass's
@@ -1057,9 +1062,9 @@ static int batch_capture(smx_batch* b, int64_t gsteps) {
 // reference (pf3d.cpp:226-291), for the fused pass kernels.  Results are cached per
 // (device, precision, grid) for the life of the process.
 struct TuneKey {
-  int dev, fp32, nx, ny, nz;
+  int dev, fp32, bc, nx, ny, nz;
   bool operator<(const TuneKey& o) const {
-    return std::tie(dev, fp32, nx, ny, nz) < std::tie(o.dev, o.fp32, o.nx, o.ny, o.nz);
+    return std::tie(dev, fp32, bc, nx, ny, nz) < std::tie(o.dev, o.fp32, o.bc, o.nx, o.ny, o.nz);
   }
 };
 static std::mutex g_tune_mu;
@@ -1069,6 +1074,22 @@ template <typename T>
 static hipError_t launch_slot(smx_handle* h, const PParams<T>& p, int slot, PassSched s, bool dry, void* a,
                               void* r) {
   const DevTables<T>& t = tables<T>(h);
+  if (h->neumann) {  // tile width only
+    if (s.v != 0) return hipErrorInvalidValue;
+    T* B = (T*)a;
+    switch (slot) {
+      case SLOT_YINV:
+        return NeumannOps<T>::ypass(B, B, h->cfg.ny, h->npencils, +1, t.twy, t.qtwy, h->st, DP_COS, nullptr, s.nc, dry);
+      case SLOT_YFWD:
+        return NeumannOps<T>::ypass(B, B, h->cfg.ny, h->npencils, -1, t.twy, t.qtwy, h->st, DP_COS, nullptr, s.nc, dry);
+      case SLOT_Z:
+        return NeumannOps<T>::zpass(Z_IN_SPEC, Z_OUT_SPEC, Z_OP_NL, B, B, nullptr, nullptr, p, t.qtwz, h->st, s.nc, dry);
+      case SLOT_X:
+        return NeumannOps<T>::xpass(X_STEP, B, B, (T*)r, nullptr, p, t.qtwx, h->st, s.nc, dry);
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
   cplx<T>* A = (cplx<T>*)a;
   switch (slot) {
     case SLOT_YINV:
@@ -1112,7 +1133,7 @@ static int autotune_T(smx_handle* h, int reps_in, PassSched* best) {
   PParams<T> p = params<T>(h, h->fp32);
   if (p.partial) p.partial = h->partial + partial_len(h);  // the tuner's own partial-sum region
   p.mean_state = h->mean_state + 4;                        // scratch entries 4..7
-  const int64_t ncomp = 2 * h->nspec;                      // real words of one spectrum
+  const int64_t ncomp = (h->neumann ? 1 : 2) * h->nspec;   // real words of one stage array
   T* wa = (T*)h->D1;
   T* wr = (T*)h->D2;
   auto reset = [&]() -> int {
@@ -1134,7 +1155,7 @@ static int autotune_T(smx_handle* h, int reps_in, PassSched* best) {
   // one chain; slot ks's launch is stamped with (ev0, ev1) when ev0 is set
   auto chain = [&](const PassSched* s, int ks, hipEvent_t ev0, hipEvent_t ev1) -> hipError_t {
     PParams<T> q = p;
-    q.npartial = PeriodicOps<T>::zpass_blocks(p, s[SLOT_Z].nc);
+    q.npartial = h->neumann ? NeumannOps<T>::zpass_blocks(p, s[SLOT_Z].nc) : PeriodicOps<T>::zpass_blocks(p, s[SLOT_Z].nc);
     for (int k = 0; k < NSLOTS; ++k) {
       if (k == ks && ev0) launch_events() = LaunchEvents{ev0, ev1};
       const hipError_t e = launch_slot<T>(h, q, k, s[k], false, wa, wr);
@@ -1144,7 +1165,7 @@ static int autotune_T(smx_handle* h, int reps_in, PassSched* best) {
     return hipSuccess;
   };
   PassSched cur[NSLOTS];
-  for (int k = 0; k < NSLOTS; ++k) cur[k] = PassSched{};
+  for (int k = 0; k < NSLOTS; ++k) cur[k] = h->neumann ? h->sched[k] : PassSched{};  // Neumann: default widths
   const char* vb = std::getenv("SMX_TUNE_VERBOSE");  // per-candidate medians on stderr
   const bool verbose = vb != nullptr && vb[0] != '\0' && vb[0] != '0';
   std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
@@ -1156,7 +1177,7 @@ static int autotune_T(smx_handle* h, int reps_in, PassSched* best) {
     double tbest = 1e300;
     PassSched pick = cur[k];
     const int* vs = (k == SLOT_X) ? kSchedVX : kSchedV;
-    const int nv = (k == SLOT_X) ? 8 : 2;
+    const int nv = h->neumann ? 1 : ((k == SLOT_X) ? 8 : 2);  // Neumann passes have one radix plan
     for (int iv = 0; iv < nv && rc == SMX_OK; ++iv)
       for (int nc : kSchedNC) {
         const PassSched c{nc, vs[iv]};
@@ -1336,6 +1357,11 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
   h->dx = c.lx / c.nx;
   h->dy = c.ly / c.ny;
   h->dz = c.lz / c.nz;
+  if (h->neumann) {  // the default widths; the x pass's tile must divide ny (else 16 columns)
+    const int ncd = h->fp32 ? neumann_nc<float>() : neumann_nc<double>();
+    for (PassSched& ps : h->sched) ps = PassSched{ncd, 0};
+    if (c.ny % ncd != 0) h->sched[SLOT_X].nc = 16;
+  }
   auto bail = [&](int code) {
     smx_destroy(h);
     return code;
@@ -1817,12 +1843,12 @@ int smx_get_schedule(const smx_handle* h, smx_schedule* s) {
 int smx_set_schedule(smx_handle* h, const smx_schedule* s) {
   if (!h || !s) return fail(SMX_ERR_INVALID, "null argument");
   PassSched c[NSLOTS];
-  bool is_default = true;
-  for (int k = 0; k < NSLOTS; ++k) {
-    c[k] = PassSched{s->nc[k], s->variant[k]};
-    is_default = is_default && c[k].nc == kNC && c[k].v == 0;
+  for (int k = 0; k < NSLOTS; ++k) c[k] = PassSched{s->nc[k], s->variant[k]};
+  if (h->neumann && h->cfg.advect) {
+    for (int k = 0; k < NSLOTS; ++k)
+      if (c[k].nc != h->sched[k].nc || c[k].v != h->sched[k].v)
+        return fail(SMX_ERR_UNSUPPORTED, "the advected Neumann step keeps its fixed tile widths");
   }
-  if (!is_default && h->neumann) return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic step only");
   HIPCHK(hipSetDevice(h->cfg.device));
   CHK(h->fp32 ? sched_check<float>(h, c) : sched_check<double>(h, c));
   for (int k = 0; k < NSLOTS; ++k) h->sched[k] = c[k];
@@ -1832,9 +1858,12 @@ int smx_set_schedule(smx_handle* h, const smx_schedule* s) {
 int smx_autotune(smx_handle* h, int reps, smx_schedule* chosen) {
   if (!h) return fail(SMX_ERR_INVALID, "null handle");
   if (reps < 1 || reps > 1000) return fail(SMX_ERR_INVALID, "reps must lie in [1, 1000]");
-  if (h->neumann) return fail(SMX_ERR_UNSUPPORTED, "tile schedules apply to the periodic step only");
+  if (h->neumann && h->cfg.advect) {  // the advected Neumann step keeps its fixed tiles
+    if (chosen) CHK(smx_get_schedule(h, chosen));
+    return SMX_OK;
+  }
   HIPCHK(hipSetDevice(h->cfg.device));
-  const TuneKey key{h->cfg.device, h->fp32 ? 1 : 0, h->cfg.nx, h->cfg.ny, h->cfg.nz};
+  const TuneKey key{h->cfg.device, h->fp32 ? 1 : 0, h->cfg.bc, h->cfg.nx, h->cfg.ny, h->cfg.nz};
   std::array<PassSched, NSLOTS> best;
   bool hit = false;
   {
@@ -1921,8 +1950,8 @@ int smx_batch_run(smx_batch* b, int64_t nsteps, int64_t graph_steps) {
 
 int smx_batch_autotune(smx_batch* b, int reps) {
   if (!b) return fail(SMX_ERR_INVALID, "null batch");
-  for (smx_handle* h : b->boxes)  // Neumann boxes keep their fixed tiles
-    if (!h->neumann) CHK(smx_autotune(h, reps, nullptr));
+  for (smx_handle* h : b->boxes)  // advected Neumann boxes keep their fixed tiles
+    CHK(smx_autotune(h, reps, nullptr));
   // the captured graphs hold the old schedules: capture again on the next run
   batch_drop_adj_graph(b);
   if (b->exec) {

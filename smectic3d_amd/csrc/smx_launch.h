@@ -216,36 +216,57 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
 
 // Tile width (columns, in pairs of adjacent lines) of the Neumann passes: rows of 128 bytes.
 template <typename T> constexpr int neumann_nc() { return sizeof(T) == 4 ? 32 : 16; }
+// The second width of the plain Neumann step chain's kernels (rows of 64 bytes), one of the two
+// candidates of the autotuner; diagnostics and the advected step keep the first.
+template <typename T> constexpr int neumann_nc_alt() { return sizeof(T) == 4 ? 16 : 8; }
 
+// nc: columns per tile, 0 for neumann_nc<T>(); neumann_nc_alt<T>() exists for the step chain's
+// forms (cosine y passes, the z pass spectrum -> spectrum, every x pass).  dry: only check that
+// the kernel exists for this grid.
 template <typename T>
 struct NeumannOps {
   // y pass on real pencils (npencils = nz*nx), dir -1: DCT-II, +1: DCT-III.
   // par: parity of the pencils along y (DPar; DP_DSIN with dir +1 and kq = k_y).
   static hipError_t ypass(const T* in, T* out, int ny, int64_t npencils, int dir, const cplx<T>* tw,
-                          const cplx<T>* qtw, hipStream_t st, int par = DP_COS, const T* kq = nullptr);
+                          const cplx<T>* qtw, hipStream_t st, int par = DP_COS, const T* kq = nullptr, int nc = 0,
+                          bool dry = false);
   static hipError_t xpass(int mode, const T* in, T* out, T* rbuf, const T* psi0, const PParams<T>& p,
-                          const cplx<T>* qtw, hipStream_t st);
+                          const cplx<T>* qtw, hipStream_t st, int nc = 0, bool dry = false);
   static hipError_t zpass(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
-                          double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st);
-  static int zpass_blocks(const PParams<T>& p) { return (int)(p.nxy / neumann_nc<T>()); }
+                          double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st, int nc = 0,
+                          bool dry = false);
+  static int zpass_blocks(const PParams<T>& p, int nc = 0) { return (int)(p.nxy / (nc > 0 ? nc : neumann_nc<T>())); }
 };
+
+template <typename T, int L, int NC>
+static hipError_t dct_ypass_launch(const T* in, T* out, int64_t npencils, int dir, const cplx<T>* tw,
+                                   const cplx<T>* qtw, hipStream_t st, int par, const T* kq) {
+  const dim3 grid((unsigned)(npencils / NC)), block((NC / 2) * (L / Plan<L>::E));
+  if (dir < 0 && par == DP_COS) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC>), grid, block, 0, st, in, out, tw, qtw, kq);
+  else if (par == DP_COS) SMX_LAUNCH((k_dct_ypass<T, L, +1, NC>), grid, block, 0, st, in, out, tw, qtw, kq);
+  else if constexpr (NC == neumann_nc<T>()) {
+    if (dir < 0) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC, DP_SIN>), grid, block, 0, st, in, out, tw, qtw, kq);
+    else if (par == DP_SIN) SMX_LAUNCH((k_dct_ypass<T, L, +1, NC, DP_SIN>), grid, block, 0, st, in, out, tw, qtw, kq);
+    else SMX_LAUNCH((k_dct_ypass<T, L, +1, NC, DP_DSIN>), grid, block, 0, st, in, out, tw, qtw, kq);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 template <typename T>
 hipError_t NeumannOps<T>::ypass(const T* in, T* out, int ny, int64_t npencils, int dir, const cplx<T>* tw,
-                                const cplx<T>* qtw, hipStream_t st, int par, const T* kq) {
-  constexpr int NC = neumann_nc<T>();
-  if (npencils % NC != 0) return hipErrorInvalidValue;  // whole tiles of pencil pairs
+                                const cplx<T>* qtw, hipStream_t st, int par, const T* kq, int nc, bool dry) {
+  constexpr int NCD = neumann_nc<T>(), NCA = neumann_nc_alt<T>();
+  if (nc == 0) nc = NCD;
+  if (nc != NCD && !(nc == NCA && par == DP_COS)) return hipErrorInvalidValue;
+  if (npencils % nc != 0) return hipErrorInvalidValue;  // whole tiles of pencil pairs
   if (par == DP_DSIN && (dir < 0 || kq == nullptr)) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)(npencils / NC));
-#define SMX_CASE(L)                                                                                       \
-  if (ny == L) {                                                                                          \
-    const dim3 block((NC / 2) * (L / Plan<L>::E));                                                        \
-    if (dir < 0 && par == DP_COS) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC>), grid, block, 0, st, in, out, tw, qtw, kq); \
-    else if (dir < 0) SMX_LAUNCH((k_dct_ypass<T, L, -1, NC, DP_SIN>), grid, block, 0, st, in, out, tw, qtw, kq);     \
-    else if (par == DP_COS) SMX_LAUNCH((k_dct_ypass<T, L, +1, NC>), grid, block, 0, st, in, out, tw, qtw, kq);       \
-    else if (par == DP_SIN) SMX_LAUNCH((k_dct_ypass<T, L, +1, NC, DP_SIN>), grid, block, 0, st, in, out, tw, qtw, kq); \
-    else SMX_LAUNCH((k_dct_ypass<T, L, +1, NC, DP_DSIN>), grid, block, 0, st, in, out, tw, qtw, kq);                \
-    return hipGetLastError();                                                                             \
+#define SMX_CASE(L)                                                                                   \
+  if (ny == L) {                                                                                      \
+    if (dry) return hipSuccess;                                                                       \
+    return nc == NCD ? dct_ypass_launch<T, L, NCD>(in, out, npencils, dir, tw, qtw, st, par, kq)     \
+                     : dct_ypass_launch<T, L, NCA>(in, out, npencils, dir, tw, qtw, st, par, kq);    \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
@@ -268,40 +289,57 @@ static hipError_t dct_xpass_launch(int mode, const T* in, T* out, T* rbuf, const
 
 template <typename T>
 hipError_t NeumannOps<T>::xpass(int mode, const T* in, T* out, T* rbuf, const T* psi0, const PParams<T>& p,
-                                const cplx<T>* qtw, hipStream_t st) {
-  // 128-byte rows where ny allows it (ny is a multiple of 16)
-  constexpr int NC = neumann_nc<T>();
-  const bool wide = (p.ny % NC == 0);
+                                const cplx<T>* qtw, hipStream_t st, int nc, bool dry) {
+  // 128-byte rows where ny allows it (ny is a multiple of 16), else the 16-column tile
+  constexpr int NCD = neumann_nc<T>(), NCA = neumann_nc_alt<T>();
   if (p.ny % 16 != 0) return hipErrorInvalidValue;
+  if (nc == 0) nc = (p.ny % NCD == 0) ? NCD : 16;
+  if (nc != NCD && nc != NCA && nc != 16) return hipErrorInvalidValue;
+  if (p.ny % nc != 0) return hipErrorInvalidValue;
 #define SMX_CASE(L)                                                                          \
   if (p.nx == L) {                                                                           \
-    if (wide) return dct_xpass_launch<T, L, NC>(mode, in, out, rbuf, psi0, p, qtw, st);     \
-    if constexpr (NC != 16) return dct_xpass_launch<T, L, 16>(mode, in, out, rbuf, psi0, p, qtw, st); \
-    return hipErrorInvalidValue;                                                             \
+    if (dry) return hipSuccess;                                                              \
+    if (nc == NCD) return dct_xpass_launch<T, L, NCD>(mode, in, out, rbuf, psi0, p, qtw, st); \
+    if (nc == NCA) return dct_xpass_launch<T, L, NCA>(mode, in, out, rbuf, psi0, p, qtw, st); \
+    return dct_xpass_launch<T, L, 16>(mode, in, out, rbuf, psi0, p, qtw, st);                \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
   return hipErrorInvalidValue;
 }
 
+template <typename T, int L, int NC>
+static hipError_t dct_zpass_launch(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
+                                   double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st) {
+  const dim3 grid((unsigned)(p.nxy / NC)), block((NC / 2) * (L / Plan<L>::E));
+  if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)
+    SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw);
+  else if constexpr (NC == neumann_nc<T>()) {
+    if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)
+      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw);
+    else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)
+      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t NeumannOps<T>::zpass(int in_mode, int out_mode, int op, const T* in, T* out, const double* real_in,
-                                double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st) {
-  constexpr int NC = neumann_nc<T>();
-  if (p.nxy % NC != 0) return hipErrorInvalidValue;  // whole tiles
-  const dim3 grid((unsigned)zpass_blocks(p));
-#define SMX_CASE(L)                                                                                  \
-  if (p.nz == L) {                                                                                   \
-    const dim3 block((NC / 2) * (L / Plan<L>::E));                                                   \
-    if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)                                              \
-      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
-    else if (in_mode == Z_IN_REAL && out_mode == Z_OUT_SPEC)                                         \
-      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_REAL, Z_OUT_SPEC>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
-    else if (in_mode == Z_IN_SPEC && out_mode == Z_OUT_REAL)                                         \
-      SMX_LAUNCH((k_dct_zpass<T, L, NC, Z_IN_SPEC, Z_OUT_REAL>), grid, block, 0, st, in, out, real_in, real_out, op, p, qtw); \
-    else                                                                                             \
-      return hipErrorInvalidValue;                                                                   \
-    return hipGetLastError();                                                                        \
+                                double* real_out, const PParams<T>& p, const cplx<T>* qtw, hipStream_t st, int nc,
+                                bool dry) {
+  constexpr int NCD = neumann_nc<T>(), NCA = neumann_nc_alt<T>();
+  if (nc == 0) nc = NCD;
+  if (nc != NCD && !(nc == NCA && in_mode == Z_IN_SPEC && out_mode == Z_OUT_SPEC)) return hipErrorInvalidValue;
+  if (p.nxy % nc != 0) return hipErrorInvalidValue;  // whole tiles
+#define SMX_CASE(L)                                                                                           \
+  if (p.nz == L) {                                                                                            \
+    if (dry) return hipSuccess;                                                                               \
+    return nc == NCD ? dct_zpass_launch<T, L, NCD>(in_mode, out_mode, op, in, out, real_in, real_out, p, qtw, st) \
+                     : dct_zpass_launch<T, L, NCA>(in_mode, out_mode, op, in, out, real_in, real_out, p, qtw, st); \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE

@@ -1,8 +1,9 @@
-"""Tile schedules and the autotuner of the periodic step (libsmectic through the C ABI).
+"""Tile schedules and the autotuner of the periodic and Neumann steps (libsmectic through the
+C ABI).
 
-Every schedule candidate (columns per tile 8/16/32, radix-plan variant 0/1, and for the
-x pass the carry-order forms 2-7) of every step kernel must give the oracle's
-result to the usual tolerance: FP64 relative
+Every schedule candidate (periodic: columns per tile 8/16/32, radix-plan variant 0/1, and for
+the x pass the carry-order forms 2-7; Neumann: tile widths 32/16 in FP32, 16/8 in FP64) of
+every step kernel must give the oracle's result to the usual tolerance: FP64 relative
 L2(psi) <= 1e-11 after 5 steps, FP32-mixed <= 2e-5.  The autotuner must leave the
 state untouched, return a valid schedule, and the tuned step must keep parity.
 """
@@ -77,6 +78,64 @@ def test_every_schedule_candidate_matches_oracle(shape, prec):
     assert ran >= 12  # the default plus at least two alternatives per slot
 
 
+NEU_SHAPES = [(64, 64, 64), (32, 48, 96), (96, 32, 48), (48, 96, 32), (16, 32, 384)]
+
+
+@pytest.mark.parametrize("shape", NEU_SHAPES)
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+def test_every_neumann_tile_width_matches_oracle(shape, prec):
+    say("neumann widths", shape, prec)
+    cfg = cfg_reference(*shape, bc=A.SMX_BC_NEUMANN, eps=0.5, beta=0.3, dt=0.05, nl_model=A.SMX_NL_CUBIC,
+                        precision=prec)
+    psi = O.ic_noise(*shape, 29, 0.3)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(5)
+    want = orc.get_psi()
+    tol = 1e-11 if prec == A.SMX_FP64 else 2e-5
+    wide, narrow = (32, 16) if prec == A.SMX_FP32_MIXED else (16, 8)
+    ran = 0
+    with Solver(cfg) as s:
+        base = s.schedule()
+        assert base == [(wide, 0)] * 4 or base[3] == (16, 0)  # x falls back to 16 when ny is not a multiple
+        for sched in ([(narrow, 0)] + base[1:], base[:1] + [(narrow, 0)] + base[2:],
+                      base[:2] + [(narrow, 0)] + base[3:], base[:3] + [(narrow, 0)], [(narrow, 0)] * 4):
+            try:
+                s.set_schedule(sched)
+            except SmxError as e:
+                assert e.code == A.SMX_ERR_UNSUPPORTED
+                continue
+            s.set_psi(psi)
+            s.step(5)
+            err = rel_l2(s.get_psi(), want)
+            say("  ", sched, "err", err)
+            assert err < tol, (sched, err)
+            ran += 1
+    assert ran >= 4
+
+
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+def test_neumann_autotune_keeps_state_and_parity(prec):
+    shape = (64, 48, 96)
+    cfg = cfg_reference(*shape, bc=A.SMX_BC_NEUMANN, eps=0.5, beta=0.3, dt=0.05, nl_model=A.SMX_NL_CUBIC,
+                        precision=prec)
+    psi = O.ic_noise(*shape, 37, 0.05)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(20)
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        s.step(10)
+        before = s.get_psi()
+        chosen = s.autotune(5)
+        say("  neumann chosen", chosen)
+        assert s.schedule() == chosen and all(v == 0 for _, v in chosen)
+        assert np.array_equal(s.get_psi(), before)
+        s.step(10)
+        err = rel_l2(s.get_psi(), orc.get_psi())
+    assert err < (1e-11 if prec == A.SMX_FP64 else 1e-5)
+
+
 def test_invalid_schedules_rejected():
     say("invalid schedules")
     with Solver(cfg_reference(64, 48, 32)) as s:
@@ -86,14 +145,17 @@ def test_invalid_schedules_rejected():
                 s.set_schedule(bad)  # nc 7; y variant 2; x tiles of 32 do not divide ny = 48; x variant 8
             assert ei.value.code == A.SMX_ERR_UNSUPPORTED
         assert s.schedule() == DEFAULT
-    with Solver(cfg_reference(32, 32, 32, bc=A.SMX_BC_NEUMANN)) as s:
-        s.set_schedule(DEFAULT)
+    with Solver(cfg_reference(32, 32, 32, bc=A.SMX_BC_NEUMANN)) as s:  # FP64 Neumann: widths 16 or 8
+        assert s.schedule() == [(16, 0)] * 4
+        for bad in ([(32, 0)] + [(16, 0)] * 3, [(16, 1)] + [(16, 0)] * 3):
+            with pytest.raises(SmxError) as ei:
+                s.set_schedule(bad)  # no 32-wide FP64 Neumann y pass; no second radix plan
+            assert ei.value.code == A.SMX_ERR_UNSUPPORTED
+    with Solver(cfg_reference(32, 32, 32, bc=A.SMX_BC_NEUMANN, advect=1, dt=2e-3)) as s:
         with pytest.raises(SmxError) as ei:
-            s.set_schedule([(8, 0)] + DEFAULT[1:])
+            s.set_schedule([(8, 0)] + [(16, 0)] * 3)  # the advected Neumann step keeps its tiles
         assert ei.value.code == A.SMX_ERR_UNSUPPORTED
-        with pytest.raises(SmxError) as ei:
-            s.autotune(2)
-        assert ei.value.code == A.SMX_ERR_UNSUPPORTED
+        assert s.autotune(2) == [(16, 0)] * 4
 
 
 @pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
