@@ -92,6 +92,7 @@ struct smx_handle {
   void* S1 = nullptr;  // S_y then U_y
   void* S2 = nullptr;  // S_z then U_z
   double* surf = nullptr;    // 4 x nxy persistent surfTrack outputs (zero-initialised)
+  double* hsurf = nullptr;   // pinned host copy of them (allocated on first use)
   double* sfield = nullptr;  // surfTrack real fields, 8 x N (allocated on first use)
   void* advw[6] = {};        // FP64 stage arrays of the advected dpsi/dt (allocated on first use)
   DevTables<float> t32;
@@ -1406,6 +1407,7 @@ void smx_destroy(smx_handle* h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->hflag) (void)hipHostFree(h->hflag);
+  if (h->hsurf) (void)hipHostFree(h->hsurf);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
 }
@@ -1781,12 +1783,15 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
     hipLaunchKernelGGL(k_surf_periodic, dim3(blocks), dim3(256), 0, h->st, f, f + N, f + 2 * N, f + 3 * N, f + 4 * N,
                        f + 5 * N, f + 6 * N, f + 7 * N, h->real_io, kcol, sp, h->surf, h->flag);
   HIPCHK(hipGetLastError());
-  double* outs[4] = {surfz, vel, H, K};
-  for (int i = 0; i < 4; ++i)
-    HIPCHK(hipMemcpyAsync(outs[i], h->surf + i * nxy, sizeof(double) * nxy, hipMemcpyDeviceToHost, h->st));
-  unsigned int nf = 0;
-  HIPCHK(hipMemcpyAsync(&nf, h->flag, sizeof(nf), hipMemcpyDeviceToHost, h->st));
+  // the four output arrays and the count in one transfer to pinned host memory
+  if (!h->hsurf) HIPCHK(hipHostMalloc((void**)&h->hsurf, sizeof(double) * (4 * (size_t)nxy + 1), hipHostMallocDefault));
+  HIPCHK(hipMemcpyAsync(h->hsurf, h->surf, sizeof(double) * 4 * (size_t)nxy, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(h->hsurf + 4 * nxy, h->flag, sizeof(unsigned int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  double* outs[4] = {surfz, vel, H, K};
+  for (int i = 0; i < 4; ++i) std::memcpy(outs[i], h->hsurf + i * nxy, sizeof(double) * nxy);
+  unsigned int nf = 0;
+  std::memcpy(&nf, h->hsurf + 4 * nxy, sizeof(nf));
   if (found) *found = nf;
   return SMX_OK;
 }

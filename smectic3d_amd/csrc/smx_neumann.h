@@ -331,11 +331,11 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
     }
     InvFFT<T, L, FP>::run(v, F, q, tc, twl);
   } else {
-    for (int i = tid; i < NC * L; i += NT) {
-      const int c = i / L, z = i - c * L;
-      const T val = (T)real_in[(f0 + c) * L + z];
-      if ((c & 1) == 0) F[z * FP + (c >> 1)].x = val;
-      else F[z * FP + (c >> 1)].y = val;
+    // a lane reads one z of both columns of a pair (two coalesced rows) and writes the pair as
+    // one complex: consecutive lanes hit consecutive LDS rows, without bank conflicts
+    for (int i = tid; i < NPR * L; i += NT) {
+      const int qq = i / L, z = i - qq * L;
+      F[z * FP + qq] = mk((T)real_in[(f0 + 2 * qq) * L + z], (T)real_in[(f0 + 2 * qq + 1) * L + z]);
     }
     __syncthreads();
 #pragma unroll
@@ -348,9 +348,11 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
 #pragma unroll
     for (int e = 0; e < E; ++e) F[mk_src(Tile<T, L, E, FP>::template pos_in<RP>(tc, e), L) * FP + q] = v[e];
     __syncthreads();
-    for (int i = tid; i < NC * L; i += NT) {
-      const int c = i / L, z = i - c * L;
-      real_out[(f0 + c) * L + z] = (double)((c & 1) == 0 ? F[z * FP + (c >> 1)].x : F[z * FP + (c >> 1)].y);
+    for (int i = tid; i < NPR * L; i += NT) {
+      const int qq = i / L, z = i - qq * L;
+      const cplx<T> w = F[z * FP + qq];
+      real_out[(f0 + 2 * qq) * L + z] = (double)w.x;
+      real_out[(f0 + 2 * qq + 1) * L + z] = (double)w.y;
     }
     if constexpr ((OUT & Z_OUT_SPEC) != 0) __syncthreads();  // the forward transform reuses the tile
   }

@@ -338,11 +338,11 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     }
     InvFFT<T, L, FP, V>::run(v, F, q, tc, twl);
   } else {
-    for (int i = tid; i < NC * L; i += NT) {
-      const int c = i / L, n = i - c * L;
-      const T val = (T)real_in[(f0 + c) * L + n];
-      if ((c & 1) == 0) F[n * FP + (c >> 1)].x = val;
-      else F[n * FP + (c >> 1)].y = val;
+    // a lane reads one z of both columns of a pair (two coalesced rows) and writes the pair as
+    // one complex: consecutive lanes hit consecutive LDS rows, without bank conflicts
+    for (int i = tid; i < NPR * L; i += NT) {
+      const int qq = i / L, n = i - qq * L;
+      F[n * FP + qq] = mk((T)real_in[(f0 + 2 * qq) * L + n], (T)real_in[(f0 + 2 * qq + 1) * L + n]);
     }
     __syncthreads();
     Tile<T, L, E, FP>::template load<fwd_first<L, V>()>(v, F, q, tc);
@@ -354,9 +354,11 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     __syncthreads();
     Tile<T, L, E, FP>::template store<RP>(v, F, q, tc);
     __syncthreads();
-    for (int i = tid; i < NC * L; i += NT) {
-      const int c = i / L, n = i - c * L;
-      real_out[(f0 + c) * L + n] = (double)((c & 1) == 0 ? F[n * FP + (c >> 1)].x : F[n * FP + (c >> 1)].y);
+    for (int i = tid; i < NPR * L; i += NT) {
+      const int qq = i / L, n = i - qq * L;
+      const cplx<T> w = F[n * FP + qq];
+      real_out[(f0 + 2 * qq) * L + n] = (double)w.x;
+      real_out[(f0 + 2 * qq + 1) * L + n] = (double)w.y;
     }
   }
   if constexpr ((OUT & Z_OUT_SPEC) != 0) {
