@@ -80,7 +80,7 @@ struct smx_handle {
   void* D2 = nullptr;
   double* real_io = nullptr;
   double* real_prev = nullptr;  // FP64 scratch: psi^n of the L1 criterion, psi0 across the adjoint
-  void* Aprev = nullptr;        // stage buffer before the last step of an smx_step call (L1; on first use)
+  void* Aprev = nullptr;        // stage buffer before the last step of an smx_step call (for L1)
   double* partial = nullptr;  // z-pass FP64 partial sums (mixed path mean mode)
   double* mean_state = nullptr;
   double* red = nullptr;  // reduction partials
@@ -728,8 +728,7 @@ static int run_steps(smx_handle* h, int64_t nsteps, double* partial = nullptr, b
   PParams<T> p = params<T>(h, h->fp32);
   if (partial && p.partial) p.partial = partial;
   const size_t sbytes = h->wbytes * (size_t)h->nspec;
-  snapshot = snapshot && h->l1_track;
-  if (snapshot && !h->Aprev) CHK(dalloc(h, &h->Aprev, sbytes));
+  snapshot = snapshot && h->l1_track;  // Aprev is allocated with the handle: nothing is allocated here
   for (int64_t n = 0; n < nsteps; ++n) {
     // stage buffer of psi^n for the L1 criterion of the last step (smx_l1 transforms it)
     if (n == nsteps - 1 && snapshot)
@@ -1374,6 +1373,7 @@ int smx_create(const smx_config* cfg, smx_handle** out) {
   const size_t dbytes = (h->neumann ? 8 : 16) * (size_t)h->nspec;
   int rc = SMX_OK;
   if ((rc = dalloc(h, &h->A, h->wbytes * h->nspec)) || (rc = dalloc(h, &h->R, h->wbytes * h->nspec)) ||
+      (rc = dalloc(h, &h->Aprev, h->wbytes * h->nspec)) ||
       (rc = dalloc(h, &h->P, h->wbytes * h->nspec)) || (rc = dalloc(h, &h->D1, dbytes)) ||
       (rc = dalloc(h, &h->D2, dbytes)) || (rc = dalloc(h, (void**)&h->real_io, sizeof(double) * h->N)) ||
       (rc = dalloc(h, (void**)&h->real_prev, sizeof(double) * h->N)) ||
@@ -2022,8 +2022,13 @@ int smx_batch_adjoint_grad(smx_batch* b, const double* const* psi0, int64_t row_
       budget = (int64_t)std::min(need, avail * need / need_shared);
       keep_all = false;
     }
-    CHK(adj_prepare(h, nsteps, K, budget));
-    HIPCHK(hipStreamSynchronize(h->st));
+    int prc = adj_prepare(h, nsteps, K, budget);
+    if (prc == SMX_OK && hipStreamSynchronize(h->st) != hipSuccess) prc = fail(SMX_ERR_HIP, "hipStreamSynchronize");
+    if (prc != SMX_OK) {  // what the default budgets lent for this call goes back
+      for (smx_handle* g : b->boxes)
+        if (g->adj_budget < 0) adj_release(g);
+      return prc;
+    }
   }
   std::vector<double> key;
   key.push_back((double)nsteps);
