@@ -92,8 +92,9 @@ struct smx_handle {
   void* S1 = nullptr;  // S_y then U_y
   void* S2 = nullptr;  // S_z then U_z
   double* surf = nullptr;    // 4 x nxy persistent surfTrack outputs (zero-initialised)
-  double* hsurf = nullptr;   // pinned host copy of them (allocated on first use)
-  double* sfield = nullptr;  // surfTrack real fields, 8 x N (allocated on first use)
+  double* hsurf = nullptr;   // mapped pinned host copy of them and the count (allocated on first use)
+  double* hsurf_dev = nullptr;  // its device address
+  double* sfield = nullptr;  // surfTrack work arrays: real psi, two stage arrays, point values (on first use)
   void* advw[6] = {};        // FP64 stage arrays of the advected dpsi/dt (allocated on first use)
   DevTables<float> t32;
   DevTables<double> t64;
@@ -285,15 +286,20 @@ static AParams<T> aparams(smx_handle* h, bool mixed_mean) {
 
 // ---- pass facade over both boundary conditions ----
 // s: tile schedule of a periodic pass; on a Neumann handle s.nc is the pass's tile width when
-// neu is set (the Neumann step chain), else the Neumann default width is used.
+// neu is set (the Neumann step chain), else the Neumann default width is used.  kop: k_y
+// multiplier of an inverse pass's input (KM_IKY periodic only, KM_DYY; < 0 none).
 template <typename T>
-static int ypass(smx_handle* h, const void* in, void* out, int dir, PassSched s = PassSched{}, bool neu = false) {
+static int ypass(smx_handle* h, const void* in, void* out, int dir, PassSched s = PassSched{}, bool neu = false,
+                 int kop = -1) {
   const DevTables<T>& t = tables<T>(h);
+  if (kop >= 0 && (dir < 0 || (kop != KM_DYY && (h->neumann || kop != KM_IKY))))
+    return fail(SMX_ERR_INVALID, "y pass: unsupported multiplier");
+  const T* kq = kop >= 0 ? t.ky : nullptr;
   hipError_t e = h->neumann
                      ? NeumannOps<T>::ypass((const T*)in, (T*)out, h->cfg.ny, h->npencils, dir, t.twy, t.qtwy, h->st,
-                                            DP_COS, nullptr, neu ? s.nc : 0)
+                                            DP_COS, kq, neu ? s.nc : 0)
                      : PeriodicOps<T>::ypass((const cplx<T>*)in, (cplx<T>*)out, h->cfg.ny, h->npencils, dir, t.twy,
-                                             h->st, nullptr, s);
+                                             h->st, kq, s, false, kop >= 0 ? kop : KM_IKY);
   if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("y pass: ") + hipGetErrorString(e));
   return SMX_OK;
 }
@@ -1694,17 +1700,21 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   if (!h->has_state) return fail(SMX_ERR_STATE, "no state: call smx_set_psi first");
   if (!(opts->arm >= 0.0)) return fail(SMX_ERR_INVALID, "arm must be >= 0 (the scan arms above it, then crosses 0)");
   HIPCHK(hipSetDevice(h->cfg.device));
-  const int64_t N = h->N;
+  const int64_t N = h->N, nxy = h->nxy;
   const size_t sdoubles = (h->neumann ? 1 : 2) * (size_t)h->nspec;  // one FP64 stage array
-  // real fields f[0..7], two FP64 stage arrays (W0, W1) and the crossing index of each column
+  // real psi, two FP64 stage arrays (W0, W1), the field values at each column's evaluation
+  // index, and the crossing and evaluation indices of each column
   if (!h->sfield)
-    CHK(dalloc(h, (void**)&h->sfield, sizeof(double) * (8 * (size_t)N + 2 * sdoubles) + sizeof(int) * (size_t)h->nxy));
-  double* f = h->sfield;
-  double* W0 = f + 8 * N;       // x stage of psi_hat: [kz][x][ky]
-  double* W1 = W0 + sdoubles;   // y inverse of it: [kz][x][y]
-  int* kcol = reinterpret_cast<int*>(W1 + sdoubles);
+    CHK(dalloc(h, (void**)&h->sfield,
+               sizeof(double) * ((size_t)N + 2 * sdoubles + 8 * (size_t)nxy) + 2 * sizeof(int) * (size_t)nxy));
+  double* f = h->sfield;         // psi, dense [x][y][z]
+  double* W0 = f + N;            // x stage of psi_hat: [kz][x][ky]
+  double* W1 = W0 + sdoubles;    // y inverse of it: [kz][x][y]
+  double* val = W1 + sdoubles;   // SurfVal fields, nxy apiece
+  int* kcol = reinterpret_cast<int*>(val + 8 * nxy);
+  int* kev = kcol + nxy;
   PParams<double> p = params<double>(h, false);
-  // W0 from the state (the FP64 state is its own x stage), psi_hat into D2, psi into f[0]
+  // W0 from the state (the FP64 state is its own x stage), psi_hat into D2, psi into f
   const void* x_stage = h->A;
   if (h->fp32) {
     CHK(stage_to_f64(h, h->A, W0));
@@ -1717,46 +1727,73 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   }
   CHK(ypass<double>(h, x_stage, W1, +1));
   CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, W1, nullptr, nullptr, f));
-  // dpsi/dt into f[1]: N_hat of psi (with the advective term when advection is on), then
-  // F^-1[aLin psi_hat + N_hat] (cosNoAdvMorphCurv.cpp:744,764; surfTrack.cpp:585-611)
+  // the crossing of every column (one wave per column) and the index its fields are taken at
+  const unsigned sblocks = (unsigned)((nxy + 3) / 4), blocks = (unsigned)((nxy + 255) / 256);
+  if (h->neumann)
+    hipLaunchKernelGGL(k_surf_scan<false>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol,
+                       kev);
+  else
+    hipLaunchKernelGGL(k_surf_scan<true>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol,
+                       kev);
+  HIPCHK(hipGetLastError());
+  // Derivative and rate fields, each evaluated only at the columns' evaluation indices: the x
+  // and y transforms of the whole field, then the z transform at one point per column.  Each
+  // multiplier is applied by the inverse pass that loads its input, at the stage where the
+  // transforms it does not depend on are already done: on W1 (x and y inverted) for one of
+  // kz, on W0 (x inverted) for one of ky, on the full spectrum D2 otherwise.
+  const double* kzt = h->t64.kz;
+  auto point = [&](const double* zspec, int kop, int slot) -> int {
+    const unsigned pblocks = (unsigned)((nxy + 63) / 64);
+    if (h->neumann)
+      hipLaunchKernelGGL(k_surf_point<true>, dim3(pblocks), dim3(256), 0, h->st, zspec, kev, nxy, h->cfg.nz, kzt, kop,
+                         val + (size_t)slot * nxy);
+    else
+      hipLaunchKernelGGL(k_surf_point<false>, dim3(pblocks), dim3(256), 0, h->st, zspec, kev, nxy, h->cfg.nz, kzt,
+                         kop, val + (size_t)slot * nxy);
+    HIPCHK(hipGetLastError());
+    return SMX_OK;
+  };
+  auto along_z = [&](int mode, int slot) -> int { return point(W1, mode, slot); };
+  auto along_y = [&](int mode, int slot) -> int {
+    CHK(ypass<double>(h, x_stage, h->D1, +1, PassSched{}, false, mode));
+    return point((const double*)h->D1, -1, slot);
+  };
+  auto x_then_y = [&](int mode, const void* spec) -> int {  // x inverse (with the multiplier), y inverse, in D1
+    PParams<double> pk = p;
+    pk.kop = mode;
+    CHK(xpass<double>(h, pk, X_INV, spec, h->D1, nullptr, nullptr));
+    return ypass<double>(h, h->D1, h->D1, +1);
+  };
+  auto along_x = [&](int mode, int slot) -> int {
+    CHK(x_then_y(mode, h->D2));
+    return point((const double*)h->D1, -1, slot);
+  };
+  // dpsi/dt = F^-1[aLin psi_hat + N_hat] (cosNoAdvMorphCurv.cpp:744,764; surfTrack.cpp:585-611).
+  // Without advection N_hat is the transform of the real N(psi), so F^-1 N_hat = N(psi): the
+  // rate field takes the linear part and the output kernel adds N(psi) at each crossing.  With
+  // advection N_hat carries -U.grad psi of the spectral flow.
   if (h->cfg.advect) {
     CHK(advected_n_spectrum(h));
+    CHK(kmul(h, KM_DTPSI, h->D2, h->D1));
+    CHK(x_then_y(-1, h->D1));
+    CHK(point((const double*)h->D1, -1, SV_RATE));
   } else {
-    CHK(nl_spectrum_of(h, f));
+    CHK(along_x(KM_ALIN, SV_RATE));
   }
-  CHK(kmul(h, KM_DTPSI, h->D2, h->D1));
-  CHK(c2r_spectrum_d1(h, f + N));
-  // Derivatives.  A multiplier that depends on one wavenumber commutes with the transforms
-  // of the other axes, so it is applied where those are already done: on W1 (x and y
-  // inverted) for kz, on W0 (x inverted) for ky, on the full spectrum D2 for kx.
-  auto along_z = [&](int mode, double* out) -> int {
-    CHK(kmul(h, mode, W1, h->D1));
-    return zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out);
-  };
-  auto along_y = [&](int mode, double* out) -> int {
-    CHK(kmul(h, mode, x_stage, h->D1));
-    CHK(ypass<double>(h, h->D1, h->D1, +1));
-    return zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, out);
-  };
-  auto along_x = [&](int mode, double* out) -> int {
-    CHK(kmul(h, mode, h->D2, h->D1));
-    return c2r_spectrum_d1(h, out);
-  };
   if (!h->neumann) {
-    // f[2..4] spectral gradient, f[5] psi_xx + psi_yy, f[6] psi_xx, f[7] psi_yy, psi_xy in
-    // real_io (surfTrack.cpp:611-626)
-    CHK(along_x(KM_IKX, f + 2 * N));
-    CHK(along_y(KM_IKY, f + 3 * N));
-    CHK(along_z(KM_IKZ, f + 4 * N));
-    CHK(along_x(KM_LAP_XY, f + 5 * N));
-    CHK(along_x(KM_DXX, f + 6 * N));
-    CHK(along_y(KM_DYY, f + 7 * N));
-    CHK(along_x(KM_DXY, h->real_io));
+    // spectral gradient, psi_xx + psi_yy, psi_xx, psi_yy, psi_xy (surfTrack.cpp:611-626)
+    CHK(along_x(KM_IKX, SV_GX));
+    CHK(along_y(KM_IKY, SV_GY));
+    CHK(along_z(KM_IKZ, SV_GZ));
+    CHK(along_x(KM_LAP_XY, SV_HNUM));
+    CHK(along_x(KM_DXX, SV_DXX));
+    CHK(along_y(KM_DYY, SV_DYY));
+    CHK(along_x(KM_DXY, SV_DXY));
   } else {
-    // f[2] psi_xx, f[3] psi_yy, f[4] psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750)
-    CHK(along_x(KM_DXX, f + 2 * N));
-    CHK(along_y(KM_DYY, f + 3 * N));
-    CHK(along_z(KM_DZZ, f + 4 * N));
+    // psi_xx, psi_yy, psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750)
+    CHK(along_x(KM_DXX, SV_DXX));
+    CHK(along_y(KM_DYY, SV_DYY));
+    CHK(along_z(KM_DZZ, SV_DZZ));
   }
   HIPCHK(hipMemsetAsync(h->flag, 0, sizeof(unsigned int), h->st));
   SurfParams sp;
@@ -1768,31 +1805,34 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   sp.tdx = 2 * h->dx;  // cosNoAdvMorphCurv.cpp:167-169
   sp.tdy = 2 * h->dy;
   sp.tdz = 2 * h->dz;
-  const int64_t nxy = h->nxy;
-  // the crossing of every column (one wave per column), then each column's outputs (one thread)
-  const unsigned sblocks = (unsigned)((nxy + 3) / 4), blocks = (unsigned)((nxy + 255) / 256);
+  sp.add_nl = h->cfg.advect ? 0 : 1;
+  sp.nl_model = p.nl_model;
+  sp.mid = h->neumann ? -1 : p.mid;  // the DCT step has no mid-plane penalty
+  sp.beta = p.beta;
+  sp.gamma = p.gamma;
+  sp.pen = p.pen;
+  // each column's outputs (one thread per column)
   if (h->neumann)
-    hipLaunchKernelGGL(k_surf_scan<false>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol);
+    hipLaunchKernelGGL(k_surf_neumann, dim3(blocks), dim3(256), 0, h->st, f, val, kcol, kev, sp, h->surf, h->flag);
   else
-    hipLaunchKernelGGL(k_surf_scan<true>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol);
-  HIPCHK(hipGetLastError());
-  if (h->neumann)
-    hipLaunchKernelGGL(k_surf_neumann, dim3(blocks), dim3(256), 0, h->st, f, f + N, f + 2 * N, f + 3 * N, f + 4 * N,
-                       kcol, sp, h->surf, h->flag);
-  else
-    hipLaunchKernelGGL(k_surf_periodic, dim3(blocks), dim3(256), 0, h->st, f, f + N, f + 2 * N, f + 3 * N, f + 4 * N,
-                       f + 5 * N, f + 6 * N, f + 7 * N, h->real_io, kcol, sp, h->surf, h->flag);
+    hipLaunchKernelGGL(k_surf_periodic, dim3(blocks), dim3(256), 0, h->st, f, val, kcol, kev, sp, h->surf, h->flag);
   HIPCHK(hipGetLastError());
   // the four output arrays and the count in one transfer to pinned host memory
-  if (!h->hsurf) HIPCHK(hipHostMalloc((void**)&h->hsurf, sizeof(double) * (4 * (size_t)nxy + 1), hipHostMallocDefault));
-  HIPCHK(hipMemcpyAsync(h->hsurf, h->surf, sizeof(double) * 4 * (size_t)nxy, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(h->hsurf + 4 * nxy, h->flag, sizeof(unsigned int), hipMemcpyDeviceToHost, h->st));
+  if (!h->hsurf_dev) {  // fine-grained (coherent): the kernel's writes are visible once the stream is idle
+    if (!h->hsurf)
+      HIPCHK(hipHostMalloc((void**)&h->hsurf, sizeof(double) * (4 * (size_t)nxy + 1),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&h->hsurf_dev, h->hsurf, 0));
+  }
+  // written by a kernel in stream order: a runtime device-to-host copy here starts only some
+  // 200 us after the last kernel
+  hipLaunchKernelGGL(k_surf_export, dim3((unsigned)std::min<int64_t>((4 * nxy + 255) / 256, 1024)), dim3(256), 0,
+                     h->st, h->surf, h->flag, 4 * nxy, h->hsurf_dev);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->st));
   double* outs[4] = {surfz, vel, H, K};
   for (int i = 0; i < 4; ++i) std::memcpy(outs[i], h->hsurf + i * nxy, sizeof(double) * nxy);
-  unsigned int nf = 0;
-  std::memcpy(&nf, h->hsurf + 4 * nxy, sizeof(nf));
-  if (found) *found = nf;
+  if (found) *found = (int64_t)h->hsurf[4 * nxy];
   return SMX_OK;
 }
 

@@ -20,6 +20,21 @@
 
 namespace smx {
 
+// k-space multipliers of the derivative and rate fields (the kmul kernels, and the transform
+// passes that apply one to their spectral input as it is loaded)
+enum KMode {
+  KM_DTPSI = 0,   // out = aLin(k) in + out         (cosNoAdvMorphCurv.cpp:744, surfTrack.cpp:611)
+  KM_IKX = 1,     // out = i kx in                 (pf3d.cpp:404)
+  KM_IKY = 2,     //                               (:405)
+  KM_IKZ = 3,     //                               (:406)
+  KM_DXX = 4,     // out = -kx^2 in                (surfTrack.cpp:615, cosNoAdvMorphCurv.cpp:746)
+  KM_DYY = 5,     //                               (surfTrack.cpp:616, cosNoAdvMorphCurv.cpp:748)
+  KM_DZZ = 6,     //                               (cosNoAdvMorphCurv.cpp:750)
+  KM_DXY = 7,     // out = (i ky)(i kx) in         (surfTrack.cpp:617)
+  KM_LAP_XY = 8,  // out = -(kx^2 + ky^2) in       (surfTrack.cpp:612-613)
+  KM_ALIN = 9,    // out = aLin(k) in              (the linear part of dpsi/dt)
+};
+
 template <typename T>
 struct __attribute__((aligned(2 * sizeof(T)))) cplx {
   T x, y;

@@ -26,19 +26,6 @@ struct KParams {
   double eps, alpha, q02;
 };
 
-// k-space multipliers
-enum KMode {
-  KM_DTPSI = 0,   // out = aLin(k) in + out         (cosNoAdvMorphCurv.cpp:744, surfTrack.cpp:611)
-  KM_IKX = 1,     // out = i kx in                 (pf3d.cpp:404)
-  KM_IKY = 2,     //                               (:405)
-  KM_IKZ = 3,     //                               (:406)
-  KM_DXX = 4,     // out = -kx^2 in                (surfTrack.cpp:615, cosNoAdvMorphCurv.cpp:746)
-  KM_DYY = 5,     //                               (surfTrack.cpp:616, cosNoAdvMorphCurv.cpp:748)
-  KM_DZZ = 6,     //                               (cosNoAdvMorphCurv.cpp:750)
-  KM_DXY = 7,     // out = (i ky)(i kx) in         (surfTrack.cpp:617)
-  KM_LAP_XY = 8,  // out = -(kx^2 + ky^2) in       (surfTrack.cpp:612-613)
-};
-
 // (kz, x, ky) of entry i of a [kz][x][ky] array.  32-bit arithmetic: every supported grid has
 // fewer than 2^31 spectral entries (512^3 Neumann: 1.3e8), and 64-bit division costs as much
 // issue time as these grid-stride kernels spend on memory.
@@ -213,6 +200,16 @@ __global__ void k_zero_f64(double* __restrict__ a, int64_t n) {
     a[i] = 0.0;
 }
 
+// The persistent surfTrack outputs (n doubles) and the crossing count into a mapped pinned host
+// buffer (count after them, as a double): written by the kernel in stream order, so the host
+// reads them once the stream is idle, without a runtime copy.
+__global__ void k_surf_export(const double* __restrict__ a, const unsigned int* __restrict__ count, int64_t n,
+                              double* __restrict__ host) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    host[i] = a[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) host[n] = (double)count[0];
+}
+
 __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = (double)a[i];
@@ -382,7 +379,26 @@ struct SurfParams {
   int nx, ny, nz;
   double arm, vsign;
   double tdx, tdy, tdz;  // 2 dx, 2 dy, 2 dz
+  // add_nl: the rate field holds F^-1[aLin psi_hat] alone and N(psi) is added at the crossing
+  // (F^-1 F N = N without advection); the model, beta, gamma and the mid-plane penalty of N
+  int add_nl, nl_model, mid;
+  double beta, gamma, pen;
 };
+
+// N(psi) at z index k of a column (pf3d.cpp:460-461; the cubic model; the mid-plane penalty
+// 2 wMid (psi_{k-1} - 2 psi_k + psi_{k+1})/dz^2 of pf3dMorph.cpp:572-581 at k = mid).
+__device__ __forceinline__ double surf_nl(const double* psi, const SurfParams& s, int64_t col, int k) {
+  const double v = psi[col + k];
+  double n;
+  if (s.nl_model == 1) {
+    n = v * v * (s.beta - s.gamma * v);
+  } else {
+    const double v2 = v * v;
+    n = v2 * v * (s.beta - s.gamma * v2);
+  }
+  if (k == s.mid && s.mid > 0 && s.mid < s.nz - 1) n += s.pen * (psi[col + k - 1] - 2.0 * v + psi[col + k + 1]);
+  return n;
+}
 
 // Lanes from `a` on (0 <= a < 64): a crossing may be taken at the arming index itself, as in the
 // reference's loop (possible only for a negative threshold, which the host refuses).
@@ -391,11 +407,13 @@ __device__ __forceinline__ uint64_t lanes_from(int a) { return ~((1ull << a) - 1
 // Column scan of the reference's sequential loop (arm at psi > arm, then the first psi < 0):
 // one wave per column, every 64-value chunk of the column (nz <= 512: at most 8) loaded
 // before the tests, which take the chunks in scan order with ballots (lane order = scan
-// order).  The crossing index (-1: none) goes to kc_out[column]; the output kernels below
-// then evaluate each column in its own lane.  blockDim = 256: 4 columns per block.
+// order).  The crossing index k (-1: none) goes to kc_out[column] and the index the
+// derivatives are taken at to ke_out[column]: upward (surfTrack.cpp:779-783) k-1 when
+// |psi(k)| > |psi(k-1)|, downward (cosNoAdvMorphCurv.cpp:893-900) k+1 when |psi(k)| >
+// |psi(k+1)|, else k.  blockDim = 256: 4 columns per block.
 template <bool UP>
 __global__ __launch_bounds__(256) void k_surf_scan(const double* __restrict__ psi, int64_t nxy, int nz, double arm,
-                                                   int* __restrict__ kc_out) {
+                                                   int* __restrict__ kc_out, int* __restrict__ ke_out) {
   const int lane = threadIdx.x & 63;
   const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // uniform across the wave
   if (c >= nxy) return;
@@ -426,31 +444,109 @@ __global__ __launch_bounds__(256) void k_surf_scan(const double* __restrict__ ps
       }
     }
   }
-  if (lane == 0) kc_out[c] = kc;
+  if (lane == 0) {
+    int ke = kc;
+    if (kc >= 0) {
+      const int nb = UP ? kc - 1 : kc + 1;
+      if (nb >= 0 && nb < nz && fabs(colp[kc]) > fabs(colp[nb])) ke = nb;
+    }
+    kc_out[c] = kc;
+    ke_out[c] = ke;
+  }
 }
 
+// Value of a field at z index ke[c] of every column c that has a crossing, from its z
+// spectrum (layout [kz][x][y], x and y already inverted): the inverse z transform of one
+// entry per column in place of the whole field.  Neumann (REDFT01, cosNoAdvMorphCurv.cpp:750):
+//   y_n = m_0 x_0 + 2 sum_{k=1}^{N-1} m_k x_k cos(pi k (2n+1) / 2N);
+// periodic (C2R of the half spectrum, surfTrack.cpp:611-626):
+//   y_n = sum_{k=0}^{N/2} w_k Re(m_k x_k e^{2 pi i k n / N}),  w_0 = w_{N/2} = 1, else 2,
+// so that the imaginary parts at k = 0 and N/2 drop out as in an FFTW c2r.  m_k: the k_z
+// multiplier kop (KM_IKZ, KM_DZZ; < 0: 1).  The cosines come from an LDS table indexed by
+// k(2n+1) mod 4N (Neumann) or kn mod N (periodic), so every term is exact to rounding.
+// Block: 64 consecutive columns, one per lane (coalesced rows), and 4 waves that split the
+// k range; grid ceil(nxy / 64).
+template <bool NEU>
+__global__ __launch_bounds__(256) void k_surf_point(const double* __restrict__ spec, const int* __restrict__ ke,
+                                                    int64_t nxy, int nz, const double* __restrict__ kz, int kop,
+                                                    double* __restrict__ out) {
+  constexpr int kMaxN = 512;
+  __shared__ double ct[NEU ? 4 * kMaxN : kMaxN];
+  __shared__ double st[NEU ? 1 : kMaxN];
+  __shared__ double part[3][64];
+  __shared__ double kzs[kMaxN];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int period = NEU ? 4 * nz : nz;
+  for (int j = threadIdx.x; j < period; j += 256) {
+    double sv, cv;
+    sincospi(NEU ? (double)j / (2.0 * nz) : 2.0 * (double)j / nz, &sv, &cv);
+    ct[j] = cv;
+    if constexpr (!NEU) st[j] = sv;
+  }
+  if (kop >= 0)
+    for (int j = threadIdx.x; j < nz; j += 256) kzs[j] = kz[j];
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
+  const int n = (c < nxy) ? ke[c] : -1;
+  const int nk = NEU ? nz : nz / 2 + 1;
+  const int k0 = w * nk / 4, k1 = (w + 1) * nk / 4;
+  double acc = 0.0;
+  if (n >= 0) {
+    const int step = NEU ? 2 * n + 1 : n;
+    int j = (int)(((int64_t)k0 * step) % period);
+    for (int k = k0; k < k1; ++k) {
+      if constexpr (NEU) {
+        double x = spec[(int64_t)k * nxy + c];
+        if (kop == KM_DZZ) x *= -kzs[k] * kzs[k];
+        acc += ((k == 0) ? 1.0 : 2.0) * x * ct[j];
+      } else {
+        const double* e = spec + 2 * ((int64_t)k * nxy + c);
+        double re = e[0], im = e[1];
+        if (kop == KM_IKZ) {
+          const double t = re;
+          re = -kzs[k] * im;
+          im = kzs[k] * t;
+        } else if (kop == KM_DZZ) {
+          re *= -kzs[k] * kzs[k];
+          im *= -kzs[k] * kzs[k];
+        }
+        acc += ((k == 0 || 2 * k == nz) ? 1.0 : 2.0) * (re * ct[j] - im * st[j]);
+      }
+      j += step;
+      if (j >= period) j -= period;
+    }
+  }
+  if (w > 0) part[w - 1][lane] = acc;
+  __syncthreads();
+  if (w == 0 && n >= 0) out[c] = acc + part[0][lane] + part[1][lane] + part[2][lane];
+}
+
+// Field values at each column's evaluation index (k_surf_point), nxy apiece.
+enum SurfVal { SV_RATE = 0, SV_DXX = 1, SV_DYY = 2, SV_DZZ = 3, SV_GX = 3, SV_GY = 4, SV_GZ = 5, SV_HNUM = 6,
+               SV_DXY = 7 };
+
 // surfTrack.cpp:767-806 at the crossing k of k_surf_scan<true> (scan k upward, arm at
-// psi > arm, first psi < 0 after arming): derivatives taken at k-1 when |psi(k)| > |psi(k-1)|;
-// reports k.  One thread per column.
-__global__ __launch_bounds__(256) void k_surf_periodic(const double* __restrict__ psi, const double* __restrict__ dt,
-                                const double* __restrict__ gx, const double* __restrict__ gy,
-                                const double* __restrict__ gz, const double* __restrict__ hnum,
-                                const double* __restrict__ dxx, const double* __restrict__ dyy,
-                                const double* __restrict__ dxy, const int* __restrict__ kc_in, SurfParams sp,
-                                double* __restrict__ out, unsigned int* __restrict__ nfound) {
+// psi > arm, first psi < 0 after arming), with the derivatives at its evaluation index
+// (val: rate, psi_xx, psi_yy, psi_x, psi_y, psi_z, psi_xx + psi_yy, psi_xy); reports k.
+// One thread per column.
+__global__ __launch_bounds__(256) void k_surf_periodic(const double* __restrict__ psi, const double* __restrict__ val,
+                                                       const int* __restrict__ kc_in, const int* __restrict__ ke_in,
+                                                       SurfParams sp, double* __restrict__ out,
+                                                       unsigned int* __restrict__ nfound) {
   const int64_t nxy = (int64_t)sp.nx * sp.ny;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
   const int kc = kc_in[c];
   if (kc < 0) return;
-  const int64_t col = c * sp.nz;
-  int64_t id = col + kc;
-  if (kc > 0 && fabs(psi[id]) > fabs(psi[id - 1])) id -= 1;
-  const double g = sqrt(gx[id] * gx[id] + gy[id] * gy[id] + gz[id] * gz[id]);
+  const int ke = ke_in[c];
+  auto V = [&](int f) { return val[f * nxy + c]; };
+  const double gx = V(SV_GX), gy = V(SV_GY), gz = V(SV_GZ), dxx = V(SV_DXX), dyy = V(SV_DYY), dxy = V(SV_DXY);
+  const double g = sqrt(gx * gx + gy * gy + gz * gz);
+  const double rate = V(SV_RATE) + (sp.add_nl ? surf_nl(psi, sp, c * sp.nz, ke) : 0.0);
   out[c] = (double)kc;
-  out[nxy + c] = sp.vsign * dt[id] / g;
-  out[2 * nxy + c] = hnum[id] / g;
-  out[3 * nxy + c] = (dxx[id] * dyy[id] - dxy[id] * dxy[id]) / (g * g);
+  out[nxy + c] = sp.vsign * rate / g;
+  out[2 * nxy + c] = V(SV_HNUM) / g;
+  out[3 * nxy + c] = (dxx * dyy - dxy * dxy) / (g * g);
   atomicAdd(nfound, 1u);
 }
 
@@ -476,13 +572,14 @@ __device__ __forceinline__ double fd_z(const double* p, const SurfParams& s, int
 }
 
 // cosNoAdvMorphCurv.cpp:877-966 at the crossing k of k_surf_scan<false> (scan k downward,
-// arm at psi > arm, first psi < 0): index moved to k+1 when |psi(k)| > |psi(k+1)|; Megrabov
-// H = div(grad psi/|grad psi|) and Gaussian curvature K from the FD gradient and the spectral
-// second derivatives.  One thread per column.
-__global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__ psi, const double* __restrict__ dt,
-                               const double* __restrict__ dxx, const double* __restrict__ dyy,
-                               const double* __restrict__ dzz, const int* __restrict__ kc_in, SurfParams sp,
-                               double* __restrict__ out, unsigned int* __restrict__ nfound) {
+// arm at psi > arm, first psi < 0), evaluated at its index k2 (k+1 when |psi(k)| >
+// |psi(k+1)|): Megrabov H = div(grad psi/|grad psi|) and Gaussian curvature K from the FD
+// gradient and the spectral second derivatives (val: rate, psi_xx, psi_yy, psi_zz at k2).
+// One thread per column.
+__global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__ psi, const double* __restrict__ val,
+                                                      const int* __restrict__ kc_in, const int* __restrict__ ke_in,
+                                                      SurfParams sp, double* __restrict__ out,
+                                                      unsigned int* __restrict__ nfound) {
   const int64_t nxy = (int64_t)sp.nx * sp.ny;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
@@ -490,10 +587,7 @@ __global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__
   if (kc < 0) return;
   const int i = (int)(c / sp.ny), j = (int)(c - (int64_t)i * sp.ny);
   const int64_t col = c * sp.nz;
-  const int k = kc;
-  int k2 = k;
-  if (k + 1 < sp.nz && fabs(psi[col + k]) > fabs(psi[col + k + 1])) k2 = k + 1;
-  const int64_t id = col + k2;
+  const int k = kc, k2 = ke_in[c];
   const double px = fd_x(psi, sp, i, j, k2), py = fd_y(psi, sp, i, j, k2), pz = fd_z(psi, sp, i, j, k2);
   const double g = sqrt(px * px + py * py + pz * pz);
   double d_xy;
@@ -513,7 +607,7 @@ __global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__
     d_xz = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
     d_yz = 2.0 * (fd_y(psi, sp, i, j, k2) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
   }
-  const double fxx = dxx[id], fyy = dyy[id], fzz = dzz[id];
+  const double fxx = val[SV_DXX * nxy + c], fyy = val[SV_DYY * nxy + c], fzz = val[SV_DZZ * nxy + c];
   const double Hc = ((py * py + pz * pz) * fxx + (px * px + pz * pz) * fyy + (px * px + py * py) * fzz -
                      2.0 * (px * py * d_xy + px * pz * d_xz + py * pz * d_yz)) /
                     (g * g * g);
@@ -522,8 +616,9 @@ __global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__
                      2.0 * (py * d_xy * (pz * d_xz - px * fzz) + px * d_xz * (py * d_yz - pz * fyy) +
                             pz * d_yz * (px * d_xy - py * fxx))) /
                     (g * g * g * g);
+  const double rate = val[SV_RATE * nxy + c] + (sp.add_nl ? surf_nl(psi, sp, col, k2) : 0.0);
   out[c] = (double)k2;
-  out[nxy + c] = sp.vsign * dt[id] / g;
+  out[nxy + c] = sp.vsign * rate / g;
   out[2 * nxy + c] = Hc;
   out[3 * nxy + c] = Kc;
   atomicAdd(nfound, 1u);

@@ -82,11 +82,12 @@ hipError_t sched_dispatch(const PassSched& s, F&& f) {
 
 template <typename T>
 struct PeriodicOps {
-  // y pass, in place allowed.  sign -1 forward, +1 inverse.  dry: only report whether the
+  // y pass, in place allowed.  sign -1 forward, +1 inverse.  kmul: k_y table of a multiplier
+  // applied to the input (kop KM_IKY: i k_y, KM_DYY: -k_y^2).  dry: only report whether the
   // schedule is valid for this length (no launch).
   static hipError_t ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
                           const cplx<T>* tw, hipStream_t st, const T* kmul = nullptr, PassSched s = PassSched{},
-                          bool dry = false);
+                          bool dry = false, int kop = KM_IKY);
   // x pass (mode: XMode), tiles over (y, kz).  Only X_STEP takes a non-default schedule.
   static hipError_t xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx<T>* rbuf, const cplx<T>* psi0,
                           const PParams<T>& p, hipStream_t st, PassSched s = PassSched{}, bool dry = false);
@@ -100,19 +101,21 @@ struct PeriodicOps {
 
 template <typename T>
 hipError_t PeriodicOps<T>::ypass(const cplx<T>* in, cplx<T>* out, int ny, int64_t npencils, int sign,
-                                 const cplx<T>* tw, hipStream_t st, const T* kmul, PassSched s, bool dry) {
+                                 const cplx<T>* tw, hipStream_t st, const T* kmul, PassSched s, bool dry,
+                                 int kop) {
+  if (kmul != nullptr && kop != KM_IKY && kop != KM_DYY) return hipErrorInvalidValue;
 #define SMX_CASE(L)                                                                                      \
   if (ny == L)                                                                                           \
     return sched_dispatch<L>(s, [&](auto nc, auto v) -> hipError_t {                                     \
       constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
       constexpr int NT = NC * (L / PlanV<L, V>::E);                                                      \
-      if constexpr (NT > 1024 || (NC * ColAddr<L>::LP + L) * sizeof(cplx<T>) > kMaxLds) {                      \
+      if constexpr (NT > 1024 || (NC * ColAddr<L>::LP + L) * sizeof(cplx<T>) + L * sizeof(T) > kMaxLds) {    \
         return hipErrorInvalidValue;                                                                     \
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \
         const dim3 grid((unsigned)((npencils + NC - 1) / NC));                                           \
-        if (sign < 0) SMX_LAUNCH((k_ypass<T, L, -1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul); \
-        else SMX_LAUNCH((k_ypass<T, L, +1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul);          \
+        if (sign < 0) SMX_LAUNCH((k_ypass<T, L, -1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul, kop); \
+        else SMX_LAUNCH((k_ypass<T, L, +1, NC, V>), grid, dim3(NT), 0, st, in, out, npencils, tw, kmul, kop);          \
         return hipGetLastError();                                                                        \
       }                                                                                                  \
     });
@@ -262,6 +265,7 @@ hipError_t NeumannOps<T>::ypass(const T* in, T* out, int ny, int64_t npencils, i
   if (nc != NCD && !(nc == NCA && par == DP_COS)) return hipErrorInvalidValue;
   if (npencils % nc != 0) return hipErrorInvalidValue;  // whole tiles of pencil pairs
   if (par == DP_DSIN && (dir < 0 || kq == nullptr)) return hipErrorInvalidValue;
+  if (par == DP_COS && dir < 0 && kq != nullptr) return hipErrorInvalidValue;  // -k^2 is an inverse input
 #define SMX_CASE(L)                                                                                   \
   if (ny == L) {                                                                                      \
     if (dry) return hipSuccess;                                                                       \

@@ -69,7 +69,8 @@ __device__ __forceinline__ cplx<T> dct3_par(cplx<T> rk, cplx<T> rm, int k, int L
 // y pass: contiguous real pencils of length L = ny; pencils g = kz*nx + x, pairs of adjacent
 // pencils.  Lanes run along the pencils (tc fastest).  DIR -1: DCT-II, +1: DCT-III.
 // The grid covers whole tiles (nx*nz is a multiple of NC; the launcher checks it).
-// PAR: parity of the pencils along y (DP_DSIN: DIR +1 only, kq = k_y).
+// PAR: parity of the pencils along y (DP_DSIN: DIR +1 only, kq = k_y).  DP_COS with DIR +1 and
+// kq = k_y multiplies the cosine coefficients by -k_y^2 (the second derivative).
 template <typename T, int L, int DIR, int NC, int PAR = DP_COS>
 __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const T* in, T* out,
                                                                            const cplx<T>* __restrict__ tw,
@@ -119,6 +120,15 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
     for (int e = 0; e < E; ++e) {
       const int k = TL::template pos_in<inv_first<L>()>(tc, e);
       v[e] = mk(a[k], b[k]);
+    }
+    if constexpr (PAR == DP_COS) {
+      if (kq != nullptr) {  // -k_y^2, after every load is issued
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const T k = kq[TL::template pos_in<inv_first<L>()>(tc, e)];
+          v[e] = (-k * k) * v[e];
+        }
+      }
     }
     TL::template store<inv_first<L>()>(v, S, q, tc);
     __syncthreads();
@@ -170,11 +180,27 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
     nsum = block_sum(acc, red);
   }
   if constexpr (MODE == X_INV) {
+    // each entry is loaded once: a thread's own slots k stay in registers, the partners L-k
+    // come from the tile
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      v[e] = ld_off(src, (uint32_t)Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e) * rowb + q * CB);
+    if (p.kop >= 0) {  // the k-space multiplier, columns a and b of the pair (after every load is issued)
+      const T kya = p.ky[y0 + 2 * q], kyb = p.ky[y0 + 2 * q + 1], kzv = p.kz[kz];
+      __syncthreads();  // the KX fill
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const T qx = KX[Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e)];
+        v[e] = mk(kfactor(p, p.kop, qx, kya, kzv) * v[e].x, kfactor(p, p.kop, qx, kyb, kzv) * v[e].y);
+      }
+    }
+    Tile<T, L, E, FP>::template store<inv_first<L>()>(v, F, q, tc);
+    __syncthreads();  // also orders the QW fill before its reads
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
-      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : ld_off(src, (uint32_t)(L - k) * rowb + q * CB);
-      v[e] = dct3_in(ld_off(src, (uint32_t)k * rowb + q * CB), pn, qtw[k]);
+      const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : F[(L - k) * FP + q];
+      v[e] = dct3_in(v[e], pn, QW[k]);
     }
   } else {
 #pragma unroll
@@ -321,6 +347,15 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
 #pragma unroll
     for (int e = 0; e < E; ++e)
       v[e] = ld_off(src, (uint32_t)Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e) * rowb + q * CB);
+    if constexpr (OUT == Z_OUT_REAL) {
+      if (p.kop >= 0) {  // multiplier of k_z: both columns of the pair share it
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
+          v[e] = kfactor(p, p.kop, (T)0, (T)0, p.kz[k]) * v[e];
+        }
+      }
+    }
     Tile<T, L, E, FP>::template store<inv_first<L>()>(v, F, q, tc);
     __syncthreads();  // also orders the QW fill before its reads
 #pragma unroll

@@ -44,6 +44,9 @@ struct PParams {
   // Multiplier of the z pass in the Z_OP_MUL form (the adjoint sweep: the recorded N'(psi_n)),
   // layout [z][x][y].
   const T* mul_np = nullptr;
+  // k-space multiplier (KMode, < 0: none) applied to the spectral input of the inverse x pass
+  // (X_INV: any mode) and of the spectrum-to-real z pass (modes of k_z alone) as it is loaded
+  int kop = -1;
 };
 
 // dN/dpsi: 3 beta psi^2 - 5 gamma psi^4 (quintic) or 2 beta psi - 3 gamma psi^2 (cubic).
@@ -70,13 +73,39 @@ __device__ __forceinline__ T nonlin(const PParams<T>& p, T s) {
   return (p.nl_model == 1) ? nonlin3(p, s) : nonlin5(p, s);
 }
 
+// Factor of the k-space multiplier `op` at (qx, qy, qz): the multiplier itself, or its
+// imaginary part for the first derivatives (i q).
+template <typename T>
+__device__ __forceinline__ T kfactor(const PParams<T>& p, int op, T qx, T qy, T qz) {
+  switch (op) {
+    case KM_ALIN: {
+      const T d = qx * qx + qy * qy + qz * qz - p.q02;
+      return p.eps - p.alpha * d * d;
+    }
+    case KM_IKX: return qx;
+    case KM_IKY: return qy;
+    case KM_IKZ: return qz;
+    case KM_DXX: return -qx * qx;
+    case KM_DYY: return -qy * qy;
+    case KM_DZZ: return -qz * qz;
+    case KM_DXY: return -qx * qy;
+    default: return -(qx * qx) - qy * qy;  // KM_LAP_XY
+  }
+}
+// The multiplier `op` applied to one complex coefficient.
+template <typename T>
+__device__ __forceinline__ cplx<T> kapply(const PParams<T>& p, int op, T qx, T qy, T qz, cplx<T> a) {
+  const T m = kfactor(p, op, qx, qy, qz);
+  return (op == KM_IKX || op == KM_IKY || op == KM_IKZ) ? mk(-m * a.y, m * a.x) : m * a;
+}
+
 // ---------------------------------------------------------------------------
 // y pass: contiguous pencils of length L = ny; pencils g = kz*nx + x.
 template <typename T, int L, int SIGN, int NC, int V = 0>
 __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>* in, cplx<T>* out,
                                                                int64_t npencils,
                                                                const cplx<T>* __restrict__ tw,
-                                                               const T* __restrict__ kmul) {
+                                                               const T* __restrict__ kmul, int kop) {
   // lanes run along each pencil (tc fastest), so the first-pass loads and last-pass
   // stores go straight between HBM and registers (16 contiguous complex per lane group)
   constexpr int E = PlanV<L, V>::E, TPC = L / E;
@@ -89,6 +118,13 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
   const int64_t pen = (int64_t)blockIdx.x * NC + col;
   const bool valid = pen < npencils;
   const cplx<T>* src = in + pen * L;
+  // LDS copy of the multiplier's k_y (global loads behind the row loads would double the
+  // vector memory instructions)
+  __shared__ T KM[L];
+  if (kmul != nullptr) {
+    for (int i = tid; i < L; i += NC * TPC) KM[i] = kmul[i];
+    __syncthreads();
+  }
   cplx<T> v[E];
   using FFT = BlockFFTA<T, L, E, AD, SIGN,
                         typename std::conditional<(SIGN < 0), typename PlanV<L, V>::Fwd, typename PlanV<L, V>::Inv>::type>;
@@ -96,11 +132,11 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
   constexpr int RL = (SIGN < 0) ? fwd_last<L, V>() : inv_last<L, V>();
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = valid ? src[TL::template pos_in<RF>(tc, e)] : mk<T>(0, 0);
-  if (kmul != nullptr) {  // multiply by i k_y before the transform (spectral d/dy, pf3d.cpp:405)
+  if (kmul != nullptr) {  // multiply by i k_y (spectral d/dy, pf3d.cpp:405) or by -k_y^2 (kop KM_DYY)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const T k = kmul[TL::template pos_in<RF>(tc, e)];
-      v[e] = mk(-k * v[e].y, k * v[e].x);
+      const T k = KM[TL::template pos_in<RF>(tc, e)];
+      v[e] = (kop == KM_DYY) ? (-k * k) * v[e] : mk(-k * v[e].y, k * v[e].x);
     }
   }
   FFT::run(v, S, col, tc, twl);
@@ -137,10 +173,15 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
-  __shared__ T KX[(MODE == X_STEP || MODE == X_BOOT) ? L : 1];
+  __shared__ T KX[(MODE == X_STEP || MODE == X_BOOT || MODE == X_INV) ? L : 1];
   if constexpr (MODE == X_STEP || MODE == X_BOOT) {
     for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
     __syncthreads();  // a one-pass plan has no barrier of its own before the update reads KX
+  } else if constexpr (MODE == X_INV) {
+    if (p.kop >= 0) {  // k_x of the multiplier
+      for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
+      __syncthreads();
+    }
   }
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
@@ -175,6 +216,14 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
 #pragma unroll
     for (int e = 0; e < E; ++e)
       v[e] = ld_off(src, (uint32_t)Tile<T, L, E, NCP>::template pos_in<R0>(tc, e) * rowb + cofs);
+    if constexpr (MODE == X_INV) {
+      if (p.kop >= 0) {
+        const T kyv = p.ky[ky], kzv = p.kz[kz];
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          v[e] = kapply(p, p.kop, KX[Tile<T, L, E, NCP>::template pos_in<R0>(tc, e)], kyv, kzv, v[e]);
+      }
+    }
   }
   // FP64 mean mode (mixed path): the block owning k = 0 sums the z-pass partials.
   double nsum = 0.0;
@@ -322,6 +371,15 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     // straddle L/2.
     const cplx2<T>* col = reinterpret_cast<const cplx2<T>*>(in + f0);
     const uint32_t tcr = (uint32_t)tc * rowb + qofs;
+    // k_z of a multiplier from an LDS copy (a global load in the loop below would wait
+    // behind the row loads)
+    constexpr bool kKZ = (OUT == Z_OUT_REAL);
+    __shared__ T KZ[kKZ ? L / 2 + 1 : 1];
+    const bool kzop = kKZ && p.kop >= 0;
+    if (kzop) {
+      for (int i = tid; i <= L / 2; i += NT) KZ[i] = p.kz[i];
+      __syncthreads();
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int c = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(0, e);
@@ -330,6 +388,13 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
       const uint32_t off = lo ? tcr + (uint32_t)c * rowb : (uint32_t)(L - c) * rowb - tcr + 2 * qofs;
       const cplx2<T> ab = ld_off(col, off);
       cplx<T> A = ab.a, B = ab.b;
+      if constexpr (OUT == Z_OUT_REAL) {
+        if (kzop) {  // multiplier of k_z at the stored row min(n, L - n)
+          const T kzv = KZ[lo ? n : L - n];
+          A = kapply(p, p.kop, (T)0, (T)0, kzv, A);
+          B = kapply(p, p.kop, (T)0, (T)0, kzv, B);
+        }
+      }
       if (n == 0 || n == L / 2) {
         A.y = 0;
         B.y = 0;

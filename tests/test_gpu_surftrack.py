@@ -74,3 +74,30 @@ def test_surftrack_columns_without_crossing_keep_previous_values():
         (z2, v2, h2, k2), f2 = s.surftrack(0.7)
         assert f2 == 0
         np.testing.assert_array_equal(z2, z1)  # persistent values, as the reference's surf arrays
+
+
+@pytest.mark.parametrize("bc,model,beta,w_mid,mid", [
+    (A.SMX_BC_PERIODIC, A.SMX_NL_CUBIC, 0.3, 0.0, -1),
+    (A.SMX_BC_PERIODIC, A.SMX_NL_QUINTIC, 0.5, 0.3, 14),
+    (A.SMX_BC_NEUMANN, A.SMX_NL_QUINTIC, 0.5, 0.0, -1),
+])
+def test_surftrack_rate_with_nonlinearity(bc, model, beta, w_mid, mid):
+    """The velocity's dpsi/dt = F^-1[aLin psi_hat] + N(psi) with N taken at the crossing: both
+    models with beta != 0, and the mid-plane penalty of N on the plane most columns cross."""
+    nx, ny, nz = 32, 32, 48
+    dx = 2 * math.pi / 8
+    cfg = A.make_config(nx, ny, nz, bc=bc, nw=8, eps=0.5, beta=beta, gamma=1.0, dt=0.05, nl_model=model)
+    cfg.w_mid, cfg.mid_plane = w_mid, mid
+    psi = O.ic_hyperboloid(nx, ny, nz, dx, dx, dx, 1.0, 0.9, 1e-3, 0)
+    orc = O.Oracle(cfg)
+    orc.set_psi(psi)
+    orc.step(40)
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        s.step(40)
+        say("surftrack rate", bc, model, beta, w_mid, mid)
+        arm, vsign = (0.1, 1.0) if bc == A.SMX_BC_PERIODIC else (0.7, -1.0)
+        (gz, _, _, _), _ = s.surftrack(arm, vsign)
+        if mid > 0:  # the evaluation index is the crossing or the one below it
+            assert np.count_nonzero((gz == mid) | (gz == mid + 1)) >= 8
+        _compare(s, orc, arm, vsign)
