@@ -529,12 +529,20 @@ static int step_once(smx_handle* h, const PParams<T>& p, hipEvent_t* ev) {
 
 // ---- FP64 diagnostics on a copy of the state ----
 // FP64 copy of a stage buffer of the working precision into dst.
+// A periodic FP32 stage is also projected onto the stage of the real field it stands for
+// (k_herm_stage), so that FP64 derivatives of the copy are those of the field smx_get_psi returns.
 static int stage_to_f64(smx_handle* h, const void* stage, void* dst) {
   const int64_t ncomp = h->neumann ? h->nspec : 2 * h->nspec;  // real components
   if (h->fp32) {
     hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, h->st, (const float*)stage,
                        (double*)dst, ncomp);
     HIPCHK(hipGetLastError());
+    if (!h->neumann) {
+      const int64_t pairs = 2 * (int64_t)h->cfg.nx * (h->cfg.ny / 2 + 1);
+      hipLaunchKernelGGL(k_herm_stage, dim3((unsigned)std::min<int64_t>((pairs + 255) / 256, 4096)), dim3(256), 0,
+                         h->st, (cplx<double>*)dst, h->cfg.nx, h->cfg.ny, h->cfg.nz);
+      HIPCHK(hipGetLastError());
+    }
   } else {
     HIPCHK(hipMemcpyAsync(dst, stage, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, h->st));
   }

@@ -200,6 +200,28 @@ __global__ void k_zero_f64(double* __restrict__ a, int64_t n) {
     a[i] = 0.0;
 }
 
+// Hermitian part of the kz = 0 and kz = nz/2 planes of a periodic x stage ([kz][x][ky],
+// complex): G(x, ky) <- (G(x, ky) + conj G(x, -ky)) / 2, the stage of the real field that its
+// C2R gives.  An FP32 state carries anti-Hermitian rounding noise on those planes; the C2R drops
+// it, but a first derivative at the Nyquist row (k = +pi/d there, pf3d.cpp:162-190) would turn
+// it into a real contribution.  One thread per pair (ky, -ky).
+__global__ void k_herm_stage(cplx<double>* __restrict__ W, int nx, int ny, int nz) {
+  const int half = ny / 2;
+  const int64_t per_plane = (int64_t)nx * (half + 1);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * per_plane;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int pl = (int)(i / per_plane);
+    const int64_t r = i - pl * per_plane;
+    const int x = (int)(r / (half + 1)), ky = (int)(r - (int64_t)x * (half + 1));
+    cplx<double>* row = W + ((int64_t)(pl == 0 ? 0 : nz / 2) * nx + x) * ny;
+    const int km = (ny - ky) % ny;
+    const cplx<double> a = row[ky], b = row[km];
+    const double re = 0.5 * (a.x + b.x), im = 0.5 * (a.y - b.y);
+    row[ky] = mk(re, im);
+    row[km] = mk(re, -im);
+  }
+}
+
 // The persistent surfTrack outputs (n doubles) and the crossing count into a mapped pinned host
 // buffer (count after them, as a double): written by the kernel in stream order, so the host
 // reads them once the stream is idle, without a runtime copy.

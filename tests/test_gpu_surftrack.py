@@ -101,3 +101,25 @@ def test_surftrack_rate_with_nonlinearity(bc, model, beta, w_mid, mid):
         if mid > 0:  # the evaluation index is the crossing or the one below it
             assert np.count_nonzero((gz == mid) | (gz == mid + 1)) >= 8
         _compare(s, orc, arm, vsign)
+
+
+@pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
+@pytest.mark.parametrize("bc", [A.SMX_BC_PERIODIC, A.SMX_BC_NEUMANN])
+@pytest.mark.parametrize("shape", [(32, 48, 24), (48, 32, 64), (16, 16, 96)])
+def test_surftrack_shapes_and_precisions(shape, bc, prec):
+    """Every axis length role (the point evaluation splits nz/2+1 or nz coefficients over four
+    waves unevenly here) and both working precisions: the oracle evaluates the GPU's own
+    current field (get_psi), so an FP32-mixed state is compared on identical data."""
+    nx, ny, nz = shape
+    dx = 2 * math.pi / 8
+    cfg = A.make_config(nx, ny, nz, bc=bc, nw=8, eps=0.5, beta=0.3, gamma=1.0, dt=0.05,
+                        nl_model=A.SMX_NL_CUBIC, precision=prec)
+    psi = O.ic_hyperboloid(nx, ny, nz, dx, dx, dx, 1.0, 0.9, 1e-3, 0)
+    arm = 0.1 if bc == A.SMX_BC_PERIODIC else 0.7
+    say("surftrack", shape, bc, prec)
+    with Solver(cfg) as s:
+        s.set_psi(psi)
+        s.step(30)
+        orc = O.Oracle(cfg)
+        orc.set_psi(s.get_psi())
+        _compare(s, orc, arm, 1.0)
