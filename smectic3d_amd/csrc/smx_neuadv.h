@@ -106,7 +106,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_force(cons
   constexpr int E = Z::E;
   __shared__ cplx<T> F[L * Z::FP];
   const PParams<T>& p = ap.p;
-  const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
+  const int tid0 = threadIdx.x, q0 = tid0 % Z::NPR, tc0 = tid0 / Z::NPR;
+  const int tid = tid0;
   SMX_DZ_TABLES(Z::NT)
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
@@ -119,6 +120,10 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_force(cons
 #pragma unroll 1
   for (int i = 0; i < 5; ++i) {
     if (i == 1 && !mu) continue;
+    // per-iteration copies of the lane indices: the address arithmetic of the transforms then
+    // stays inside the loop instead of being hoisted into registers live across it
+    int q = q0, tc = tc0, tid = tid0;  // shadows the function-scope tid
+    asm volatile("" : "+v"(q), "+v"(tc), "+v"(tid));
     if (i == 0) Z::stage_form(bs, f0, rowb, q, tc, DP_COS, KZ, QW, F, w);
     else if (i == 1) Z::template stage_form<true>(bpsi, f0, rowb, q, tc, DP_COS, KZ, QW, F, w, g);
     else if (i == 2 && mu) {
@@ -154,7 +159,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
   __shared__ cplx<T> F[L * Z::FP];
   __shared__ double red[64];
   const PParams<T>& p = ap.p;
-  const int tid = threadIdx.x, q = tid % Z::NPR, tc = tid / Z::NPR;
+  const int tid0 = threadIdx.x, q0 = tid0 % Z::NPR, tc0 = tid0 / Z::NPR;
+  const int tid = tid0;
   SMX_DZ_TABLES(Z::NT)
   const int64_t f0 = (int64_t)blockIdx.x * NC;
   const uint32_t rowb = (uint32_t)p.nxy * sizeof(T);
@@ -165,6 +171,8 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
   cplx<T> w[E], aux[E], nacc[E];
 #pragma unroll 1
   for (int i = 0; i < 7; ++i) {
+    int q = q0, tc = tc0;  // per-iteration lane indices (as in k_dctz_force)
+    asm volatile("" : "+v"(q), "+v"(tc));
     if (i == 0) Z::template stage_form<true>(bpsi, f0, rowb, q, tc, DP_COS, KZ, QW, F, w, aux);
     else if (i == 1) {
 #pragma unroll
@@ -185,7 +193,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dctz_advnl(cons
     }
   }
   if (p.partial != nullptr) wave_partial<E, Z::NT>(nacc, red, tid);
-  Z::r2c_store(nacc, nout, DP_COS, f0, rowb, q, tc, tid, QW, F, twl);  // its barriers order red[]
+  Z::r2c_store(nacc, nout, DP_COS, f0, rowb, q0, tc0, tid, QW, F, twl);  // its barriers order red[]
   if (p.partial != nullptr) store_partial<Z::NT>(red, p.partial + blockIdx.x, tid);
 }
 
