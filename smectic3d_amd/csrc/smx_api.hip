@@ -304,9 +304,17 @@ static int ypass(smx_handle* h, const void* in, void* out, int dir, PassSched s 
   return SMX_OK;
 }
 
+// A k-space multiplier p.kop on a cosine series must be real: the first derivatives i k are
+// periodic only (their Neumann counterparts change the parity, DP_DSIN).
+static bool kop_valid(const smx_handle* h, int kop) {
+  return kop < 0 || !h->neumann || (kop != KM_IKX && kop != KM_IKY && kop != KM_IKZ);
+}
+
 template <typename T>
 static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, void* out, void* r, const void* psi0,
                  PassSched s = PassSched{}, bool neu = false) {
+  if (!kop_valid(h, p.kop) || (p.kop >= 0 && mode != X_INV))
+    return fail(SMX_ERR_INVALID, "x pass: unsupported multiplier");
   hipError_t e = h->neumann ? NeumannOps<T>::xpass(mode, (const T*)in, (T*)out, (T*)r, (const T*)psi0, p,
                                                    tables<T>(h).qtwx, h->st, neu ? s.nc : 0)
                             : PeriodicOps<T>::xpass(mode, (const cplx<T>*)in, (cplx<T>*)out, (cplx<T>*)r,
@@ -318,6 +326,7 @@ static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, v
 template <typename T>
 static int zpass(smx_handle* h, const PParams<T>& p, int in_mode, int out_mode, int op, const void* in, void* out,
                  const double* real_in, double* real_out, PassSched s = PassSched{}, bool neu = false) {
+  if (p.kop >= 0) return fail(SMX_ERR_INVALID, "z pass: no multiplier (surfTrack evaluates k_z ones per point)");
   hipError_t e = h->neumann ? NeumannOps<T>::zpass(in_mode, out_mode, op, (const T*)in, (T*)out, real_in, real_out,
                                                    p, tables<T>(h).qtwz, h->st, neu ? s.nc : 0)
                             : PeriodicOps<T>::zpass(in_mode, out_mode, op, (const cplx<T>*)in, (cplx<T>*)out,

@@ -347,15 +347,6 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zpass(const
 #pragma unroll
     for (int e = 0; e < E; ++e)
       v[e] = ld_off(src, (uint32_t)Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e) * rowb + q * CB);
-    if constexpr (OUT == Z_OUT_REAL) {
-      if (p.kop >= 0) {  // multiplier of k_z: both columns of the pair share it
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
-          v[e] = kfactor(p, p.kop, (T)0, (T)0, p.kz[k]) * v[e];
-        }
-      }
-    }
     Tile<T, L, E, FP>::template store<inv_first<L>()>(v, F, q, tc);
     __syncthreads();  // also orders the QW fill before its reads
 #pragma unroll

@@ -45,7 +45,7 @@ struct PParams {
   // layout [z][x][y].
   const T* mul_np = nullptr;
   // k-space multiplier (KMode, < 0: none) applied to the spectral input of the inverse x pass
-  // (X_INV: any mode) and of the spectrum-to-real z pass (modes of k_z alone) as it is loaded
+  // (X_INV) as it is loaded
   int kop = -1;
 };
 
@@ -371,15 +371,6 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
     // straddle L/2.
     const cplx2<T>* col = reinterpret_cast<const cplx2<T>*>(in + f0);
     const uint32_t tcr = (uint32_t)tc * rowb + qofs;
-    // k_z of a multiplier from an LDS copy (a global load in the loop below would wait
-    // behind the row loads)
-    constexpr bool kKZ = (OUT == Z_OUT_REAL);
-    __shared__ T KZ[kKZ ? L / 2 + 1 : 1];
-    const bool kzop = kKZ && p.kop >= 0;
-    if (kzop) {
-      for (int i = tid; i <= L / 2; i += NT) KZ[i] = p.kz[i];
-      __syncthreads();
-    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int c = Tile<T, L, E, FP>::template pos_in<inv_first<L, V>()>(0, e);
@@ -388,13 +379,6 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
       const uint32_t off = lo ? tcr + (uint32_t)c * rowb : (uint32_t)(L - c) * rowb - tcr + 2 * qofs;
       const cplx2<T> ab = ld_off(col, off);
       cplx<T> A = ab.a, B = ab.b;
-      if constexpr (OUT == Z_OUT_REAL) {
-        if (kzop) {  // multiplier of k_z at the stored row min(n, L - n)
-          const T kzv = KZ[lo ? n : L - n];
-          A = kapply(p, p.kop, (T)0, (T)0, kzv, A);
-          B = kapply(p, p.kop, (T)0, (T)0, kzv, B);
-        }
-      }
       if (n == 0 || n == L / 2) {
         A.y = 0;
         B.y = 0;
