@@ -1016,6 +1016,11 @@ struct smx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t graph_steps = 0;
+  // boxes that share precision, grid and tile schedules (periodic, no advection) advance in
+  // the graph with one grouped launch per pass: member indices, and each group's GrpArgs
+  // array in device memory (built at capture, freed with the graph)
+  std::vector<std::vector<int>> groups;
+  std::vector<void*> grp_args;
   // captured adjoint sweeps of every box, kept while every box keeps its workspace
   hipGraph_t adj_graph = nullptr;
   hipGraphExec_t adj_exec = nullptr;
@@ -1044,6 +1049,75 @@ static int box_steps(smx_handle* h, int64_t n) {
   return SMX_OK;
 }
 
+static void batch_free_groups(smx_batch* b) {
+  for (void* d : b->grp_args)
+    if (d) (void)hipFree(d);
+  b->grp_args.clear();
+  b->groups.clear();
+}
+
+static bool same_sched(const smx_handle* a, const smx_handle* b) {
+  for (int s = 0; s < NSLOTS; ++s)
+    if (a->sched[s].nc != b->sched[s].nc || a->sched[s].v != b->sched[s].v) return false;
+  return true;
+}
+
+// Boxes that can share the grouped step kernels: periodic, no advection, same precision,
+// grid and schedules.  Groups of one box keep the single-box kernels.
+static void batch_make_groups(smx_batch* b, std::vector<int>& single) {
+  const int n = (int)b->boxes.size();
+  std::vector<bool> used(n, false);
+  for (int i = 0; i < n; ++i) {
+    if (used[i]) continue;
+    const smx_handle* h = b->boxes[i];
+    std::vector<int> g{i};
+    used[i] = true;
+    if (!h->neumann && !h->cfg.advect)
+      for (int j = i + 1; j < n; ++j) {
+        const smx_handle* o = b->boxes[j];
+        if (used[j] || o->neumann || o->cfg.advect || o->fp32 != h->fp32 || o->cfg.nx != h->cfg.nx ||
+            o->cfg.ny != h->cfg.ny || o->cfg.nz != h->cfg.nz || !same_sched(o, h))
+          continue;
+        g.push_back(j);
+        used[j] = true;
+      }
+    if (g.size() > 1) b->groups.push_back(g);
+    else single.push_back(i);
+  }
+}
+
+// The GrpArgs array of a group in device memory (the parameters step_once gives each box).
+template <typename T>
+static int group_args(smx_batch* b, const std::vector<int>& members, void** out) {
+  std::vector<GrpArgs<T>> host(members.size());
+  for (size_t i = 0; i < members.size(); ++i) {
+    smx_handle* h = b->boxes[members[i]];
+    PParams<T> p = params<T>(h, h->fp32);
+    p.npartial = PeriodicOps<T>::zpass_blocks(p, h->sched[SLOT_Z].nc);
+    host[i].a = (cplx<T>*)h->A;
+    host[i].r = (cplx<T>*)h->R;
+    host[i].twy = tables<T>(h).twy;
+    host[i].p = p;
+  }
+  const size_t bytes = sizeof(GrpArgs<T>) * host.size();
+  HIPCHK(hipMalloc(out, bytes));
+  HIPCHK(hipMemcpy(*out, host.data(), bytes, hipMemcpyHostToDevice));
+  return SMX_OK;
+}
+
+template <typename T>
+static int group_steps(smx_batch* b, const std::vector<int>& members, const void* args, int64_t n,
+                       hipStream_t st) {
+  smx_handle* h = b->boxes[members[0]];
+  const PParams<T> p = params<T>(h, h->fp32);
+  for (int64_t i = 0; i < n; ++i) {
+    hipError_t e = PeriodicOps<T>::step_grp((const GrpArgs<T>*)args, (int)members.size(), p, h->npencils,
+                                            h->sched, st);
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("grouped step: ") + hipGetErrorString(e));
+  }
+  return SMX_OK;
+}
+
 static int batch_capture(smx_batch* b, int64_t gsteps) {
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
@@ -1053,10 +1127,31 @@ static int batch_capture(smx_batch* b, int64_t gsteps) {
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
+  batch_free_groups(b);
+  std::vector<int> single;
+  batch_make_groups(b, single);
+  b->grp_args.assign(b->groups.size(), nullptr);
+  for (size_t k = 0; k < b->groups.size(); ++k) {
+    const smx_handle* h = b->boxes[b->groups[k][0]];
+    CHK(h->fp32 ? group_args<float>(b, b->groups[k], &b->grp_args[k])
+                : group_args<double>(b, b->groups[k], &b->grp_args[k]));
+  }
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
   int rc = SMX_OK;
   hipError_t e = hipEventRecord(b->fork, b->st);
-  for (size_t i = 0; i < b->boxes.size() && rc == SMX_OK && e == hipSuccess; ++i) {
+  // each group on its first member's stream, each other box on its own
+  for (size_t k = 0; k < b->groups.size() && rc == SMX_OK && e == hipSuccess; ++k) {
+    const int i0 = b->groups[k][0];
+    smx_handle* h = b->boxes[i0];
+    e = hipStreamWaitEvent(h->st, b->fork, 0);
+    if (e == hipSuccess)
+      rc = h->fp32 ? group_steps<float>(b, b->groups[k], b->grp_args[k], gsteps, h->st)
+                   : group_steps<double>(b, b->groups[k], b->grp_args[k], gsteps, h->st);
+    if (rc == SMX_OK && e == hipSuccess) e = hipEventRecord(b->joins[i0], h->st);
+    if (rc == SMX_OK && e == hipSuccess) e = hipStreamWaitEvent(b->st, b->joins[i0], 0);
+  }
+  for (size_t k = 0; k < single.size() && rc == SMX_OK && e == hipSuccess; ++k) {
+    const int i = single[k];
     smx_handle* h = b->boxes[i];
     e = hipStreamWaitEvent(h->st, b->fork, 0);
     if (e == hipSuccess) rc = box_steps(h, gsteps);
@@ -1976,6 +2071,7 @@ void smx_batch_destroy(smx_batch* b) {
   batch_drop_adj_graph(b);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  batch_free_groups(b);
   for (auto e : b->joins)
     if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);

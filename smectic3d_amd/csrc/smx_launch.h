@@ -97,6 +97,11 @@ struct PeriodicOps {
                           const double* real_in, double* real_out, const PParams<T>& p, hipStream_t st,
                           PassSched s = PassSched{}, bool dry = false);
   static int zpass_blocks(const PParams<T>& p, int nc = kNC) { return (int)((p.nxy + nc - 1) / nc); }
+  // One CNAB2 step of nbox boxes that share the grid of p and the tile schedules s[NSLOTS]:
+  // the four passes, one grouped launch each (GrpArgs in device memory, one per box, with the
+  // z pass's partial count for its schedule).  dry: only check the schedules.
+  static hipError_t step_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int64_t npencils,
+                             const PassSched* s, hipStream_t st, bool dry = false);
 };
 
 template <typename T>
@@ -216,6 +221,82 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
   return hipErrorInvalidValue;
 }
 
+
+template <typename T>
+hipError_t PeriodicOps<T>::step_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int64_t npencils,
+                                    const PassSched* s, hipStream_t st, bool dry) {
+  if (nbox < 1 || nbox > 65535) return hipErrorInvalidValue;
+  const unsigned nb = (unsigned)nbox;
+  auto ygrp = [&](int sign, PassSched ps) -> hipError_t {
+#define SMX_CASE(L)                                                                                      \
+  if (p.ny == L)                                                                                         \
+    return sched_dispatch<L>(ps, [&](auto nc, auto v) -> hipError_t {                                    \
+      constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
+      constexpr int NT = NC * (L / PlanV<L, V>::E);                                                      \
+      if constexpr (NT > 1024 || (NC * ColAddr<L>::LP + L) * sizeof(cplx<T>) + L * sizeof(T) > kMaxLds) { \
+        return hipErrorInvalidValue;                                                                     \
+      } else {                                                                                           \
+        if (dry) return hipSuccess;                                                                      \
+        const dim3 grid((unsigned)((npencils + NC - 1) / NC), nb);                                       \
+        if (sign < 0) SMX_LAUNCH((k_ypass_grp<T, L, -1, NC, V>), grid, dim3(NT), 0, st, g, npencils);   \
+        else SMX_LAUNCH((k_ypass_grp<T, L, +1, NC, V>), grid, dim3(NT), 0, st, g, npencils);            \
+        return hipGetLastError();                                                                        \
+      }                                                                                                  \
+    });
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  };
+  auto zgrp = [&](PassSched ps) -> hipError_t {
+    if (p.nxy % ps.nc != 0) return hipErrorInvalidValue;
+#define SMX_CASE(L)                                                                                      \
+  if (p.nz == L)                                                                                         \
+    return sched_dispatch<L>(ps, [&](auto nc, auto v) -> hipError_t {                                    \
+      constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
+      constexpr int NT = (NC / 2) * (L / PlanV<L, V>::E);                                                \
+      if constexpr (NT > 1024 || (size_t)L * (NC / 2 + 2) * sizeof(cplx<T>) > kMaxLds) {                 \
+        return hipErrorInvalidValue;                                                                     \
+      } else {                                                                                           \
+        if (dry) return hipSuccess;                                                                      \
+        SMX_LAUNCH((k_zpass_grp<T, L, NC, V>), dim3((unsigned)(p.nxy / NC), nb), dim3(NT), 0, st, g);    \
+        return hipGetLastError();                                                                        \
+      }                                                                                                  \
+    });
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  };
+  auto xgrp = [&](PassSched ps) -> hipError_t {
+    if (p.ny % ps.nc != 0 || ps.v < 0 || ps.v > 7) return hipErrorInvalidValue;
+    const int ord = ps.v >> 1;
+    const PassSched sp{ps.nc, ps.v & 1};
+#define SMX_CASE(L)                                                                                      \
+  if (p.nx == L)                                                                                         \
+    return sched_dispatch<L>(sp, [&](auto nc, auto v) -> hipError_t {                                    \
+      constexpr int NC = decltype(nc)::value, V = decltype(v)::value;                                    \
+      constexpr int NT = NC * (L / PlanV<L, V>::E);                                                      \
+      if constexpr (NT > 1024 || (size_t)L * (NC + 2) * sizeof(cplx<T>) > kMaxLds) {                     \
+        return hipErrorInvalidValue;                                                                     \
+      } else {                                                                                           \
+        if (dry) return hipSuccess;                                                                      \
+        const dim3 grid((unsigned)(p.ny / NC), (unsigned)p.nzh, nb);                                     \
+        if (ord == 0) SMX_LAUNCH((k_xpass_grp<T, L, NC, V, 0>), grid, dim3(NT), 0, st, g);              \
+        else if (ord == 1) SMX_LAUNCH((k_xpass_grp<T, L, NC, V, 1>), grid, dim3(NT), 0, st, g);         \
+        else if (ord == 2) SMX_LAUNCH((k_xpass_grp<T, L, NC, V, 2>), grid, dim3(NT), 0, st, g);         \
+        else SMX_LAUNCH((k_xpass_grp<T, L, NC, V, 3>), grid, dim3(NT), 0, st, g);                       \
+        return hipGetLastError();                                                                        \
+      }                                                                                                  \
+    });
+    SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+    return hipErrorInvalidValue;
+  };
+  hipError_t e = ygrp(+1, s[SLOT_YINV]);
+  if (e == hipSuccess) e = zgrp(s[SLOT_Z]);
+  if (e == hipSuccess) e = ygrp(-1, s[SLOT_YFWD]);
+  if (e == hipSuccess) e = xgrp(s[SLOT_X]);
+  return e;
+}
 
 // Tile width (columns, in pairs of adjacent lines) of the Neumann passes: rows of 128 bytes.
 template <typename T> constexpr int neumann_nc() { return sizeof(T) == 4 ? 32 : 16; }

@@ -100,12 +100,12 @@ __device__ __forceinline__ cplx<T> kapply(const PParams<T>& p, int op, T qx, T q
 }
 
 // ---------------------------------------------------------------------------
-// y pass: contiguous pencils of length L = ny; pencils g = kz*nx + x.
-template <typename T, int L, int SIGN, int NC, int V = 0>
-__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>* in, cplx<T>* out,
-                                                               int64_t npencils,
-                                                               const cplx<T>* __restrict__ tw,
-                                                               const T* __restrict__ kmul, int kop) {
+// y pass: contiguous pencils of length L = ny; pencils g = kz*nx + x.  One tile (NC pencils
+// from bx*NC) per block: the body of k_ypass and of the grouped k_ypass_grp.
+template <typename T, int L, int SIGN, int NC, int V>
+__device__ __forceinline__ void ypass_tile(const cplx<T>* in, cplx<T>* out, int64_t npencils,
+                                           const cplx<T>* __restrict__ tw, const T* __restrict__ kmul, int kop,
+                                           int bx) {
   // lanes run along each pencil (tc fastest), so the first-pass loads and last-pass
   // stores go straight between HBM and registers (16 contiguous complex per lane group)
   constexpr int E = PlanV<L, V>::E, TPC = L / E;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
   const int tid = threadIdx.x, tc = tid % TPC, col = tid / TPC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NC * TPC>(TW, tw, tid);
-  const int64_t pen = (int64_t)blockIdx.x * NC + col;
+  const int64_t pen = (int64_t)bx * NC + col;
   const bool valid = pen < npencils;
   const cplx<T>* src = in + pen * L;
   // LDS copy of the multiplier's k_y (global loads behind the row loads would double the
@@ -146,9 +146,17 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
     for (int e = 0; e < E; ++e) dst[TL::template pos_in<RL>(tc, e)] = v[e];
   }
 }
+template <typename T, int L, int SIGN, int NC, int V = 0>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>* in, cplx<T>* out,
+                                                               int64_t npencils,
+                                                               const cplx<T>* __restrict__ tw,
+                                                               const T* __restrict__ kmul, int kop) {
+  ypass_tile<T, L, SIGN, NC, V>(in, out, npencils, tw, kmul, kop, blockIdx.x);
+}
 
 // ---------------------------------------------------------------------------
-// x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1, rows x.
+// x pass.  Tile: plane kz = by, columns y0 = bx*NC .. y0+NC-1, rows x (the body of k_xpass,
+// bx = blockIdx.x, by = blockIdx.y, and of the grouped k_xpass_grp).
 enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
 
 // ORD (X_STEP only) orders the carry traffic.  0: R is read and rewritten element by
@@ -160,11 +168,9 @@ enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
 // through one in-order counter, so a load issued behind a store waits for that store;
 // which order hides more latency depends on the precision and tile shape, and the
 // autotuner measures all four.  k_x comes from an LDS copy in every order.
-template <typename T, int L, int NC, int MODE, int V = 0, int ORD = 0>
-__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
-                                                               cplx<T>* __restrict__ rbuf,
-                                                               const cplx<T>* __restrict__ psi0,
-                                                               PParams<T> p) {
+template <typename T, int L, int NC, int MODE, int V, int ORD>
+__device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx<T>* __restrict__ rbuf,
+                                           const cplx<T>* __restrict__ psi0, const PParams<T> p, int bx, int by) {
   static_assert(ORD == 0 || MODE == X_STEP, "the late-store order is a form of the CNAB2 step");
   constexpr int E = PlanV<L, V>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
   constexpr uint32_t CB = sizeof(cplx<T>);
@@ -183,7 +189,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
       __syncthreads();
     }
   }
-  const int kz = blockIdx.y, y0 = blockIdx.x * NC;
+  const int kz = by, y0 = bx * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;  // bytes between kx rows
   // On the self-conjugate planes kz = 0 and kz = nz/2 a block takes NC/2 columns ky and
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
   const bool herm = (MODE == X_STEP || MODE == X_BOOT) && (kz == 0 || 2 * kz == p.nz);
   int ky = y0 + col, pcol = col;
   if (herm) {
-    const int b = blockIdx.x * H;
+    const int b = bx * H;
     if (col < H) {
       ky = b + col;
       pcol = (b + col == 0) ? col : col + H;
@@ -334,6 +340,13 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>
       st_off(rb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs, cr[e]);
   }
 }
+template <typename T, int L, int NC, int MODE, int V = 0, int ORD = 0>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass(const cplx<T>* in, cplx<T>* out,
+                                                               cplx<T>* __restrict__ rbuf,
+                                                               const cplx<T>* __restrict__ psi0,
+                                                               PParams<T> p) {
+  xpass_tile<T, L, NC, MODE, V, ORD>(in, out, rbuf, psi0, p, blockIdx.x, blockIdx.y);
+}
 
 // ---------------------------------------------------------------------------
 // z pass.  Tile: columns f0 .. f0+NC-1 of the (x,y) plane; adjacent columns
@@ -343,11 +356,10 @@ enum ZIn { Z_IN_SPEC = 0, Z_IN_REAL = 1 };
 enum ZOut { Z_OUT_SPEC = 1, Z_OUT_REAL = 2 };
 enum ZOp { Z_OP_ID = 0, Z_OP_NL = 1, Z_OP_MUL = 2 };
 
-template <typename T, int L, int NC, int IN, int OUT, int V = 0>
-__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const cplx<T>* in, cplx<T>* out,
-                                                                       const double* __restrict__ real_in,
-                                                                       double* __restrict__ real_out,
-                                                                       int op, PParams<T> p) {
+// One tile (columns bx*NC ..) per block: the body of k_zpass and of the grouped k_zpass_grp.
+template <typename T, int L, int NC, int IN, int OUT, int V>
+__device__ __forceinline__ void zpass_tile(const cplx<T>* in, cplx<T>* out, const double* __restrict__ real_in,
+                                           double* __restrict__ real_out, int op, const PParams<T> p, int bx) {
   // The grid covers whole tiles: nx*ny is a multiple of NC (every plan length is a
   // multiple of 8 and the launcher checks it).
   constexpr int E = PlanV<L, V>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
@@ -358,7 +370,7 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
   const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twz, tid);
-  const int64_t f0 = (int64_t)blockIdx.x * NC;
+  const int64_t f0 = (int64_t)bx * NC;
   const uint32_t rowb = (uint32_t)p.nxy * CB;  // bytes between kz rows of one column
   const uint32_t qofs = (uint32_t)q * 2 * CB;  // the lane's column pair within a row
   cplx<T> v[E];
@@ -476,8 +488,45 @@ __global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const
         st_off(dst, (uint32_t)tc * rowb + (uint32_t)(it * TPC) * rowb + qofs, xy);
       }
     }
-    if (p.partial != nullptr) store_partial<NT>(red, p.partial + blockIdx.x, tid);
+    if (p.partial != nullptr) store_partial<NT>(red, p.partial + bx, tid);
   }
+}
+template <typename T, int L, int NC, int IN, int OUT, int V = 0>
+__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass(const cplx<T>* in, cplx<T>* out,
+                                                                       const double* __restrict__ real_in,
+                                                                       double* __restrict__ real_out,
+                                                                       int op, PParams<T> p) {
+  zpass_tile<T, L, NC, IN, OUT, V>(in, out, real_in, real_out, op, p, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped step kernels: boxes of one batch that share grid and tile schedules advance with one
+// launch per pass instead of one per box (small boxes are bound by launch latency, not by
+// HBM).  Box b of the group is blockIdx.y (y, z passes) or blockIdx.z (x pass); its stage
+// array a (in place), carry r, y twiddles and parameters (with the z pass's partial count)
+// come from g[b].  Same tile bodies as the single-box kernels, so results are bitwise equal.
+template <typename T>
+struct GrpArgs {
+  cplx<T>* a;
+  cplx<T>* r;
+  const cplx<T>* twy;
+  PParams<T> p;
+};
+template <typename T, int L, int SIGN, int NC, int V>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass_grp(const GrpArgs<T>* __restrict__ g,
+                                                                   int64_t npencils) {
+  const GrpArgs<T>& b = g[blockIdx.y];
+  ypass_tile<T, L, SIGN, NC, V>(b.a, b.a, npencils, b.twy, nullptr, KM_IKY, blockIdx.x);
+}
+template <typename T, int L, int NC, int V>
+__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_grp(const GrpArgs<T>* __restrict__ g) {
+  const GrpArgs<T>& b = g[blockIdx.y];
+  zpass_tile<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC, V>(b.a, b.a, nullptr, nullptr, Z_OP_NL, b.p, blockIdx.x);
+}
+template <typename T, int L, int NC, int V, int ORD>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_grp(const GrpArgs<T>* __restrict__ g) {
+  const GrpArgs<T>& b = g[blockIdx.z];
+  xpass_tile<T, L, NC, X_STEP, V, ORD>(b.a, b.a, b.r, nullptr, b.p, blockIdx.x, blockIdx.y);
 }
 
 }  // namespace smx
