@@ -24,11 +24,13 @@ namespace smx {
 // sums are added to acc[2b], acc[2b+1] (psin == nullptr: no sums).
 // The self-conjugate planes kz = 0, nz/2 pair columns ky / -ky as in k_xpass, and alpha_n is
 // projected onto Hermitian symmetry there before it is stored.
-template <typename T, int L, int NC, int V = 0>
-__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* stage, cplx<T>* __restrict__ carry,
-                                                                   const cplx<T>* __restrict__ psin, PParams<T> p,
-                                                                   T cnext, int init, int tail, double q0d,
-                                                                   double alphad, double* __restrict__ acc) {
+// Tile (plane by, columns bx*NC ..; gx tiles per plane) of k_xpass_adj and of the grouped
+// k_xpass_adj_grp.
+template <typename T, int L, int NC, int V>
+__device__ __forceinline__ void xpass_adj_tile(cplx<T>* stage, cplx<T>* __restrict__ carry,
+                                               const cplx<T>* __restrict__ psin, const PParams<T> p, T cnext,
+                                               int init, int tail, double q0d, double alphad,
+                                               double* __restrict__ acc, int bx, int by, int gx) {
   constexpr int E = PlanV<L, V>::E, TPC = L / E, NT = NC * TPC, NCP = NC + 1;
   constexpr uint32_t CB = sizeof(cplx<T>);
   __shared__ cplx<T> S[L * NCP];
@@ -38,14 +40,14 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
   __shared__ T KX[L];  // k_x for the loop after the forward transform's barriers
   for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
-  const int kz = blockIdx.y, y0 = blockIdx.x * NC;
+  const int kz = by, y0 = bx * NC;
   const int64_t plane = (int64_t)kz * p.nxy;
   const uint32_t rowb = (uint32_t)p.ny * CB;
   constexpr int H = NC / 2;
   const bool herm = (kz == 0 || 2 * kz == p.nz);
   int ky = y0 + col, pcol = col;
   if (herm) {
-    const int b = blockIdx.x * H;
+    const int b = bx * H;
     if (col < H) {
       ky = b + col;
       pcol = (b + col == 0) ? col : col + H;
@@ -119,16 +121,53 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* 
   for (int e = 0; e < E; ++e)
     st_off(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
   if (sums && tid == 0) {  // the block's accumulators, read and written last
-    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    const int b = by * gx + bx;
     acc[2 * b] += wk * be;
     acc[2 * b + 1] += wk * 4.0 * alphad * q0d * bq;
   }
 }
+template <typename T, int L, int NC, int V = 0>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj(cplx<T>* stage, cplx<T>* __restrict__ carry,
+                                                                   const cplx<T>* __restrict__ psin, PParams<T> p,
+                                                                   T cnext, int init, int tail, double q0d,
+                                                                   double alphad, double* __restrict__ acc) {
+  xpass_adj_tile<T, L, NC, V>(stage, carry, psin, p, cnext, init, tail, q0d, alphad, acc, blockIdx.x, blockIdx.y,
+                              gridDim.x);
+}
+
+// Per-box arguments of the grouped adjoint x pass (box = blockIdx.z); cnext, init and tail are
+// the same for every box of a group at one step.
+template <typename T>
+struct AdjGrpArgs {
+  cplx<T>* stage;
+  cplx<T>* carry;
+  const cplx<T>* psin;
+  double* acc;
+  double q0d, alphad;
+  PParams<T> p;
+};
+template <typename T, int L, int NC, int V>
+__global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_adj_grp(const AdjGrpArgs<T>* __restrict__ g,
+                                                                       T cnext, int init, int tail) {
+  const AdjGrpArgs<T>& b = g[blockIdx.z];
+  xpass_adj_tile<T, L, NC, V>(b.stage, b.carry, b.psin, b.p, cnext, init, tail, b.q0d, b.alphad, b.acc, blockIdx.x,
+                              blockIdx.y, gridDim.x);
+}
+
+struct PassSched;  // tile schedule of a pass (smx_launch.h)
 
 template <typename T>
 struct AdjointOps {
   static hipError_t xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T>* psin, const PParams<T>& p, T cnext,
                               int init, int tail, double q0d, double alphad, double* acc, hipStream_t st);
+  // One reverse step of nbox boxes sharing the grid of p and the schedules s: y inverse, z
+  // (Z_OP_MUL with each box's recorded N'), y forward on the stage arrays of gz, then the
+  // adjoint x pass with gx (both arrays in device memory, one entry per box).
+  static hipError_t adj_step_grp(const GrpArgs<T>* gz, const AdjGrpArgs<T>* gx, int nbox, const PParams<T>& p,
+                                 int64_t npencils, const PassSched* s, T cnext, int init, int tail, hipStream_t st);
+  // the grouped adjoint x pass alone
+  static hipError_t xpass_adj_grp(const AdjGrpArgs<T>* gx, int nbox, const PParams<T>& p, T cnext, int init,
+                                  int tail, hipStream_t st);
 };
 
 }  // namespace smx

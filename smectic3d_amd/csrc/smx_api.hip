@@ -979,6 +979,178 @@ static int adj_enqueue(smx_handle* h, double amp) {
   return c2r_spectrum_d1(h, h->real_io);
 }
 
+// ---- adjoint sweeps of a group of boxes in lockstep ----
+// Boxes of one grid, precision, schedules and record layout (T, K, segments, resident sets)
+// run adj_enqueue's sequence together: every per-step pass is one grouped launch over the
+// group, each box reading its own records through per-step argument arrays; the once-per-sweep
+// and once-per-segment work (checkpoint copies, the target, the first and last transforms) is
+// the per-box code, all on the group's stream.
+struct AdjGrpBufs {
+  void* g = nullptr;  // GrpArgs: [plain step | record steps (set, j) | reverse z (set, j)]
+  void* x = nullptr;  // AdjGrpArgs: [first x pass | reverse x (set, j)]
+  int64_t nsets = 0;
+};
+
+static bool adj_same_plan(const smx_handle* a, const smx_handle* b) {
+  return a->adj.T == b->adj.T && a->adj.K == b->adj.K && a->adj.nseg == b->adj.nseg && a->adj.sres == b->adj.sres;
+}
+
+template <typename T>
+static int adj_grp_upload(const std::vector<smx_handle*>& boxes, const std::vector<int>& mem, AdjGrpBufs* out) {
+  const size_t nbox = mem.size();
+  const smx_handle* h0 = boxes[mem[0]];
+  const int64_t K = h0->adj.K, nsets = std::max<int64_t>(1, h0->adj.nseg - h0->adj.sres);
+  std::vector<GrpArgs<T>> g((size_t)(1 + 2 * nsets * K) * nbox);
+  std::vector<AdjGrpArgs<T>> x((size_t)(1 + nsets * K) * nbox);
+  for (size_t m = 0; m < nbox; ++m) {
+    smx_handle* h = boxes[mem[m]];
+    auto& a = h->adj;
+    PParams<T> p = params<T>(h, h->fp32);
+    p.npartial = PeriodicOps<T>::zpass_blocks(p, h->sched[SLOT_Z].nc);
+    PParams<T> pa = params<T>(h, h->fp32);
+    pa.partial = nullptr;
+    const cplx<T>* twy = tables<T>(h).twy;
+    g[m] = GrpArgs<T>{(cplx<T>*)h->A, (cplx<T>*)h->R, twy, p};
+    x[m] = AdjGrpArgs<T>{(cplx<T>*)a.stage, (cplx<T>*)a.carry, (const cplx<T>*)h->P, a.acc, h->cfg.q0, h->cfg.alpha,
+                         pa};
+    for (int64_t set = 0; set < nsets; ++set)
+      for (int64_t j = 0; j < K; ++j) {
+        const size_t r = (size_t)(set * K + j);
+        GrpArgs<T> rec{(cplx<T>*)h->A, (cplx<T>*)h->R, twy, p};
+        rec.p.rec_np = (T*)((char*)a.setnp[set] + a.npbytes * j);
+        rec.p.rec_psi = (cplx<T>*)((char*)a.setpsi[set] + a.sbytes * (j + 1));
+        g[(1 + r) * nbox + m] = rec;
+        GrpArgs<T> zr{(cplx<T>*)a.stage, nullptr, twy, pa};
+        zr.p.mul_np = (const T*)((char*)a.setnp[set] + a.npbytes * j);
+        g[(1 + nsets * K + r) * nbox + m] = zr;
+        AdjGrpArgs<T> xr = x[m];
+        xr.psin = (const cplx<T>*)((char*)a.setpsi[set] + a.sbytes * j);
+        x[(1 + r) * nbox + m] = xr;
+      }
+  }
+  HIPCHK(hipMalloc(&out->g, sizeof(GrpArgs<T>) * g.size()));
+  HIPCHK(hipMemcpy(out->g, g.data(), sizeof(GrpArgs<T>) * g.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&out->x, sizeof(AdjGrpArgs<T>) * x.size()));
+  HIPCHK(hipMemcpy(out->x, x.data(), sizeof(AdjGrpArgs<T>) * x.size(), hipMemcpyHostToDevice));
+  out->nsets = nsets;
+  return SMX_OK;
+}
+
+template <typename T>
+static int adj_enqueue_grp(const std::vector<smx_handle*>& boxes, const std::vector<int>& mem, const double* amp,
+                           const AdjGrpBufs& bufs, hipStream_t S) {
+  const int nbox = (int)mem.size();
+  std::vector<smx_handle*> hs;
+  for (int i : mem) hs.push_back(boxes[i]);
+  smx_handle* h0 = hs[0];
+  const auto& a0 = h0->adj;
+  const int64_t T_ = a0.T, K = a0.K, nseg = a0.nseg, sres = a0.sres, nsets = bufs.nsets;
+  const PParams<T> p0 = params<T>(h0, h0->fp32);
+  const GrpArgs<T>* G = (const GrpArgs<T>*)bufs.g;
+  const AdjGrpArgs<T>* X = (const AdjGrpArgs<T>*)bufs.x;
+  auto gplain = [&]() { return G; };
+  auto grec = [&](int64_t set, int64_t j) { return G + (size_t)(1 + set * K + j) * nbox; };
+  auto gz = [&](int64_t set, int64_t j) { return G + (size_t)(1 + nsets * K + set * K + j) * nbox; };
+  auto gx = [&](int64_t set, int64_t j) { return X + (size_t)(1 + set * K + j) * nbox; };
+  auto grpchk = [&](hipError_t e) -> int {
+    return e == hipSuccess ? SMX_OK : fail(SMX_ERR_HIP, std::string("grouped adjoint: ") + hipGetErrorString(e));
+  };
+  // every member's own work goes to the group's stream while its sweep is enqueued
+  std::vector<hipStream_t> own;
+  for (smx_handle* h : hs) {
+    own.push_back(h->st);
+    h->st = S;
+  }
+  auto body = [&]() -> int {
+    auto save = [&](smx_handle* h, int64_t seg) -> int {
+      auto& a = h->adj;
+      HIPCHK(hipMemcpyAsync(a.ck + 2 * a.sbytes * seg, h->A, a.sbytes, hipMemcpyDeviceToDevice, S));
+      HIPCHK(hipMemcpyAsync(a.ck + 2 * a.sbytes * seg + a.sbytes, h->R, a.sbytes, hipMemcpyDeviceToDevice, S));
+      HIPCHK(hipMemcpyAsync(a.ckmean + 2 * seg, h->mean_state, 2 * sizeof(double), hipMemcpyDeviceToDevice, S));
+      return SMX_OK;
+    };
+    auto restore = [&](smx_handle* h, int64_t seg) -> int {
+      auto& a = h->adj;
+      HIPCHK(hipMemcpyAsync(h->A, a.ck + 2 * a.sbytes * seg, a.sbytes, hipMemcpyDeviceToDevice, S));
+      HIPCHK(hipMemcpyAsync(h->R, a.ck + 2 * a.sbytes * seg + a.sbytes, a.sbytes, hipMemcpyDeviceToDevice, S));
+      HIPCHK(hipMemcpyAsync(h->mean_state, a.ckmean + 2 * seg, 2 * sizeof(double), hipMemcpyDeviceToDevice, S));
+      return SMX_OK;
+    };
+    auto record = [&](int64_t set, int64_t len) -> int {
+      for (smx_handle* h : hs) {  // psi_hat at the set's start, from the stage buffer
+        PParams<T> px = params<T>(h, false);
+        px.scale = (T)(1.0 / (double)h->cfg.nx);
+        CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, h->adj.setpsi[set], nullptr, nullptr));
+      }
+      for (int64_t j = 0; j < len; ++j)
+        CHK(grpchk(PeriodicOps<T>::step_grp(grec(set, j), nbox, p0, h0->npencils, h0->sched, S)));
+      return SMX_OK;
+    };
+    for (int64_t seg = 0; seg < nseg; ++seg) {  // forward sweep
+      const int64_t len = std::min(K, T_ - seg * K);
+      if (seg < sres) {
+        for (smx_handle* h : hs) CHK(save(h, seg));
+        for (int64_t j = 0; j < len; ++j)
+          CHK(grpchk(PeriodicOps<T>::step_grp(gplain(), nbox, p0, h0->npencils, h0->sched, S)));
+      } else {
+        CHK(record(seg - sres, len));
+      }
+    }
+    for (size_t m = 0; m < hs.size(); ++m) {  // a_T, the loss and alpha_T's inputs, per box
+      smx_handle* h = hs[m];
+      const smx_config& c = h->cfg;
+      auto& a = h->adj;
+      CHK(c2r_state(h, h->real_io));
+      hipLaunchKernelGGL(k_adj_target, dim3(kRedBlocks), dim3(kRedThreads), 0, S, h->real_io, h->N, c.nz, h->dz,
+                         c.q0, amp[m], h->red);
+      HIPCHK(hipGetLastError());
+      PParams<T> pa = params<T>(h, h->fp32);
+      pa.partial = nullptr;
+      CHK(zpass<T>(h, pa, Z_IN_REAL, Z_OUT_SPEC, Z_OP_ID, nullptr, a.stage, h->real_io, nullptr));
+      CHK(ypass<T>(h, a.stage, a.stage, -1));
+      hipLaunchKernelGGL(k_zero_f64, dim3((2 * a.nblk + 255) / 256), dim3(256), 0, S, a.acc, (int64_t)(2 * a.nblk));
+      HIPCHK(hipGetLastError());
+      PParams<T> px = params<T>(h, false);
+      px.scale = (T)(1.0 / (double)c.nx);
+      CHK(xpass<T>(h, px, X_FWD_SCALED, h->A, h->P, nullptr, nullptr));
+    }
+    // first adjoint x pass: alpha_T and w_{T-1}, the Psi_T half of step T-1
+    {
+      const hipError_t e = AdjointOps<T>::xpass_adj_grp(X, nbox, p0, (T)(T_ - 1 >= 1 ? 3 : 2), 1, 1, S);
+      CHK(grpchk(e));
+    }
+    for (int64_t seg = nseg - 1; seg >= 0; --seg) {  // reverse sweep
+      const int64_t s0 = seg * K, len = std::min(K, T_ - s0);
+      int64_t set = seg - sres;
+      if (seg < sres) {
+        set = 0;
+        for (smx_handle* h : hs) CHK(restore(h, seg));
+        CHK(record(0, len));
+      }
+      for (int64_t j = len - 1; j >= 0; --j) {
+        const int64_t n = s0 + j;
+        CHK(grpchk(AdjointOps<T>::adj_step_grp(gz(set, j), gx(set, j), nbox, p0, h0->npencils, h0->sched,
+                                               (T)(n - 1 >= 1 ? 3 : 2), 0, n >= 1 ? 1 : 0, S)));
+      }
+    }
+    for (smx_handle* h : hs) {  // dJ/dpsi0 = F^-1 alpha_0 (FP64 inverse chain)
+      const int64_t ncomp = 2 * h->nspec;
+      if (h->fp32) {
+        hipLaunchKernelGGL(k_f32_to_f64, dim3(kRedBlocks), dim3(kRedThreads), 0, S, (const float*)h->adj.carry,
+                           (double*)h->D1, ncomp);
+        HIPCHK(hipGetLastError());
+      } else {
+        HIPCHK(hipMemcpyAsync(h->D1, h->adj.carry, sizeof(double) * ncomp, hipMemcpyDeviceToDevice, S));
+      }
+      CHK(c2r_spectrum_d1(h, h->real_io));
+    }
+    return SMX_OK;
+  };
+  const int rc = body();
+  for (size_t i = 0; i < hs.size(); ++i) hs[i]->st = own[i];
+  return rc;
+}
+
 // Host side of the adjoint's results after adj_enqueue: waits for the stream, sums the loss
 // partials and the parameter sums in a fixed order.
 static int adj_finish(smx_handle* h, double* loss, double* deps, double* dq0) {
@@ -1026,6 +1198,7 @@ struct smx_batch {
   hipGraphExec_t adj_exec = nullptr;
   std::vector<double> adj_key;  // workspace generations, step counts, amplitudes and schedules
   int64_t adj_nodes = 0;
+  std::vector<void*> adj_args;  // device argument arrays of the grouped adjoint sweeps
 };
 
 static void batch_drop_adj_graph(smx_batch* b) {
@@ -1034,6 +1207,9 @@ static void batch_drop_adj_graph(smx_batch* b) {
   b->adj_exec = nullptr;
   b->adj_graph = nullptr;
   b->adj_key.clear();
+  for (void* d : b->adj_args)
+    if (d) (void)hipFree(d);
+  b->adj_args.clear();
 }
 
 static int box_steps(smx_handle* h, int64_t n) {
@@ -2197,10 +2373,59 @@ int smx_batch_adjoint_grad(smx_batch* b, const double* const* psi0, int64_t row_
   }
   if (!b->adj_exec || b->adj_key != key) {
     batch_drop_adj_graph(b);
+    // boxes that share grid, precision, schedules and record layout sweep together
+    std::vector<std::vector<int>> groups;
+    std::vector<int> single;
+    {
+      std::vector<bool> used(nb, false);
+      for (size_t i = 0; i < nb; ++i) {
+        if (used[i]) continue;
+        std::vector<int> g{(int)i};
+        used[i] = true;
+        const smx_handle* h = b->boxes[i];
+        for (size_t j = i + 1; j < nb; ++j) {
+          const smx_handle* o = b->boxes[j];
+          if (!used[j] && o->fp32 == h->fp32 && o->cfg.nx == h->cfg.nx && o->cfg.ny == h->cfg.ny &&
+              o->cfg.nz == h->cfg.nz && same_sched(o, h) && adj_same_plan(o, h)) {
+            g.push_back((int)j);
+            used[j] = true;
+          }
+        }
+        if (g.size() > 1) groups.push_back(g);
+        else single.push_back((int)i);
+      }
+    }
+    std::vector<AdjGrpBufs> bufs(groups.size());
+    for (size_t k = 0; k < groups.size(); ++k) {
+      const bool f32 = b->boxes[groups[k][0]]->fp32;
+      const int urc = f32 ? adj_grp_upload<float>(b->boxes, groups[k], &bufs[k])
+                          : adj_grp_upload<double>(b->boxes, groups[k], &bufs[k]);
+      b->adj_args.push_back(bufs[k].g);
+      b->adj_args.push_back(bufs[k].x);
+      if (urc != SMX_OK) {
+        batch_drop_adj_graph(b);
+        for (smx_handle* g : b->boxes)  // what the default budgets lent for this call goes back
+          if (g->adj_budget < 0) adj_release(g);
+        return urc;
+      }
+    }
     HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
     int rc = SMX_OK;
     hipError_t e = hipEventRecord(b->fork, b->st);
-    for (size_t i = 0; i < nb && rc == SMX_OK && e == hipSuccess; ++i) {
+    for (size_t k = 0; k < groups.size() && rc == SMX_OK && e == hipSuccess; ++k) {
+      const int i0 = groups[k][0];
+      smx_handle* h = b->boxes[i0];
+      std::vector<double> amps;
+      for (int i : groups[k]) amps.push_back(target_amp[i]);
+      e = hipStreamWaitEvent(h->st, b->fork, 0);
+      if (e == hipSuccess)
+        rc = h->fp32 ? adj_enqueue_grp<float>(b->boxes, groups[k], amps.data(), bufs[k], h->st)
+                     : adj_enqueue_grp<double>(b->boxes, groups[k], amps.data(), bufs[k], h->st);
+      if (rc == SMX_OK && e == hipSuccess) e = hipEventRecord(b->joins[i0], h->st);
+      if (rc == SMX_OK && e == hipSuccess) e = hipStreamWaitEvent(b->st, b->joins[i0], 0);
+    }
+    for (size_t k = 0; k < single.size() && rc == SMX_OK && e == hipSuccess; ++k) {
+      const int i = single[k];
       smx_handle* h = b->boxes[i];
       e = hipStreamWaitEvent(h->st, b->fork, 0);
       if (e == hipSuccess) rc = h->fp32 ? adj_enqueue<float>(h, target_amp[i]) : adj_enqueue<double>(h, target_amp[i]);
@@ -2211,6 +2436,9 @@ int smx_batch_adjoint_grad(smx_batch* b, const double* const* psi0, int64_t row_
     hipError_t e2 = hipStreamEndCapture(b->st, &g);
     if (rc != SMX_OK || e != hipSuccess || e2 != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
+      batch_drop_adj_graph(b);
+      for (smx_handle* gb : b->boxes)
+        if (gb->adj_budget < 0) adj_release(gb);
       if (rc != SMX_OK) return rc;
       return fail(SMX_ERR_HIP, std::string("adjoint batch capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
     }

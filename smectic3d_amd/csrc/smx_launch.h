@@ -102,6 +102,13 @@ struct PeriodicOps {
   // z pass's partial count for its schedule).  dry: only check the schedules.
   static hipError_t step_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int64_t npencils,
                              const PassSched* s, hipStream_t st, bool dry = false);
+  // The grouped passes one by one (in place on each box's stage array g[b].a).
+  static hipError_t ypass_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int64_t npencils, int sign,
+                              PassSched s, hipStream_t st, bool dry = false);
+  static hipError_t zpass_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int op, PassSched s,
+                              hipStream_t st, bool dry = false);
+  static hipError_t xpass_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, PassSched s, hipStream_t st,
+                              bool dry = false);
 };
 
 template <typename T>
@@ -223,11 +230,10 @@ hipError_t PeriodicOps<T>::zpass(int in_mode, int out_mode, int op, const cplx<T
 
 
 template <typename T>
-hipError_t PeriodicOps<T>::step_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int64_t npencils,
-                                    const PassSched* s, hipStream_t st, bool dry) {
+hipError_t PeriodicOps<T>::ypass_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int64_t npencils, int sign,
+                                     PassSched ps, hipStream_t st, bool dry) {
   if (nbox < 1 || nbox > 65535) return hipErrorInvalidValue;
   const unsigned nb = (unsigned)nbox;
-  auto ygrp = [&](int sign, PassSched ps) -> hipError_t {
 #define SMX_CASE(L)                                                                                      \
   if (p.ny == L)                                                                                         \
     return sched_dispatch<L>(ps, [&](auto nc, auto v) -> hipError_t {                                    \
@@ -243,12 +249,16 @@ hipError_t PeriodicOps<T>::step_grp(const GrpArgs<T>* g, int nbox, const PParams
         return hipGetLastError();                                                                        \
       }                                                                                                  \
     });
-    SMX_FOR_EACH_LENGTH(SMX_CASE)
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
-    return hipErrorInvalidValue;
-  };
-  auto zgrp = [&](PassSched ps) -> hipError_t {
-    if (p.nxy % ps.nc != 0) return hipErrorInvalidValue;
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t PeriodicOps<T>::zpass_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int op, PassSched ps,
+                                     hipStream_t st, bool dry) {
+  if (nbox < 1 || nbox > 65535 || p.nxy % ps.nc != 0) return hipErrorInvalidValue;
+  const unsigned nb = (unsigned)nbox;
 #define SMX_CASE(L)                                                                                      \
   if (p.nz == L)                                                                                         \
     return sched_dispatch<L>(ps, [&](auto nc, auto v) -> hipError_t {                                    \
@@ -258,18 +268,22 @@ hipError_t PeriodicOps<T>::step_grp(const GrpArgs<T>* g, int nbox, const PParams
         return hipErrorInvalidValue;                                                                     \
       } else {                                                                                           \
         if (dry) return hipSuccess;                                                                      \
-        SMX_LAUNCH((k_zpass_grp<T, L, NC, V>), dim3((unsigned)(p.nxy / NC), nb), dim3(NT), 0, st, g);    \
+        SMX_LAUNCH((k_zpass_grp<T, L, NC, V>), dim3((unsigned)(p.nxy / NC), nb), dim3(NT), 0, st, g, op); \
         return hipGetLastError();                                                                        \
       }                                                                                                  \
     });
-    SMX_FOR_EACH_LENGTH(SMX_CASE)
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
-    return hipErrorInvalidValue;
-  };
-  auto xgrp = [&](PassSched ps) -> hipError_t {
-    if (p.ny % ps.nc != 0 || ps.v < 0 || ps.v > 7) return hipErrorInvalidValue;
-    const int ord = ps.v >> 1;
-    const PassSched sp{ps.nc, ps.v & 1};
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t PeriodicOps<T>::xpass_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, PassSched ps,
+                                     hipStream_t st, bool dry) {
+  if (nbox < 1 || nbox > 65535 || p.ny % ps.nc != 0 || ps.v < 0 || ps.v > 7) return hipErrorInvalidValue;
+  const unsigned nb = (unsigned)nbox;
+  const int ord = ps.v >> 1;
+  const PassSched sp{ps.nc, ps.v & 1};
 #define SMX_CASE(L)                                                                                      \
   if (p.nx == L)                                                                                         \
     return sched_dispatch<L>(sp, [&](auto nc, auto v) -> hipError_t {                                    \
@@ -287,14 +301,18 @@ hipError_t PeriodicOps<T>::step_grp(const GrpArgs<T>* g, int nbox, const PParams
         return hipGetLastError();                                                                        \
       }                                                                                                  \
     });
-    SMX_FOR_EACH_LENGTH(SMX_CASE)
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
 #undef SMX_CASE
-    return hipErrorInvalidValue;
-  };
-  hipError_t e = ygrp(+1, s[SLOT_YINV]);
-  if (e == hipSuccess) e = zgrp(s[SLOT_Z]);
-  if (e == hipSuccess) e = ygrp(-1, s[SLOT_YFWD]);
-  if (e == hipSuccess) e = xgrp(s[SLOT_X]);
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t PeriodicOps<T>::step_grp(const GrpArgs<T>* g, int nbox, const PParams<T>& p, int64_t npencils,
+                                    const PassSched* s, hipStream_t st, bool dry) {
+  hipError_t e = ypass_grp(g, nbox, p, npencils, +1, s[SLOT_YINV], st, dry);
+  if (e == hipSuccess) e = zpass_grp(g, nbox, p, Z_OP_NL, s[SLOT_Z], st, dry);
+  if (e == hipSuccess) e = ypass_grp(g, nbox, p, npencils, -1, s[SLOT_YFWD], st, dry);
+  if (e == hipSuccess) e = xpass_grp(g, nbox, p, s[SLOT_X], st, dry);
   return e;
 }
 
@@ -451,6 +469,35 @@ hipError_t AdjointOps<T>::xpass_adj(cplx<T>* stage, cplx<T>* carry, const cplx<T
     constexpr int V = adj_variant<L>();                                                                  \
     SMX_LAUNCH((k_xpass_adj<T, L, kNC, V>), grid, dim3(kNC * (L / PlanV<L, V>::E)), 0, st, stage, carry, psin, \
                p, cnext, init, tail, q0d, alphad, acc);                                                  \
+    return hipGetLastError();                                                                            \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t AdjointOps<T>::adj_step_grp(const GrpArgs<T>* gz, const AdjGrpArgs<T>* gx, int nbox, const PParams<T>& p,
+                                       int64_t npencils, const PassSched* s, T cnext, int init, int tail,
+                                       hipStream_t st) {
+  using PO = PeriodicOps<T>;
+  hipError_t e = PO::ypass_grp(gz, nbox, p, npencils, +1, s[SLOT_YINV], st);
+  if (e == hipSuccess) e = PO::zpass_grp(gz, nbox, p, Z_OP_MUL, s[SLOT_Z], st);
+  if (e == hipSuccess) e = PO::ypass_grp(gz, nbox, p, npencils, -1, s[SLOT_YFWD], st);
+  if (e != hipSuccess) return e;
+  return xpass_adj_grp(gx, nbox, p, cnext, init, tail, st);
+}
+
+template <typename T>
+hipError_t AdjointOps<T>::xpass_adj_grp(const AdjGrpArgs<T>* gx, int nbox, const PParams<T>& p, T cnext, int init,
+                                        int tail, hipStream_t st) {
+  if (nbox < 1 || nbox > 65535) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)(p.ny / kNC), (unsigned)p.nzh, (unsigned)nbox);
+#define SMX_CASE(L)                                                                                      \
+  if (p.nx == L) {                                                                                       \
+    constexpr int V = adj_variant<L>();                                                                  \
+    SMX_LAUNCH((k_xpass_adj_grp<T, L, kNC, V>), grid, dim3(kNC * (L / PlanV<L, V>::E)), 0, st, gx, cnext, init, \
+               tail);                                                                                    \
     return hipGetLastError();                                                                            \
   }
   SMX_FOR_EACH_LENGTH(SMX_CASE)

@@ -519,9 +519,10 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass_grp(const Grp
   ypass_tile<T, L, SIGN, NC, V>(b.a, b.a, npencils, b.twy, nullptr, KM_IKY, blockIdx.x);
 }
 template <typename T, int L, int NC, int V>
-__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_grp(const GrpArgs<T>* __restrict__ g) {
+__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zpass_grp(const GrpArgs<T>* __restrict__ g,
+                                                                           int op) {
   const GrpArgs<T>& b = g[blockIdx.y];
-  zpass_tile<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC, V>(b.a, b.a, nullptr, nullptr, Z_OP_NL, b.p, blockIdx.x);
+  zpass_tile<T, L, NC, Z_IN_SPEC, Z_OUT_SPEC, V>(b.a, b.a, nullptr, nullptr, op, b.p, blockIdx.x);
 }
 template <typename T, int L, int NC, int V, int ORD>
 __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_xpass_grp(const GrpArgs<T>* __restrict__ g) {

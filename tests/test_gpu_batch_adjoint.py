@@ -1,6 +1,9 @@
 """Batched gradients (smx_batch_adjoint_grad): the adjoint sweeps of uneven boxes as
-parallel branches of one captured hipGraph, against the single-box smx_adjoint_grad (bit for
-bit: same kernels, same order, same data) and against the FP64 oracle's adjoint."""
+parallel branches of one captured hipGraph, against the single-box smx_adjoint_grad and against
+the FP64 oracle's adjoint.  A box alone in its branch runs the single-box kernels (bit for bit
+the same gradient); boxes of one grid sweep together through the grouped kernels, whose
+tile bodies are the same but whose multiply-adds the compiler fuses a little differently, so
+their gradients agree to rounding."""
 import math
 import sys
 
@@ -51,12 +54,18 @@ def test_batch_adjoint_equals_single_box(prec):
         nodes = b.adjoint_nodes()
         after = [s.get_psi() for s in b.boxes]
     say("  graph nodes", nodes)
-    assert nodes > len(sizes) * 8 * T
+    assert nodes > len(set(sizes)) * 8 * T  # one branch per grid
+    tol = 1e-12 if prec == A.SMX_FP64 else 1e-5
     for i, (c, p) in enumerate(zip(cfgs, psis)):
         J1, d1, e1, q1 = single(c, p, T, K, amps[i])
         say("  box", i, c.nx, "J", J[i], J1, "deps", de[i], e1, "dq0", dq[i], q1)
-        assert J[i] == J1 and de[i] == e1 and dq[i] == q1
-        assert np.array_equal(d[i], d1)
+        if sizes.count(c.nx) == 1:
+            assert J[i] == J1 and de[i] == e1 and dq[i] == q1
+            assert np.array_equal(d[i], d1)
+        else:
+            assert abs(J[i] - J1) <= tol * abs(J1)
+            assert abs(de[i] - e1) <= tol * abs(e1) and abs(dq[i] - q1) <= tol * abs(q1)
+            assert rel_l2(d[i], d1) < tol
         assert rel_l2(after[i], p) < (1e-13 if prec == A.SMX_FP64 else 1e-6)  # every box is left at psi0
 
 
