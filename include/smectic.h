@@ -225,9 +225,11 @@ int smx_adjoint_info(const smx_handle *h, int64_t *resident_steps, int64_t *reco
 /*
  * Ensembles of independent boxes of uneven sizes (parameter sweeps).  Each box is
  * a full solver (its own config and buffers, on its own stream); smx_batch_run
- * advances every box by nsteps through one captured hipGraph in which the boxes'
- * step chains are parallel branches (fork/join on the batch stream).  No
- * counterpart in the reference (it runs one box per MPI job).
+ * advances every box by nsteps through one captured hipGraph whose parallel branches
+ * (fork/join on the batch stream) are the step chains of single boxes and of groups:
+ * periodic, non-advected boxes of one precision, grid and tile schedule advance with one
+ * grouped launch per pass (equal to their single-box runs to rounding: a few multiply-adds
+ * are fused differently).  No counterpart in the reference (it runs one box per MPI job).
  */
 typedef struct smx_batch smx_batch;
 int smx_batch_create(int nbox, const smx_config *cfgs, smx_batch **out);
@@ -246,7 +248,8 @@ int smx_batch_graph_nodes(smx_batch *b, int64_t *nodes);
  * psi0[i] (host, reference layout, row stride >= that box's nz), and dJ_i/dpsi0 (into
  * dpsi0[i]), dJ_i/deps, dJ_i/dq0 -- smx_adjoint_grad of each box, with checkpoints every
  * ckpt_every steps, but the adjoint sweeps of all boxes are the parallel branches of one
- * captured hipGraph, launched once.  Record budgets: a box with an explicit
+ * captured hipGraph, launched once (boxes of one grid, schedule and record layout sweep
+ * together through grouped launches, as in smx_batch_run).  Record budgets: a box with an explicit
  * smx_set_adjoint_budget uses it and keeps its workspace (the batch then keeps the captured
  * graph for the next call of the same shape when every box does); the other boxes share
  * 80% of the free device memory in proportion to the records they need and release their
