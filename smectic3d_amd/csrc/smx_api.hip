@@ -1193,6 +1193,7 @@ struct smx_batch {
   // array in device memory (built at capture, freed with the graph)
   std::vector<std::vector<int>> groups;
   std::vector<void*> grp_args;
+  std::vector<std::array<PassSched, NSLOTS>> cap_sched;  // every box's schedules at capture
   // captured adjoint sweeps of every box, kept while every box keeps its workspace
   hipGraph_t adj_graph = nullptr;
   hipGraphExec_t adj_exec = nullptr;
@@ -1304,6 +1305,12 @@ static int batch_capture(smx_batch* b, int64_t gsteps) {
     b->graph = nullptr;
   }
   batch_free_groups(b);
+  b->cap_sched.clear();
+  for (const smx_handle* h : b->boxes) {
+    std::array<PassSched, NSLOTS> s;
+    for (int k = 0; k < NSLOTS; ++k) s[k] = h->sched[k];
+    b->cap_sched.push_back(s);
+  }
   std::vector<int> single;
   batch_make_groups(b, single);
   b->grp_args.assign(b->groups.size(), nullptr);
@@ -2270,7 +2277,12 @@ int smx_batch_run(smx_batch* b, int64_t nsteps, int64_t graph_steps) {
   }
   HIPCHK(hipSetDevice(b->boxes[0]->cfg.device));
   const int64_t ng = nsteps / graph_steps, rem = nsteps - ng * graph_steps;
-  if (ng > 0 && (!b->exec || b->graph_steps != graph_steps)) CHK(batch_capture(b, graph_steps));
+  bool stale = !b->exec || b->graph_steps != graph_steps || b->cap_sched.size() != b->boxes.size();
+  for (size_t i = 0; !stale && i < b->boxes.size(); ++i)  // a box's schedule set since the capture
+    for (int k = 0; k < NSLOTS; ++k)
+      if (b->cap_sched[i][k].nc != b->boxes[i]->sched[k].nc || b->cap_sched[i][k].v != b->boxes[i]->sched[k].v)
+        stale = true;
+  if (ng > 0 && stale) CHK(batch_capture(b, graph_steps));
   for (int64_t g = 0; g < ng; ++g) HIPCHK(hipGraphLaunch(b->exec, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   for (smx_handle* h : b->boxes) {

@@ -146,3 +146,29 @@ def test_batch_of_every_step_kind(prec):
             s.set_psi(p)
             s.step(12)
             np.testing.assert_array_equal(g, s.get_psi())
+
+
+def test_batch_recaptures_after_a_schedule_change():
+    """A box's schedule set after the capture takes it out of its group: the next run captures
+    again (the stale graph would run the old tiles and the old partial counts)."""
+    cfgs = box_cfgs(seed=11, nbox=3, sizes=(48,))
+    psis = [O.ic_noise(c.nx, c.ny, c.nz, 500 + i, 0.05) for i, c in enumerate(cfgs)]
+    alt = [(8, 0), (16, 0), (8, 0), (16, 2)]
+    with Batch(cfgs) as b:
+        for s, p in zip(b.boxes, psis):
+            s.set_psi(p)
+        b.run(4, graph_steps=2)
+        n_grouped = b.graph_nodes()
+        b.boxes[2].set_schedule(alt)
+        b.run(4, graph_steps=2)
+        assert b.graph_nodes() > n_grouped  # box 2 now runs its own branch
+        assert b.boxes[2].schedule() == alt
+        got = [s.get_psi() for s in b.boxes]
+    for i, (c, p, g) in enumerate(zip(cfgs, psis, got)):
+        with Solver(c) as s:
+            s.set_psi(p)
+            s.step(4)
+            if i == 2:
+                s.set_schedule(alt)
+            s.step(4)
+            assert_rounding_level(g, s.get_psi(), A.SMX_FP64)
