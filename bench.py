@@ -172,8 +172,16 @@ def run_neumann(steps, surf_every=20, compare=True):
     r2 = np.maximum(c[:, None, None] ** 2 + c[None, :, None] ** 2 - cz[None, None, :] ** 2, 0.0)
     psi0 = 0.4 * np.cos(np.sqrt(r2 + 4.0))
     out, samples = {}, {}
+    ncalls, nxy = steps // surf_every, nx * ny
     for prec in ((A.SMX_FP32_MIXED, A.SMX_FP64) if compare else (A.SMX_FP32_MIXED,)):
-        vs, hs = [], []
+        # output arrays written in place, touched before the timed loop: with compare, call i's
+        # velocity and H go straight into row i of the sample store (every column reported on that
+        # call, the persistent arrays as the reference's)
+        bufs = np.empty((4, nxy))
+        keep = np.empty((2, ncalls, nxy)) if compare else None
+        for b in (bufs, keep):
+            if b is not None:
+                b.fill(0.0)
         with Solver(neumann_cfg(prec)) as s:
             s.set_psi(psi0)
             s.set_l1_tracking(False)
@@ -181,13 +189,11 @@ def run_neumann(steps, surf_every=20, compare=True):
             s.sync()
             t0 = time.perf_counter()
             found = 0
-            for _ in range(steps // surf_every):
+            for i in range(ncalls):
                 s.step(surf_every)
-                (sz, v, H, K), f = s.surftrack(0.7, -1.0)
+                o = (bufs[0], keep[0, i], keep[1, i], bufs[3]) if compare else bufs
+                _, f = s.surftrack(0.7, -1.0, out=o)
                 found += f
-                if compare:  # every column reported on this call (the persistent arrays, as the reference's)
-                    vs.append(v.copy())
-                    hs.append(H.copy())
             s.sync()
             t1 = time.perf_counter()
             psi_end = s.get_psi() if compare else None
@@ -195,9 +201,9 @@ def run_neumann(steps, surf_every=20, compare=True):
                 ks = s.profile_step(20)
                 t_s0 = time.perf_counter()
                 for _ in range(20):
-                    s.surftrack(0.7, -1.0)
+                    s.surftrack(0.7, -1.0, out=bufs)
                 t_surf = (time.perf_counter() - t_s0) / 20
-        samples[prec] = (np.concatenate(vs) if vs else None, np.concatenate(hs) if hs else None, psi_end)
+        samples[prec] = (keep[0].ravel() if compare else None, keep[1].ravel() if compare else None, psi_end)
         key = "fp32" if prec == A.SMX_FP32_MIXED else "fp64"
         out[key] = {"steps_per_s_with_surftrack": steps / (t1 - t0),
                     "columns_found_per_call": found / max(1, steps // surf_every), "schedule": sched}

@@ -15,6 +15,7 @@
 #include "../../include/smectic.h"
 #include "smx_diag.h"
 #include "smx_launch.h"
+#include "smx_surf.h"
 
 using namespace smx;
 
@@ -1997,31 +1998,54 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   HIPCHK(hipSetDevice(h->cfg.device));
   const int64_t N = h->N, nxy = h->nxy;
   const size_t sdoubles = (h->neumann ? 1 : 2) * (size_t)h->nspec;  // one FP64 stage array
-  // real psi, two FP64 stage arrays (W0, W1), the field values at each column's evaluation
-  // index, and the crossing and evaluation indices of each column
-  if (!h->sfield)
+  const int ns = surf_nsplit(h->neumann, h->cfg.nx, h->cfg.nz);
+  // An FP64 state is its own x stage, and an FP32 Neumann stage is read as it is (widened on
+  // load); a periodic FP32 stage is copied to FP64 and projected (stage_to_f64) first.
+  const bool f32in = h->fp32 && h->neumann, w0copy = h->fp32 && !h->neumann;
+  // real psi, the FP64 x stage copy, the field values at each column's evaluation index (ns
+  // parts per field), the z evaluation's trigonometric table, and the crossing and evaluation
+  // indices of each column
+  const size_t nw0 = w0copy ? sdoubles : 0, nval = 8 * (size_t)ns * nxy, ntrig = 4 * (size_t)h->cfg.nz;
+  double* f = h->sfield;  // psi, dense [x][y][z]
+  if (!f) {
     CHK(dalloc(h, (void**)&h->sfield,
-               sizeof(double) * ((size_t)N + 2 * sdoubles + 8 * (size_t)nxy) + 2 * sizeof(int) * (size_t)nxy));
-  double* f = h->sfield;         // psi, dense [x][y][z]
-  double* W0 = f + N;            // x stage of psi_hat: [kz][x][ky]
-  double* W1 = W0 + sdoubles;    // y inverse of it: [kz][x][y]
-  double* val = W1 + sdoubles;   // SurfVal fields, nxy apiece
-  int* kcol = reinterpret_cast<int*>(val + 8 * nxy);
+               sizeof(double) * ((size_t)N + nw0 + nval + ntrig) + 2 * sizeof(int) * (size_t)nxy));
+    f = h->sfield;
+    hipLaunchKernelGGL(k_surf_tables, dim3((unsigned)((ntrig + 255) / 256)), dim3(256), 0, h->st, f + N + nw0 + nval,
+                       h->cfg.nz, h->neumann ? 1 : 0);
+    HIPCHK(hipGetLastError());
+  }
+  double* W0 = f + N;          // FP64 x stage of psi_hat: [kz][x][ky]
+  double* val = W0 + nw0;      // SurfVal fields, ns x nxy apiece
+  const double* trig = val + nval;
+  int* kcol = reinterpret_cast<int*>(val + nval + ntrig);
   int* kev = kcol + nxy;
   PParams<double> p = params<double>(h, false);
-  // W0 from the state (the FP64 state is its own x stage), psi_hat into D2, psi into f
+  const DevTables<double>& t = h->t64;
+  // psi into f (through its y inverse in D1) and, where an x multiplier needs it, psi_hat into D2:
+  // the plain Neumann step's two (the rate and psi_xx) are applied from the x stage itself
+  // (X_MUL2, below)
   const void* x_stage = h->A;
-  if (h->fp32) {
+  if (w0copy) {
     CHK(stage_to_f64(h, h->A, W0));
     x_stage = W0;
   }
+  const bool xmul2 = h->neumann && !h->cfg.advect;
+  const double xscale = h->neumann ? 0.5 / (double)h->cfg.nx : 1.0 / (double)h->cfg.nx;  // undo the x inverse
   {
     PParams<double> px = p;
-    px.scale = h->neumann ? 0.5 / (double)h->cfg.nx : 1.0 / (double)h->cfg.nx;  // undo the x inverse
-    CHK(xpass<double>(h, px, X_FWD_SCALED, x_stage, h->D2, nullptr, nullptr));
+    px.scale = xscale;
+    if (f32in) {
+      hipError_t e = xmul2 ? hipSuccess : surf_dct_xfwd_f32((const float*)x_stage, (double*)h->D2, px, t.qtwx, h->st);
+      if (e == hipSuccess)
+        e = surf_dct_yinv_f32((const float*)x_stage, (double*)h->D1, h->cfg.ny, h->npencils, t.twy, t.qtwy, h->st);
+      if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("surfTrack transforms: ") + hipGetErrorString(e));
+    } else {
+      if (!xmul2) CHK(xpass<double>(h, px, X_FWD_SCALED, x_stage, h->D2, nullptr, nullptr));
+      CHK(ypass<double>(h, x_stage, h->D1, +1));
+    }
   }
-  CHK(ypass<double>(h, x_stage, W1, +1));
-  CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, W1, nullptr, nullptr, f));
+  CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, f));
   // the crossing of every column (one wave per column) and the index its fields are taken at
   const unsigned sblocks = (unsigned)((nxy + 3) / 4), blocks = (unsigned)((nxy + 255) / 256);
   if (h->neumann)
@@ -2031,38 +2055,44 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
     hipLaunchKernelGGL(k_surf_scan<true>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol,
                        kev);
   HIPCHK(hipGetLastError());
-  // Derivative and rate fields, each evaluated only at the columns' evaluation indices: the x
-  // and y transforms of the whole field, then the z transform at one point per column.  Each
-  // multiplier is applied by the inverse pass that loads its input, at the stage where the
-  // transforms it does not depend on are already done: on W1 (x and y inverted) for one of
-  // kz, on W0 (x inverted) for one of ky, on the full spectrum D2 otherwise.
-  const double* kzt = h->t64.kz;
-  auto point = [&](const double* zspec, int kop, int slot) -> int {
+  // Derivative and rate fields, each evaluated only at the columns' evaluation indices.  A k_z
+  // multiplier goes on psi's own y inverse (still in D1) with the z transform at one point per
+  // column (k_surf_point); the others come from an x stage (x inverted), whose y inverse and
+  // point z transform are one kernel (surf_ypoint) that stores no y-inverted field, with a k_y
+  // multiplier applied there.  A multiplier that depends on k_x is applied by the x inverse of
+  // the full spectrum D2 into D1.
+  auto along_z = [&](int mode, int slot) -> int {
     const unsigned pblocks = (unsigned)((nxy + 63) / 64);
+    double* o = val + (size_t)slot * ns * nxy;
     if (h->neumann)
-      hipLaunchKernelGGL(k_surf_point<true>, dim3(pblocks), dim3(256), 0, h->st, zspec, kev, nxy, h->cfg.nz, kzt, kop,
-                         val + (size_t)slot * nxy);
+      hipLaunchKernelGGL(k_surf_point<true>, dim3(pblocks), dim3(256), 0, h->st, (const double*)h->D1, kev, nxy,
+                         h->cfg.nz, t.kz, mode, trig, ns, o);
     else
-      hipLaunchKernelGGL(k_surf_point<false>, dim3(pblocks), dim3(256), 0, h->st, zspec, kev, nxy, h->cfg.nz, kzt,
-                         kop, val + (size_t)slot * nxy);
+      hipLaunchKernelGGL(k_surf_point<false>, dim3(pblocks), dim3(256), 0, h->st, (const double*)h->D1, kev, nxy,
+                         h->cfg.nz, t.kz, mode, trig, ns, o);
     HIPCHK(hipGetLastError());
     return SMX_OK;
   };
-  auto along_z = [&](int mode, int slot) -> int { return point(W1, mode, slot); };
-  auto along_y = [&](int mode, int slot) -> int {
-    CHK(ypass<double>(h, x_stage, h->D1, +1, PassSched{}, false, mode));
-    return point((const double*)h->D1, -1, slot);
+  auto ypoint = [&](const void* xs, int ykop, int slot) -> int {
+    hipError_t e = surf_ypoint(h->neumann, xs == x_stage && f32in, xs, kev, h->cfg.nx, h->cfg.ny, h->cfg.nz, ns, t.twy, t.qtwy,
+                               ykop >= 0 ? t.ky : nullptr, ykop, t.kz, -1, trig, val + (size_t)slot * ns * nxy,
+                               h->st);
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("surfTrack field values: ") + hipGetErrorString(e));
+    return SMX_OK;
   };
-  auto x_then_y = [&](int mode, const void* spec) -> int {  // x inverse (with the multiplier), y inverse, in D1
+  auto x_inv = [&](int mode, const void* spec) -> int {  // x inverse (with the multiplier) into D1
     PParams<double> pk = p;
     pk.kop = mode;
-    CHK(xpass<double>(h, pk, X_INV, spec, h->D1, nullptr, nullptr));
-    return ypass<double>(h, h->D1, h->D1, +1);
+    return xpass<double>(h, pk, X_INV, spec, h->D1, nullptr, nullptr);
   };
+  auto along_y = [&](int mode, int slot) -> int { return ypoint(x_stage, mode, slot); };
   auto along_x = [&](int mode, int slot) -> int {
-    CHK(x_then_y(mode, h->D2));
-    return point((const double*)h->D1, -1, slot);
+    CHK(x_inv(mode, h->D2));
+    return ypoint(h->D1, -1, slot);
   };
+  // psi_zz (Neumann, cosNoAdvMorphCurv.cpp:750) or psi_z (periodic, surfTrack.cpp:611-626) first:
+  // the first x inverse overwrites psi's y inverse in D1
+  CHK(along_z(h->neumann ? KM_DZZ : KM_IKZ, h->neumann ? SV_DZZ : SV_GZ));
   // dpsi/dt = F^-1[aLin psi_hat + N_hat] (cosNoAdvMorphCurv.cpp:744,764; surfTrack.cpp:585-611).
   // Without advection N_hat is the transform of the real N(psi), so F^-1 N_hat = N(psi): the
   // rate field takes the linear part and the output kernel adds N(psi) at each crossing.  With
@@ -2070,8 +2100,18 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   if (h->cfg.advect) {
     CHK(advected_n_spectrum(h));
     CHK(kmul(h, KM_DTPSI, h->D2, h->D1));
-    CHK(x_then_y(-1, h->D1));
-    CHK(point((const double*)h->D1, -1, SV_RATE));
+    CHK(x_inv(-1, h->D1));
+    CHK(ypoint(h->D1, -1, SV_RATE));
+  } else if (xmul2) {
+    PParams<double> pm = p;
+    pm.scale = xscale;
+    pm.kop = KM_ALIN;
+    pm.kop2 = KM_DXX;
+    hipError_t e = f32in ? surf_dct_xmul2((const float*)x_stage, (double*)h->D1, (double*)h->D2, pm, t.qtwx, h->st)
+                         : surf_dct_xmul2((const double*)x_stage, (double*)h->D1, (double*)h->D2, pm, t.qtwx, h->st);
+    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("surfTrack x multipliers: ") + hipGetErrorString(e));
+    CHK(ypoint(h->D1, -1, SV_RATE));
+    CHK(ypoint(h->D2, -1, SV_DXX));
   } else {
     CHK(along_x(KM_ALIN, SV_RATE));
   }
@@ -2079,16 +2119,15 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
     // spectral gradient, psi_xx + psi_yy, psi_xx, psi_yy, psi_xy (surfTrack.cpp:611-626)
     CHK(along_x(KM_IKX, SV_GX));
     CHK(along_y(KM_IKY, SV_GY));
-    CHK(along_z(KM_IKZ, SV_GZ));
     CHK(along_x(KM_LAP_XY, SV_HNUM));
     CHK(along_x(KM_DXX, SV_DXX));
     CHK(along_y(KM_DYY, SV_DYY));
     CHK(along_x(KM_DXY, SV_DXY));
   } else {
-    // psi_xx, psi_yy, psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750)
-    CHK(along_x(KM_DXX, SV_DXX));
+    // psi_xx, psi_yy, psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750; psi_xx with the rate above
+    // unless advected)
+    if (!xmul2) CHK(along_x(KM_DXX, SV_DXX));
     CHK(along_y(KM_DYY, SV_DYY));
-    CHK(along_z(KM_DZZ, SV_DZZ));
   }
   HIPCHK(hipMemsetAsync(h->flag, 0, sizeof(unsigned int), h->st));
   SurfParams sp;
@@ -2106,6 +2145,7 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   sp.beta = p.beta;
   sp.gamma = p.gamma;
   sp.pen = p.pen;
+  sp.ns = ns;
   // each column's outputs (one thread per column)
   if (h->neumann)
     hipLaunchKernelGGL(k_surf_neumann, dim3(blocks), dim3(256), 0, h->st, f, val, kcol, kev, sp, h->surf, h->flag);

@@ -237,7 +237,21 @@ struct TileA {
 };
 template <typename T, int L, int E, int NCP> using Tile = TileA<T, L, E, RowAddr<NCP>>;
 
-template <typename T, int L, int E, typename AD, int SIGN, int Ns, int R, int... Rest>
+// Block barrier that orders LDS accesses only: global loads and stores in flight are not waited
+// for (a workgroup fence restricted to the local address space, as OpenCL's
+// barrier(CLK_LOCAL_MEM_FENCE)).  For a kernel whose threads exchange data through LDS alone, so
+// that a prefetch of the next tile's rows stays in flight across the transform's exchanges.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+template <bool LB> __device__ __forceinline__ void tile_barrier() {
+  if constexpr (LB) lds_barrier();
+  else __syncthreads();
+}
+
+template <bool LB, typename T, int L, int E, typename AD, int SIGN, int Ns, int R, int... Rest>
 __device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, int tc, const cplx<T>* __restrict__ tw) {
   constexpr int TPC = L / E;
   static_assert(E % R == 0, "E must be a multiple of every radix");
@@ -256,7 +270,7 @@ __device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, in
     dft<R, SIGN>(&v[u * R]);
   }
   if constexpr (sizeof...(Rest) > 0) {
-    __syncthreads();
+    tile_barrier<LB>();
 #pragma unroll
     for (int u = 0; u < E / R; ++u) {
       const int b = tc + TPC * u;
@@ -265,21 +279,22 @@ __device__ __forceinline__ void fft_passes(cplx<T>* v, cplx<T>* lds, int col, in
 #pragma unroll
       for (int r = 0; r < R; ++r) lds[AD::at(base + r * Ns, col)] = v[u * R + r];
     }
-    __syncthreads();
+    tile_barrier<LB>();
     constexpr int Rn = First<Rest...>::value;
     TileA<T, L, E, AD>::template load<Rn>(v, lds, col, tc);
-    fft_passes<T, L, E, AD, SIGN, Ns * R, Rest...>(v, lds, col, tc, tw);
+    fft_passes<LB, T, L, E, AD, SIGN, Ns * R, Rest...>(v, lds, col, tc, tw);
   }
 }
 
-template <typename T, int L, int E, typename AD, int SIGN, typename Rad> struct BlockFFTA;
-template <typename T, int L, int E, typename AD, int SIGN, int... Rs>
-struct BlockFFTA<T, L, E, AD, SIGN, Radices<Rs...>> {
+// LB: exchanges behind lds_barrier (else __syncthreads).
+template <typename T, int L, int E, typename AD, int SIGN, typename Rad, bool LB = false> struct BlockFFTA;
+template <typename T, int L, int E, typename AD, int SIGN, int... Rs, bool LB>
+struct BlockFFTA<T, L, E, AD, SIGN, Radices<Rs...>, LB> {
   // v: first-pass input positions on entry; last-pass output positions on exit.
   // The LDS tile is used for the exchanges; all threads must have finished any
   // earlier reads of it (the first exchange starts with a barrier).
   __device__ static __forceinline__ void run(cplx<T>* v, cplx<T>* lds, int col, int tc, const cplx<T>* __restrict__ tw) {
-    fft_passes<T, L, E, AD, SIGN, 1, Rs...>(v, lds, col, tc, tw);
+    fft_passes<LB, T, L, E, AD, SIGN, 1, Rs...>(v, lds, col, tc, tw);
   }
 };
 // forward (SIGN = -1) with the plan's Fwd radices, inverse (SIGN = +1) with its Inv radices
@@ -289,8 +304,8 @@ template <typename T, int L, int NCP, int V = 0>
 using InvFFT = BlockFFTA<T, L, PlanV<L, V>::E, RowAddr<NCP>, +1, typename PlanV<L, V>::Inv>;
 template <typename T, int L, typename AD, int V = 0>
 using FwdFFTA = BlockFFTA<T, L, PlanV<L, V>::E, AD, -1, typename PlanV<L, V>::Fwd>;
-template <typename T, int L, typename AD, int V = 0>
-using InvFFTA = BlockFFTA<T, L, PlanV<L, V>::E, AD, +1, typename PlanV<L, V>::Inv>;
+template <typename T, int L, typename AD, int V = 0, bool LB = false>
+using InvFFTA = BlockFFTA<T, L, PlanV<L, V>::E, AD, +1, typename PlanV<L, V>::Inv, LB>;
 template <int L, int V = 0> constexpr int fwd_first() { return RadInfo<typename PlanV<L, V>::Fwd>::first; }
 template <int L, int V = 0> constexpr int fwd_last() { return RadInfo<typename PlanV<L, V>::Fwd>::last; }
 template <int L, int V = 0> constexpr int inv_first() { return RadInfo<typename PlanV<L, V>::Inv>::first; }

@@ -405,6 +405,7 @@ struct SurfParams {
   // (F^-1 F N = N without advection); the model, beta, gamma and the mid-plane penalty of N
   int add_nl, nl_model, mid;
   double beta, gamma, pen;
+  int ns;  // kz splits of the field values (parts per slot)
 };
 
 // N(psi) at z index k of a column (pf3d.cpp:460-461; the cubic model; the mid-plane penalty
@@ -477,75 +478,16 @@ __global__ __launch_bounds__(256) void k_surf_scan(const double* __restrict__ ps
   }
 }
 
-// Value of a field at z index ke[c] of every column c that has a crossing, from its z
-// spectrum (layout [kz][x][y], x and y already inverted): the inverse z transform of one
-// entry per column in place of the whole field.  Neumann (REDFT01, cosNoAdvMorphCurv.cpp:750):
-//   y_n = m_0 x_0 + 2 sum_{k=1}^{N-1} m_k x_k cos(pi k (2n+1) / 2N);
-// periodic (C2R of the half spectrum, surfTrack.cpp:611-626):
-//   y_n = sum_{k=0}^{N/2} w_k Re(m_k x_k e^{2 pi i k n / N}),  w_0 = w_{N/2} = 1, else 2,
-// so that the imaginary parts at k = 0 and N/2 drop out as in an FFTW c2r.  m_k: the k_z
-// multiplier kop (KM_IKZ, KM_DZZ; < 0: 1).  The cosines come from an LDS table indexed by
-// k(2n+1) mod 4N (Neumann) or kn mod N (periodic), so every term is exact to rounding.
-// Block: 64 consecutive columns, one per lane (coalesced rows), and 4 waves that split the
-// k range; grid ceil(nxy / 64).
-template <bool NEU>
-__global__ __launch_bounds__(256) void k_surf_point(const double* __restrict__ spec, const int* __restrict__ ke,
-                                                    int64_t nxy, int nz, const double* __restrict__ kz, int kop,
-                                                    double* __restrict__ out) {
-  constexpr int kMaxN = 512;
-  __shared__ double ct[NEU ? 4 * kMaxN : kMaxN];
-  __shared__ double st[NEU ? 1 : kMaxN];
-  __shared__ double part[3][64];
-  __shared__ double kzs[kMaxN];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int period = NEU ? 4 * nz : nz;
-  for (int j = threadIdx.x; j < period; j += 256) {
-    double sv, cv;
-    sincospi(NEU ? (double)j / (2.0 * nz) : 2.0 * (double)j / nz, &sv, &cv);
-    ct[j] = cv;
-    if constexpr (!NEU) st[j] = sv;
-  }
-  if (kop >= 0)
-    for (int j = threadIdx.x; j < nz; j += 256) kzs[j] = kz[j];
-  __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
-  const int n = (c < nxy) ? ke[c] : -1;
-  const int nk = NEU ? nz : nz / 2 + 1;
-  const int k0 = w * nk / 4, k1 = (w + 1) * nk / 4;
-  double acc = 0.0;
-  if (n >= 0) {
-    const int step = NEU ? 2 * n + 1 : n;
-    int j = (int)(((int64_t)k0 * step) % period);
-    for (int k = k0; k < k1; ++k) {
-      if constexpr (NEU) {
-        double x = spec[(int64_t)k * nxy + c];
-        if (kop == KM_DZZ) x *= -kzs[k] * kzs[k];
-        acc += ((k == 0) ? 1.0 : 2.0) * x * ct[j];
-      } else {
-        const double* e = spec + 2 * ((int64_t)k * nxy + c);
-        double re = e[0], im = e[1];
-        if (kop == KM_IKZ) {
-          const double t = re;
-          re = -kzs[k] * im;
-          im = kzs[k] * t;
-        } else if (kop == KM_DZZ) {
-          re *= -kzs[k] * kzs[k];
-          im *= -kzs[k] * kzs[k];
-        }
-        acc += ((k == 0 || 2 * k == nz) ? 1.0 : 2.0) * (re * ct[j] - im * st[j]);
-      }
-      j += step;
-      if (j >= period) j -= period;
-    }
-  }
-  if (w > 0) part[w - 1][lane] = acc;
-  __syncthreads();
-  if (w == 0 && n >= 0) out[c] = acc + part[0][lane] + part[1][lane] + part[2][lane];
-}
-
-// Field values at each column's evaluation index (k_surf_point), nxy apiece.
+// Field values at each column's evaluation index: slot f holds ns parts of nxy entries (one per
+// kz split of the evaluation kernel), added in split order by sval.
 enum SurfVal { SV_RATE = 0, SV_DXX = 1, SV_DYY = 2, SV_DZZ = 3, SV_GX = 3, SV_GY = 4, SV_GZ = 5, SV_HNUM = 6,
                SV_DXY = 7 };
+__device__ __forceinline__ double sval(const double* val, const SurfParams& sp, int f, int64_t nxy, int64_t c) {
+  const double* v = val + (int64_t)f * sp.ns * nxy + c;
+  double s = v[0];
+  for (int i = 1; i < sp.ns; ++i) s += v[i * nxy];
+  return s;
+}
 
 // surfTrack.cpp:767-806 at the crossing k of k_surf_scan<true> (scan k upward, arm at
 // psi > arm, first psi < 0 after arming), with the derivatives at its evaluation index
@@ -561,7 +503,7 @@ __global__ __launch_bounds__(256) void k_surf_periodic(const double* __restrict_
   const int kc = kc_in[c];
   if (kc < 0) return;
   const int ke = ke_in[c];
-  auto V = [&](int f) { return val[f * nxy + c]; };
+  auto V = [&](int f) { return sval(val, sp, f, nxy, c); };
   const double gx = V(SV_GX), gy = V(SV_GY), gz = V(SV_GZ), dxx = V(SV_DXX), dyy = V(SV_DYY), dxy = V(SV_DXY);
   const double g = sqrt(gx * gx + gy * gy + gz * gz);
   const double rate = V(SV_RATE) + (sp.add_nl ? surf_nl(psi, sp, c * sp.nz, ke) : 0.0);
@@ -629,7 +571,8 @@ __global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__
     d_xz = 2.0 * (fd_x(psi, sp, i, j, k2) - fd_x(psi, sp, i, j, k2 - 1)) / sp.tdz;
     d_yz = 2.0 * (fd_y(psi, sp, i, j, k2) - fd_y(psi, sp, i, j, k2 - 1)) / sp.tdz;
   }
-  const double fxx = val[SV_DXX * nxy + c], fyy = val[SV_DYY * nxy + c], fzz = val[SV_DZZ * nxy + c];
+  const double fxx = sval(val, sp, SV_DXX, nxy, c), fyy = sval(val, sp, SV_DYY, nxy, c),
+               fzz = sval(val, sp, SV_DZZ, nxy, c);
   const double Hc = ((py * py + pz * pz) * fxx + (px * px + pz * pz) * fyy + (px * px + py * py) * fzz -
                      2.0 * (px * py * d_xy + px * pz * d_xz + py * pz * d_yz)) /
                     (g * g * g);
@@ -638,7 +581,7 @@ __global__ __launch_bounds__(256) void k_surf_neumann(const double* __restrict__
                      2.0 * (py * d_xy * (pz * d_xz - px * fzz) + px * d_xz * (py * d_yz - pz * fyy) +
                             pz * d_yz * (px * d_xy - py * fxx))) /
                     (g * g * g * g);
-  const double rate = val[SV_RATE * nxy + c] + (sp.add_nl ? surf_nl(psi, sp, col, k2) : 0.0);
+  const double rate = sval(val, sp, SV_RATE, nxy, c) + (sp.add_nl ? surf_nl(psi, sp, col, k2) : 0.0);
   out[c] = (double)k2;
   out[nxy + c] = sp.vsign * rate / g;
   out[2 * nxy + c] = Hc;

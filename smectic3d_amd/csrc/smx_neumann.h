@@ -71,8 +71,10 @@ __device__ __forceinline__ cplx<T> dct3_par(cplx<T> rk, cplx<T> rm, int k, int L
 // The grid covers whole tiles (nx*nz is a multiple of NC; the launcher checks it).
 // PAR: parity of the pencils along y (DP_DSIN: DIR +1 only, kq = k_y).  DP_COS with DIR +1 and
 // kq = k_y multiplies the cosine coefficients by -k_y^2 (the second derivative).
-template <typename T, int L, int DIR, int NC, int PAR = DP_COS>
-__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const T* in, T* out,
+// TI: the input's type (FP32 rows widened on load into an FP64 transform; surfTrack reads an FP32
+// stage this way).
+template <typename T, int L, int DIR, int NC, int PAR = DP_COS, typename TI = T>
+__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const TI* in, T* out,
                                                                            const cplx<T>* __restrict__ tw,
                                                                            const cplx<T>* __restrict__ qtw,
                                                                            const T* __restrict__ kq) {
@@ -92,14 +94,14 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
   if constexpr (PAR == DP_DSIN)
     for (int i = tid; i < L; i += NT) KQ[i] = kq[i];
   const int64_t pa = (int64_t)blockIdx.x * NC + 2 * q;  // pencils pa (column a), pa + 1 (column b)
-  const T* a = in + pa * L;
-  const T* b = a + L;
+  const TI* a = in + pa * L;
+  const TI* b = a + L;
   cplx<T> v[E];
   if constexpr (DIR < 0) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int s = mk_src(TL::template pos_in<fwd_first<L>()>(tc, e), L);
-      v[e] = mk(a[s], b[s]);
+      v[e] = mk((T)a[s], (T)b[s]);
     }
     FwdFFTA<T, L, AD>::run(v, S, q, tc, twl);
     __syncthreads();
@@ -119,7 +121,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = TL::template pos_in<inv_first<L>()>(tc, e);
-      v[e] = mk(a[k], b[k]);
+      v[e] = mk((T)a[k], (T)b[k]);
     }
     if constexpr (PAR == DP_COS) {
       if (kq != nullptr) {  // -k_y^2, after every load is issued
@@ -150,8 +152,9 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
 
 // ---------------------------------------------------------------------------
 // x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1 (pairs of adjacent ky), rows x.
-template <typename T, int L, int NC, int MODE>
-__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const T* in, T* out, T* __restrict__ rbuf,
+// TI: the input's type, as in k_dct_ypass (an FP32 stage into the FP64 X_FWD_SCALED of surfTrack).
+template <typename T, int L, int NC, int MODE, typename TI = T>
+__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const TI* in, T* out, T* __restrict__ rbuf,
                                                                            const T* __restrict__ psi0, PParams<T> p,
                                                                            const cplx<T>* __restrict__ qtw) {
   constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
@@ -169,7 +172,12 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
   const int kz = blockIdx.y, y0 = blockIdx.x * NC;
   const int64_t plane = (int64_t)kz * p.nxy + y0;
   const uint32_t rowb = (uint32_t)p.ny * sizeof(T);
-  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in + plane);
+  const cplx<TI>* src = reinterpret_cast<const cplx<TI>*>(in + plane);
+  const uint32_t rowbi = (uint32_t)p.ny * sizeof(TI);
+  auto ld_in = [&](int row, int col) -> cplx<T> {
+    const cplx<TI> w = ld_off(src, (uint32_t)row * rowbi + (uint32_t)col * (uint32_t)sizeof(cplx<TI>));
+    return mk((T)w.x, (T)w.y);
+  };
   cplx<T>* dst = reinterpret_cast<cplx<T>*>(out + plane);
   cplx<T> v[E];
   double nsum = 0.0;
@@ -183,8 +191,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
     // each entry is loaded once: a thread's own slots k stay in registers, the partners L-k
     // come from the tile
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      v[e] = ld_off(src, (uint32_t)Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e) * rowb + q * CB);
+    for (int e = 0; e < E; ++e) v[e] = ld_in(Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e), q);
     if (p.kop >= 0) {  // the k-space multiplier, columns a and b of the pair (after every load is issued)
       const T kya = p.ky[y0 + 2 * q], kyb = p.ky[y0 + 2 * q + 1], kzv = p.kz[kz];
       __syncthreads();  // the KX fill
@@ -205,13 +212,58 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
   } else {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int s = mk_src(Tile<T, L, E, FP>::template pos_in<fwd_first<L>()>(tc, e), L);
-      v[e] = ld_off(src, (uint32_t)s * rowb + q * CB);
+      v[e] = ld_in(mk_src(Tile<T, L, E, FP>::template pos_in<fwd_first<L>()>(tc, e), L), q);
     }
     FwdFFT<T, L, FP>::run(v, F, q, tc, twl);
     __syncthreads();
     Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
     __syncthreads();
+    if constexpr (MODE == X_MUL2) {
+      // Two multiplied inverse transforms of one forward transform: F^-1[m_kop F in] into out,
+      // F^-1[m_kop2 F in] into rbuf (p.scale undoes the round trip).  The scaled DCT-II outputs
+      // of a thread's items (k and L-k of its own pair, k <= L/2) stay in registers between the
+      // two; each inverse takes its DCT-III inputs through the tile, as X_STEP's does.
+      constexpr int NIT = (L / 2 + TPC) / TPC;
+      const T kzv = p.kz[kz], kya = p.ky[y0 + 2 * q], kyb = p.ky[y0 + 2 * q + 1];
+      cplx<T> ys[NIT][2];
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int k = tc + it * TPC, km = (L - k) % L;
+        const bool inr = k <= L / 2, two = inr && k != 0 && 2 * k != L;
+        ys[it][0] = inr ? p.scale * dct2_out(F[k * FP + q], F[km * FP + q], QW[k]) : mk<T>(0, 0);
+        ys[it][1] = two ? p.scale * dct2_out(F[km * FP + q], F[k * FP + q], QW[km]) : mk<T>(0, 0);
+      }
+      T* outs[2] = {out, rbuf};
+#pragma unroll
+      for (int pass = 0; pass < 2; ++pass) {
+        const int op = pass == 0 ? p.kop : p.kop2;
+        __syncthreads();  // every read of the tile (forward outputs, or the first inverse) is done
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+          const int k = tc + it * TPC;
+          if (k > L / 2) continue;
+          const int km = (L - k) % L;
+          const bool two = (k != 0 && 2 * k != L);
+          const T kx0 = KX[k], kx1 = KX[km];
+          const cplx<T> pk = mk(kfactor(p, op, kx0, kya, kzv) * ys[it][0].x, kfactor(p, op, kx0, kyb, kzv) * ys[it][0].y);
+          const cplx<T> pq = mk(kfactor(p, op, kx1, kya, kzv) * ys[it][1].x, kfactor(p, op, kx1, kyb, kzv) * ys[it][1].y);
+          // the partner of k = 0 is X_L = Y_L = 0; k = L/2 is its own partner
+          const cplx<T> pm = two ? pq : (k == 0 ? mk<T>(0, 0) : pk);
+          F[k * FP + q] = dct3_in(pk, pm, QW[k]);
+          if (two) F[km * FP + q] = dct3_in(pq, pk, QW[km]);
+        }
+        __syncthreads();
+        Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F, q, tc);
+        InvFFT<T, L, FP>::run(v, F, q, tc, twl);
+        cplx<T>* o = reinterpret_cast<cplx<T>*>(outs[pass] + plane);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int d = mk_src(Tile<T, L, E, FP>::template pos_in<inv_last<L>()>(tc, e), L);
+          st_off(o, (uint32_t)d * rowb + q * CB, v[e]);
+        }
+      }
+      return;
+    }
     // one item: the DCT-II outputs at k and L-k of pair qq, the k-space work there, and the
     // DCT-III inputs at k and L-k
     cplx<T>* rb = rbuf ? reinterpret_cast<cplx<T>*>(rbuf + plane) : nullptr;

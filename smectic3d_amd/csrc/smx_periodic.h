@@ -45,8 +45,9 @@ struct PParams {
   // layout [z][x][y].
   const T* mul_np = nullptr;
   // k-space multiplier (KMode, < 0: none) applied to the spectral input of the inverse x pass
-  // (X_INV) as it is loaded
+  // (X_INV) as it is loaded; the two multipliers of X_MUL2 (kop, kop2)
   int kop = -1;
+  int kop2 = -1;
 };
 
 // dN/dpsi: 3 beta psi^2 - 5 gamma psi^4 (quintic) or 2 beta psi - 3 gamma psi^2 (cubic).
@@ -157,7 +158,9 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
 // ---------------------------------------------------------------------------
 // x pass.  Tile: plane kz = by, columns y0 = bx*NC .. y0+NC-1, rows x (the body of k_xpass,
 // bx = blockIdx.x, by = blockIdx.y, and of the grouped k_xpass_grp).
-enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3 };
+// X_MUL2 (Neumann x pass only): F^-1[m_kop F in] into out and F^-1[m_kop2 F in] into rbuf from one
+// forward transform (surfTrack's x-dependent multipliers).
+enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3, X_MUL2 = 4 };
 
 // ORD (X_STEP only) orders the carry traffic.  0: R is read and rewritten element by
 // element between the two transforms.  1: all of R (and k_x) is read first, the new R is

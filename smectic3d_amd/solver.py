@@ -117,9 +117,19 @@ class Solver:
         check(lib().smx_psi_avg(self._h, int(nw), _ptr(out), self.cfg.nz))
         return out
 
-    def surftrack(self, arm, vel_sign=1.0):
+    def surftrack(self, arm, vel_sign=1.0, out=None):
+        """([z index, velocity, H, K], found): each an nx*ny float64 array over the columns, the
+        values of the last crossing found in each column (columns without one keep them).  out:
+        four such arrays to write in place of four new ones (a caller that tracks every few
+        steps skips their allocation and first-touch page faults)."""
         nx, ny, _ = self.shape
-        outs = [np.zeros(nx * ny) for _ in range(4)]
+        if out is None:
+            outs = [np.empty(nx * ny) for _ in range(4)]
+        else:
+            outs = list(out)
+            if len(outs) != 4 or not all(isinstance(o, np.ndarray) and o.dtype == np.float64 and
+                                         o.size == nx * ny and o.flags.c_contiguous for o in outs):
+                raise ValueError("out: four C-contiguous float64 arrays of nx*ny entries")
         found = C.c_int64()
         opts = _abi.SmxSurfOpts(arm, vel_sign)
         check(lib().smx_surftrack(self._h, C.byref(opts), *[_ptr(o) for o in outs], C.byref(found)))

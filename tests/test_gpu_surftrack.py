@@ -74,6 +74,14 @@ def test_surftrack_columns_without_crossing_keep_previous_values():
         (z2, v2, h2, k2), f2 = s.surftrack(0.7)
         assert f2 == 0
         np.testing.assert_array_equal(z2, z1)  # persistent values, as the reference's surf arrays
+        # the same values written into caller-owned arrays (out=), in place
+        bufs = np.full((4, nx * ny), 7.0)
+        outs, f3 = s.surftrack(0.7, out=bufs)
+        assert f3 == 0 and all(np.shares_memory(o, b) for o, b in zip(outs, bufs))
+        for b, ref in zip(bufs, (z2, v2, h2, k2)):
+            np.testing.assert_array_equal(b, ref)
+        with pytest.raises(ValueError):
+            s.surftrack(0.7, out=[np.zeros(nx * ny)] * 3)
 
 
 @pytest.mark.parametrize("bc,model,beta,w_mid,mid", [
@@ -105,11 +113,13 @@ def test_surftrack_rate_with_nonlinearity(bc, model, beta, w_mid, mid):
 
 @pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
 @pytest.mark.parametrize("bc", [A.SMX_BC_PERIODIC, A.SMX_BC_NEUMANN])
-@pytest.mark.parametrize("shape", [(32, 48, 24), (48, 32, 64), (16, 16, 96)])
+@pytest.mark.parametrize("shape", [(32, 48, 24), (48, 32, 64), (16, 16, 96), (16, 96, 48), (24, 128, 32)])
 def test_surftrack_shapes_and_precisions(shape, bc, prec):
-    """Every axis length role (the point evaluation splits nz/2+1 or nz coefficients over four
-    waves unevenly here) and both working precisions: the oracle evaluates the GPU's own
-    current field (get_psi), so an FP32-mixed state is compared on identical data."""
+    """Every axis length role and both working precisions.  The field values come from one
+    kernel per field that inverts y and sums the z transform at each column's index over kz
+    splits (up to 16 of them for a short x axis): here the splits and the rounds of pencils
+    within them come out uneven and the last round is partly idle.  The oracle evaluates the GPU's own current field
+    (get_psi), so an FP32-mixed state is compared on identical data."""
     nx, ny, nz = shape
     dx = 2 * math.pi / 8
     cfg = A.make_config(nx, ny, nz, bc=bc, nw=8, eps=0.5, beta=0.3, gamma=1.0, dt=0.05,
