@@ -298,10 +298,10 @@ struct BlockFFTA<T, L, E, AD, SIGN, Radices<Rs...>, LB> {
   }
 };
 // forward (SIGN = -1) with the plan's Fwd radices, inverse (SIGN = +1) with its Inv radices
-template <typename T, int L, int NCP, int V = 0>
-using FwdFFT = BlockFFTA<T, L, PlanV<L, V>::E, RowAddr<NCP>, -1, typename PlanV<L, V>::Fwd>;
-template <typename T, int L, int NCP, int V = 0>
-using InvFFT = BlockFFTA<T, L, PlanV<L, V>::E, RowAddr<NCP>, +1, typename PlanV<L, V>::Inv>;
+template <typename T, int L, int NCP, int V = 0, bool LB = false>
+using FwdFFT = BlockFFTA<T, L, PlanV<L, V>::E, RowAddr<NCP>, -1, typename PlanV<L, V>::Fwd, LB>;
+template <typename T, int L, int NCP, int V = 0, bool LB = false>
+using InvFFT = BlockFFTA<T, L, PlanV<L, V>::E, RowAddr<NCP>, +1, typename PlanV<L, V>::Inv, LB>;
 template <typename T, int L, typename AD, int V = 0>
 using FwdFFTA = BlockFFTA<T, L, PlanV<L, V>::E, AD, -1, typename PlanV<L, V>::Fwd>;
 template <typename T, int L, typename AD, int V = 0, bool LB = false>

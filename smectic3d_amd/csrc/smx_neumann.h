@@ -152,6 +152,9 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_ypass(const
 
 // ---------------------------------------------------------------------------
 // x pass.  Tile: plane kz = blockIdx.y, columns y0 .. y0+NC-1 (pairs of adjacent ky), rows x.
+// Its barriers order LDS only (lds_barrier): R stores issued before the inverse transform drain
+// while it runs (each R entry is read and written by one thread; the in-place rows are loaded and
+// consumed before the first exchange).
 // TI: the input's type, as in k_dct_ypass (an FP32 stage into the FP64 X_FWD_SCALED of surfTrack).
 template <typename T, int L, int NC, int MODE, typename TI = T>
 __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const TI* in, T* out, T* __restrict__ rbuf,
@@ -194,7 +197,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
     for (int e = 0; e < E; ++e) v[e] = ld_in(Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e), q);
     if (p.kop >= 0) {  // the k-space multiplier, columns a and b of the pair (after every load is issued)
       const T kya = p.ky[y0 + 2 * q], kyb = p.ky[y0 + 2 * q + 1], kzv = p.kz[kz];
-      __syncthreads();  // the KX fill
+      lds_barrier();  // the KX fill
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const T qx = KX[Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e)];
@@ -202,7 +205,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
       }
     }
     Tile<T, L, E, FP>::template store<inv_first<L>()>(v, F, q, tc);
-    __syncthreads();  // also orders the QW fill before its reads
+    lds_barrier();  // also orders the QW fill before its reads
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = Tile<T, L, E, FP>::template pos_in<inv_first<L>()>(tc, e);
@@ -214,10 +217,10 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
     for (int e = 0; e < E; ++e) {
       v[e] = ld_in(mk_src(Tile<T, L, E, FP>::template pos_in<fwd_first<L>()>(tc, e), L), q);
     }
-    FwdFFT<T, L, FP>::run(v, F, q, tc, twl);
-    __syncthreads();
+    FwdFFT<T, L, FP, 0, true>::run(v, F, q, tc, twl);
+    lds_barrier();
     Tile<T, L, E, FP>::template store<fwd_last<L>()>(v, F, q, tc);
-    __syncthreads();
+    lds_barrier();
     if constexpr (MODE == X_MUL2) {
       // Two multiplied inverse transforms of one forward transform: F^-1[m_kop F in] into out,
       // F^-1[m_kop2 F in] into rbuf (p.scale undoes the round trip).  The scaled DCT-II outputs
@@ -237,7 +240,7 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
 #pragma unroll
       for (int pass = 0; pass < 2; ++pass) {
         const int op = pass == 0 ? p.kop : p.kop2;
-        __syncthreads();  // every read of the tile (forward outputs, or the first inverse) is done
+        lds_barrier();  // every read of the tile (forward outputs, or the first inverse) is done
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
           const int k = tc + it * TPC;
@@ -252,9 +255,9 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
           F[k * FP + q] = dct3_in(pk, pm, QW[k]);
           if (two) F[km * FP + q] = dct3_in(pq, pk, QW[km]);
         }
-        __syncthreads();
+        lds_barrier();
         Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F, q, tc);
-        InvFFT<T, L, FP>::run(v, F, q, tc, twl);
+        InvFFT<T, L, FP, 0, true>::run(v, F, q, tc, twl);
         cplx<T>* o = reinterpret_cast<cplx<T>*>(outs[pass] + plane);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -355,10 +358,10 @@ __global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_xpass(const
       }
     }
     if constexpr (MODE == X_FWD_SCALED) return;
-    __syncthreads();
+    lds_barrier();
     Tile<T, L, E, FP>::template load<inv_first<L>()>(v, F, q, tc);
   }
-  InvFFT<T, L, FP>::run(v, F, q, tc, twl);
+  InvFFT<T, L, FP, 0, true>::run(v, F, q, tc, twl);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int d = mk_src(Tile<T, L, E, FP>::template pos_in<inv_last<L>()>(tc, e), L);

@@ -170,7 +170,11 @@ enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3, X_MUL2 = 4 };
 // loop issues no stores and keeps no extra registers.  A wave's loads and stores retire
 // through one in-order counter, so a load issued behind a store waits for that store;
 // which order hides more latency depends on the precision and tile shape, and the
-// autotuner measures all four.  k_x comes from an LDS copy in every order.
+// autotuner measures all four.  k_x comes from an LDS copy in every order.  The tile's
+// exchanges order LDS only (lds_barrier), so R stores issued before the inverse transform
+// drain while it runs: every global access of a thread is to entries no other thread of the
+// block reads or writes, except the in-place rows, whose loads are consumed before the first
+// exchange.
 template <typename T, int L, int NC, int MODE, int V, int ORD>
 __device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx<T>* __restrict__ rbuf,
                                            const cplx<T>* __restrict__ psi0, const PParams<T> p, int bx, int by) {
@@ -244,7 +248,7 @@ __device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx
   }
   cplx<T> cr[(ORD == 1 || ORD == 2) ? E : 1];  // ORD 1, 2: R (ORD 1: then the new R)
   if constexpr (MODE != X_INV) {
-    FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
+    FwdFFT<T, L, NCP, V, true>::run(v, S, col, tc, twl);
     const T kyv = p.ky[ky], kzv = p.kz[kz];
     if constexpr (ORD == 1 || ORD == 2) {
 #pragma unroll
@@ -254,7 +258,7 @@ __device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx
       }
     }
     constexpr bool kStage = (ORD == 3);  // every plane's R' through the LDS tile
-    if (herm || kStage) __syncthreads();  // S receives R' below; the FFT's last reads must be done
+    if (herm || kStage) lds_barrier();  // S receives R' below; the FFT's last reads must be done
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
@@ -309,7 +313,7 @@ __device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx
       }
     }
     if (herm) {  // R(k) <- (R(k) + conj R(-k))/2 with -k = (nx - kx, partner column)
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
@@ -332,7 +336,7 @@ __device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx
     for (int e = 0; e < E; ++e)
       st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
   } else {
-    InvFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
+    InvFFT<T, L, NCP, V, true>::run(v, S, col, tc, twl);
 #pragma unroll
     for (int e = 0; e < E; ++e)
       st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
