@@ -2045,16 +2045,27 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
       CHK(ypass<double>(h, x_stage, h->D1, +1));
     }
   }
-  CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, f));
-  // the crossing of every column (one wave per column) and the index its fields are taken at
-  const unsigned sblocks = (unsigned)((nxy + 3) / 4), blocks = (unsigned)((nxy + 255) / 256);
-  if (h->neumann)
-    hipLaunchKernelGGL(k_surf_scan<false>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol,
-                       kev);
-  else
-    hipLaunchKernelGGL(k_surf_scan<true>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol,
-                       kev);
-  HIPCHK(hipGetLastError());
+  // psi, the crossing of every column and the index its fields are taken at; a Neumann grid whose
+  // z blocks are whole waves also gets psi_zz there from the same pass (k_dct_zinv_scan)
+  bool zfused = false;
+  if (h->neumann) {
+    hipError_t e = surf_dct_zinv_scan((const double*)h->D1, f, nxy, h->cfg.nz, t.twz, t.qtwz, t.kz, trig, opts->arm,
+                                      kcol, kev, ns, val + (size_t)SV_DZZ * ns * nxy, h->st);
+    if (e == hipSuccess) zfused = true;
+    else if (e != hipErrorNotSupported) return fail(SMX_ERR_HIP, std::string("surfTrack z pass: ") + hipGetErrorString(e));
+  }
+  const unsigned blocks = (unsigned)((nxy + 255) / 256);
+  if (!zfused) {
+    CHK(zpass<double>(h, p, Z_IN_SPEC, Z_OUT_REAL, Z_OP_ID, h->D1, nullptr, nullptr, f));
+    const unsigned sblocks = (unsigned)((nxy + 3) / 4);  // one wave per column
+    if (h->neumann)
+      hipLaunchKernelGGL(k_surf_scan<false>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol,
+                         kev);
+    else
+      hipLaunchKernelGGL(k_surf_scan<true>, dim3(sblocks), dim3(256), 0, h->st, f, nxy, h->cfg.nz, opts->arm, kcol,
+                         kev);
+    HIPCHK(hipGetLastError());
+  }
   // Derivative and rate fields, each evaluated only at the columns' evaluation indices.  A k_z
   // multiplier goes on psi's own y inverse (still in D1) with the z transform at one point per
   // column (k_surf_point); the others come from an x stage (x inverted), whose y inverse and
@@ -2092,7 +2103,7 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   };
   // psi_zz (Neumann, cosNoAdvMorphCurv.cpp:750) or psi_z (periodic, surfTrack.cpp:611-626) first:
   // the first x inverse overwrites psi's y inverse in D1
-  CHK(along_z(h->neumann ? KM_DZZ : KM_IKZ, h->neumann ? SV_DZZ : SV_GZ));
+  if (!zfused) CHK(along_z(h->neumann ? KM_DZZ : KM_IKZ, h->neumann ? SV_DZZ : SV_GZ));
   // dpsi/dt = F^-1[aLin psi_hat + N_hat] (cosNoAdvMorphCurv.cpp:744,764; surfTrack.cpp:585-611).
   // Without advection N_hat is the transform of the real N(psi), so F^-1 N_hat = N(psi): the
   // rate field takes the linear part and the output kernel adds N(psi) at each crossing.  With

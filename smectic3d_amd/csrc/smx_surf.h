@@ -384,6 +384,140 @@ inline hipError_t surf_ypoint(bool neu, bool f32in, const void* in, const int* k
   return hipErrorInvalidValue;
 }
 
+// Neumann: psi, its crossings and psi_zz at them in one pass over the y-inverted stage `in`
+// (layout [kz][x][y], FP64).  The z inverse of each column as k_dct_zpass (Z_IN_SPEC ->
+// Z_OUT_REAL) writes psi (dense [x][y][z]); the column scan of k_surf_scan<false> (from the top,
+// armed above `arm`, first psi < 0) then runs on the block's tile, one wave per column; and psi_zz
+// at each column's evaluation index is summed from the spectral rows the thread loaded, kept in
+// registers (the sum of k_surf_point<true> with KM_DZZ): each thread adds its own rows, and the
+// TPC partial sums of a column pair are added in order.  dzz: ns parts of nxy (the sum in part 0,
+// zeros in the others).  The block's threads form whole waves (L >= 96 with 16 columns).
+template <int L, int NC>
+__global__ __launch_bounds__((NC / 2) * (L / Plan<L>::E)) void k_dct_zinv_scan(
+    const double* __restrict__ in, double* __restrict__ psi, int64_t nxy, const cplx<double>* __restrict__ twz,
+    const cplx<double>* __restrict__ qtw, const double* __restrict__ kzt, const double* __restrict__ trig,
+    double arm, int* __restrict__ kc_out, int* __restrict__ ke_out, int ns, double* __restrict__ dzz) {
+  using T = double;
+  constexpr int E = Plan<L>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
+  static_assert(NT % 64 == 0, "the scan takes one whole wave per column");
+  static_assert(2 * NT <= 2 * L * FP, "the partial sums fit in the tile");
+  constexpr uint32_t CB = sizeof(cplx<T>);
+  using TL = Tile<T, L, E, FP>;
+  constexpr int RP = fwd_first<L>();
+  static_assert(RP == inv_last<L>(), "plan symmetry");
+  __shared__ cplx<T> F[L * FP];
+  __shared__ cplx<T> TW[L];
+  __shared__ cplx<T> QW[L];
+  __shared__ int KE[NC];
+  const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
+  const cplx<T>* twl = tw_lds<L, NT>(TW, twz, tid);
+  tw_lds<L, NT>(QW, qtw, tid);
+  const int64_t f0 = (int64_t)blockIdx.x * NC;
+  const uint32_t rowb = (uint32_t)nxy * sizeof(T);
+  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in + f0);
+  cplx<T> v[E], sp[E];  // sp: the loaded spectral rows (pair q at kz = pos_in<inv_first>)
+#pragma unroll
+  for (int e = 0; e < E; ++e) sp[e] = v[e] = ld_off(src, (uint32_t)TL::template pos_in<inv_first<L>()>(tc, e) * rowb + q * CB);
+  TL::template store<inv_first<L>()>(v, F, q, tc);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = TL::template pos_in<inv_first<L>()>(tc, e);
+    const cplx<T> pn = (k == 0) ? mk<T>(0, 0) : F[(L - k) * FP + q];
+    v[e] = dct3_in(v[e], pn, QW[k]);
+  }
+  InvFFT<T, L, FP>::run(v, F, q, tc, twl);
+  // psi rows through the tile, each column's L values one contiguous row
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) F[mk_src(TL::template pos_in<RP>(tc, e), L) * FP + q] = v[e];
+  __syncthreads();
+  for (int i = tid; i < NPR * L; i += NT) {
+    const int qq = i / L, z = i - qq * L;
+    const cplx<T> w = F[z * FP + qq];
+    psi[(f0 + 2 * qq) * L + z] = w.x;
+    psi[(f0 + 2 * qq + 1) * L + z] = w.y;
+  }
+  // the scan: wave wv takes columns wv, wv + NT/64, ...
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int c = wv; c < NC; c += NT / 64) {
+    const int qq = c >> 1;
+    const bool hi = (c & 1) != 0;
+    auto P = [&](int z) -> double {
+      const cplx<T> w = F[z * FP + qq];
+      return hi ? w.y : w.x;
+    };
+    bool armed = false;
+    int kc = -1;
+    for (int ch = 0; ch < (L + 63) / 64; ++ch) {
+      const int k = L - 1 - ch * 64 - lane;
+      const bool inr = k >= 0;
+      const double val = inr ? P(k) : 0.0;
+      const uint64_t am = __ballot(inr && val > arm);
+      uint64_t neg = __ballot(inr && val < 0.0);
+      if (kc < 0) {
+        if (!armed && am) {
+          neg &= lanes_from(__ffsll((unsigned long long)am) - 1);
+          armed = true;
+        }
+        if (armed && neg) kc = L - 1 - ch * 64 - (__ffsll((unsigned long long)neg) - 1);
+      }
+    }
+    if (lane == 0) {
+      int ke = kc;
+      if (kc >= 0 && kc + 1 < L && fabs(P(kc)) > fabs(P(kc + 1))) ke = kc + 1;
+      kc_out[f0 + c] = kc;
+      ke_out[f0 + c] = ke;
+      KE[c] = ke;
+    }
+  }
+  __syncthreads();  // KE, and every read of the tile is done
+  // psi_zz at ke: y_n = sum_k w_k (-k_z^2) x_k cos(pi k (2n+1) / 2L), w_0 = 1, else 2
+  const int na = KE[2 * q], nb = KE[2 * q + 1];
+  const uint32_t sa = na >= 0 ? 2 * na + 1 : 0, sb = nb >= 0 ? 2 * nb + 1 : 0, P4 = 4 * L;
+  double aa = 0.0, ab = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = TL::template pos_in<inv_first<L>()>(tc, e);
+    const double w = ((k == 0) ? 1.0 : 2.0) * (-kzt[k] * kzt[k]);
+    aa += w * sp[e].x * trig[((uint32_t)k * sa) % P4];
+    ab += w * sp[e].y * trig[((uint32_t)k * sb) % P4];
+  }
+  double* R = reinterpret_cast<double*>(F);
+  R[2 * (tc * NPR + q)] = aa;
+  R[2 * (tc * NPR + q) + 1] = ab;
+  __syncthreads();
+  if (tid < NC) {  // column f0 + tid: pair tid / 2, half tid % 2
+    const int qq = tid >> 1, h = tid & 1;
+    double sum = 0.0;
+    for (int t = 0; t < TPC; ++t) sum += R[2 * (t * NPR + qq) + h];
+    dzz[f0 + tid] = sum;
+    for (int i = 1; i < ns; ++i) dzz[(int64_t)i * nxy + f0 + tid] = 0.0;
+  }
+}
+
+// Launch of k_dct_zinv_scan over nxy columns (16 per block); hipErrorNotSupported for a length
+// whose block is not whole waves (the caller then runs the z pass, k_surf_scan and k_surf_point).
+inline hipError_t surf_dct_zinv_scan(const double* in, double* psi, int64_t nxy, int nz, const cplx<double>* twz,
+                                     const cplx<double>* qtw, const double* kzt, const double* trig, double arm,
+                                     int* kc, int* ke, int ns, double* dzz, hipStream_t st) {
+  constexpr int NC = 16;
+  if (nxy % NC != 0) return hipErrorInvalidValue;
+#define SMX_CASE(L)                                                                                          \
+  if (nz == L) {                                                                                             \
+    if constexpr (((NC / 2) * (L / Plan<L>::E)) % 64 == 0) {                                                 \
+      SMX_LAUNCH((k_dct_zinv_scan<L, NC>), dim3((unsigned)(nxy / NC)), dim3((NC / 2) * (L / Plan<L>::E)), 0, st, \
+                 in, psi, nxy, twz, qtw, kzt, trig, arm, kc, ke, ns, dzz);                                   \
+      return hipGetLastError();                                                                              \
+    } else {                                                                                                 \
+      return hipErrorNotSupported;                                                                           \
+    }                                                                                                        \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
 // An FP32 Neumann stage read straight by surfTrack's FP64 passes (rows widened on load), so that
 // no FP64 copy of it is made: the scaled forward x transform into the FP64 spectrum (as
 // NeumannOps<double>::xpass X_FWD_SCALED) and the inverse y transform into FP64 (as
