@@ -114,14 +114,16 @@ def test_surftrack_rate_with_nonlinearity(bc, model, beta, w_mid, mid):
 @pytest.mark.parametrize("prec", [A.SMX_FP64, A.SMX_FP32_MIXED])
 @pytest.mark.parametrize("bc", [A.SMX_BC_PERIODIC, A.SMX_BC_NEUMANN])
 @pytest.mark.parametrize("shape", [(32, 48, 24), (48, 32, 64), (16, 16, 96), (16, 96, 48), (24, 128, 32),
-                                   (32, 16, 192), (16, 32, 128)])
+                                   (32, 16, 192), (16, 32, 128), (16, 384, 24), (384, 16, 16), (512, 16, 24),
+                                   (16, 512, 16), (16, 16, 512), (16, 16, 384)])
 def test_surftrack_shapes_and_precisions(shape, bc, prec):
     """Every axis length role and both working precisions.  The field values come from one
     kernel per field that inverts y and sums the z transform at each column's index over kz
     splits (up to 16 of them for a short x axis): here the splits and the rounds of pencils
     within them come out uneven and the last round is partly idle.  A Neumann grid with nz >= 96
     scans its columns and sums psi_zz inside the z inverse; a shorter one runs the separate scan
-    and point kernels: both forms are here.  The oracle evaluates the GPU's own current field
+    and point kernels: both forms are here, and every kernel at the longest axis lengths (384 and
+    512 along each axis).  The oracle evaluates the GPU's own current field
     (get_psi), so an FP32-mixed state is compared on identical data."""
     nx, ny, nz = shape
     dx = 2 * math.pi / 8
