@@ -314,7 +314,8 @@ static bool kop_valid(const smx_handle* h, int kop) {
 template <typename T>
 static int xpass(smx_handle* h, const PParams<T>& p, int mode, const void* in, void* out, void* r, const void* psi0,
                  PassSched s = PassSched{}, bool neu = false) {
-  if (!kop_valid(h, p.kop) || (p.kop >= 0 && mode != X_INV))
+  if (!kop_valid(h, p.kop) || (p.kop >= 0 && mode != X_INV && mode != X_MUL2) ||
+      (mode == X_MUL2 && (p.kop < 0 || p.kop2 < 0 || !kop_valid(h, p.kop2))))
     return fail(SMX_ERR_INVALID, "x pass: unsupported multiplier");
   hipError_t e = h->neumann ? NeumannOps<T>::xpass(mode, (const T*)in, (T*)out, (T*)r, (const T*)psi0, p,
                                                    tables<T>(h).qtwx, h->st, neu ? s.nc : 0)
@@ -2005,7 +2006,7 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
   // real psi, the FP64 x stage copy, the field values at each column's evaluation index (ns
   // parts per field), the z evaluation's trigonometric table, and the crossing and evaluation
   // indices of each column
-  const size_t nw0 = w0copy ? sdoubles : 0, nval = 8 * (size_t)ns * nxy, ntrig = 4 * (size_t)h->cfg.nz;
+  const size_t nw0 = w0copy ? sdoubles : 0, nval = 7 * (size_t)ns * nxy, ntrig = 4 * (size_t)h->cfg.nz;
   double* f = h->sfield;  // psi, dense [x][y][z]
   if (!f) {
     CHK(dalloc(h, (void**)&h->sfield,
@@ -2030,7 +2031,9 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
     CHK(stage_to_f64(h, h->A, W0));
     x_stage = W0;
   }
-  const bool xmul2 = h->neumann && !h->cfg.advect;
+  // without advection the x multipliers come from the x stage in pairs (X_MUL2) and psi_hat
+  // itself is not needed
+  const bool xmul2 = !h->cfg.advect;
   const double xscale = h->neumann ? 0.5 / (double)h->cfg.nx : 1.0 / (double)h->cfg.nx;  // undo the x inverse
   {
     PParams<double> px = p;
@@ -2097,6 +2100,22 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
     return xpass<double>(h, pk, X_INV, spec, h->D1, nullptr, nullptr);
   };
   auto along_y = [&](int mode, int slot) -> int { return ypoint(x_stage, mode, slot); };
+  // two x multipliers from the x stage in one pass (X_MUL2) into D1 and D2, then their values
+  auto x_pair = [&](int op1, int slot1, int op2, int slot2) -> int {
+    PParams<double> pm = p;
+    pm.scale = xscale;
+    pm.kop = op1;
+    pm.kop2 = op2;
+    if (h->neumann) {
+      hipError_t e = f32in ? surf_dct_xmul2((const float*)x_stage, (double*)h->D1, (double*)h->D2, pm, t.qtwx, h->st)
+                           : surf_dct_xmul2((const double*)x_stage, (double*)h->D1, (double*)h->D2, pm, t.qtwx, h->st);
+      if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("surfTrack x multipliers: ") + hipGetErrorString(e));
+    } else {
+      CHK(xpass<double>(h, pm, X_MUL2, x_stage, h->D1, h->D2, nullptr));
+    }
+    CHK(ypoint(h->D1, -1, slot1));
+    return ypoint(h->D2, -1, slot2);
+  };
   auto along_x = [&](int mode, int slot) -> int {
     CHK(x_inv(mode, h->D2));
     return ypoint(h->D1, -1, slot);
@@ -2113,27 +2132,21 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
     CHK(kmul(h, KM_DTPSI, h->D2, h->D1));
     CHK(x_inv(-1, h->D1));
     CHK(ypoint(h->D1, -1, SV_RATE));
-  } else if (xmul2) {
-    PParams<double> pm = p;
-    pm.scale = xscale;
-    pm.kop = KM_ALIN;
-    pm.kop2 = KM_DXX;
-    hipError_t e = f32in ? surf_dct_xmul2((const float*)x_stage, (double*)h->D1, (double*)h->D2, pm, t.qtwx, h->st)
-                         : surf_dct_xmul2((const double*)x_stage, (double*)h->D1, (double*)h->D2, pm, t.qtwx, h->st);
-    if (e != hipSuccess) return fail(SMX_ERR_HIP, std::string("surfTrack x multipliers: ") + hipGetErrorString(e));
-    CHK(ypoint(h->D1, -1, SV_RATE));
-    CHK(ypoint(h->D2, -1, SV_DXX));
   } else {
-    CHK(along_x(KM_ALIN, SV_RATE));
+    CHK(x_pair(KM_ALIN, SV_RATE, KM_DXX, SV_DXX));
   }
   if (!h->neumann) {
-    // spectral gradient, psi_xx + psi_yy, psi_xx, psi_yy, psi_xy (surfTrack.cpp:611-626)
-    CHK(along_x(KM_IKX, SV_GX));
+    // spectral gradient, psi_xx, psi_yy, psi_xy (surfTrack.cpp:611-626); psi_xx + psi_yy is the
+    // sum of the two at the crossing (k_surf_periodic)
+    if (xmul2) {
+      CHK(x_pair(KM_IKX, SV_GX, KM_DXY, SV_DXY));
+    } else {
+      CHK(along_x(KM_IKX, SV_GX));
+      CHK(along_x(KM_DXX, SV_DXX));
+      CHK(along_x(KM_DXY, SV_DXY));
+    }
     CHK(along_y(KM_IKY, SV_GY));
-    CHK(along_x(KM_LAP_XY, SV_HNUM));
-    CHK(along_x(KM_DXX, SV_DXX));
     CHK(along_y(KM_DYY, SV_DYY));
-    CHK(along_x(KM_DXY, SV_DXY));
   } else {
     // psi_xx, psi_yy, psi_zz (spectral, cosNoAdvMorphCurv.cpp:746-750; psi_xx with the rate above
     // unless advected)

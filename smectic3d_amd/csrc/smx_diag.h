@@ -480,8 +480,7 @@ __global__ __launch_bounds__(256) void k_surf_scan(const double* __restrict__ ps
 
 // Field values at each column's evaluation index: slot f holds ns parts of nxy entries (one per
 // kz split of the evaluation kernel), added in split order by sval.
-enum SurfVal { SV_RATE = 0, SV_DXX = 1, SV_DYY = 2, SV_DZZ = 3, SV_GX = 3, SV_GY = 4, SV_GZ = 5, SV_HNUM = 6,
-               SV_DXY = 7 };
+enum SurfVal { SV_RATE = 0, SV_DXX = 1, SV_DYY = 2, SV_DZZ = 3, SV_GX = 3, SV_GY = 4, SV_GZ = 5, SV_DXY = 6 };
 __device__ __forceinline__ double sval(const double* val, const SurfParams& sp, int f, int64_t nxy, int64_t c) {
   const double* v = val + (int64_t)f * sp.ns * nxy + c;
   double s = v[0];
@@ -491,7 +490,8 @@ __device__ __forceinline__ double sval(const double* val, const SurfParams& sp, 
 
 // surfTrack.cpp:767-806 at the crossing k of k_surf_scan<true> (scan k upward, arm at
 // psi > arm, first psi < 0 after arming), with the derivatives at its evaluation index
-// (val: rate, psi_xx, psi_yy, psi_x, psi_y, psi_z, psi_xx + psi_yy, psi_xy); reports k.
+// (val: rate, psi_xx, psi_yy, psi_x, psi_y, psi_z, psi_xy; the Laplacian numerator psi_xx + psi_yy
+// is their sum there); reports k.
 // One thread per column.
 __global__ __launch_bounds__(256) void k_surf_periodic(const double* __restrict__ psi, const double* __restrict__ val,
                                                        const int* __restrict__ kc_in, const int* __restrict__ ke_in,
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(256) void k_surf_periodic(const double* __restrict_
   const double rate = V(SV_RATE) + (sp.add_nl ? surf_nl(psi, sp, c * sp.nz, ke) : 0.0);
   out[c] = (double)kc;
   out[nxy + c] = sp.vsign * rate / g;
-  out[2 * nxy + c] = V(SV_HNUM) / g;
+  out[2 * nxy + c] = (dxx + dyy) / g;  // psi_xx + psi_yy (surfTrack.cpp:614)
   out[3 * nxy + c] = (dxx * dyy - dxy * dxy) / (g * g);
   atomicAdd(nfound, 1u);
 }

@@ -150,6 +150,12 @@ hipError_t PeriodicOps<T>::xpass(int mode, const cplx<T>* in, cplx<T>* out, cplx
       case X_FWD_SCALED: SMX_LAUNCH((k_xpass<T, L, kNC, X_FWD_SCALED>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
       case X_BOOT: SMX_LAUNCH((k_xpass<T, L, kNC, X_BOOT>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
       case X_INV: SMX_LAUNCH((k_xpass<T, L, kNC, X_INV>), grid, block, 0, st, in, out, rbuf, psi0, p); break; \
+      case X_MUL2:                                                                                    \
+        if constexpr (std::is_same<T, double>::value)                                                 \
+          SMX_LAUNCH((k_xpass<T, L, kNC, X_MUL2>), grid, block, 0, st, in, out, rbuf, psi0, p);      \
+        else                                                                                          \
+          return hipErrorInvalidValue;                                                                \
+        break;                                                                                        \
       default: return hipErrorInvalidValue;                                                           \
     }                                                                                                 \
     return hipGetLastError();                                                                         \

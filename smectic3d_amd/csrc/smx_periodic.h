@@ -158,8 +158,8 @@ __global__ __launch_bounds__(NC*(L / PlanV<L, V>::E)) void k_ypass(const cplx<T>
 // ---------------------------------------------------------------------------
 // x pass.  Tile: plane kz = by, columns y0 = bx*NC .. y0+NC-1, rows x (the body of k_xpass,
 // bx = blockIdx.x, by = blockIdx.y, and of the grouped k_xpass_grp).
-// X_MUL2 (Neumann x pass only): F^-1[m_kop F in] into out and F^-1[m_kop2 F in] into rbuf from one
-// forward transform (surfTrack's x-dependent multipliers).
+// X_MUL2: F^-1[m_kop F in] into out and F^-1[m_kop2 F in] into rbuf from one forward transform
+// (surfTrack's x-dependent multipliers; FP64).
 enum XMode { X_STEP = 0, X_FWD_SCALED = 1, X_BOOT = 2, X_INV = 3, X_MUL2 = 4 };
 
 // ORD (X_STEP only) orders the carry traffic.  0: R is read and rewritten element by
@@ -186,8 +186,8 @@ __device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx
   const int tid = threadIdx.x, col = tid % NC, tc = tid / NC;
   __shared__ cplx<T> TW[L];
   const cplx<T>* twl = tw_lds<L, NT>(TW, p.twx, tid);
-  __shared__ T KX[(MODE == X_STEP || MODE == X_BOOT || MODE == X_INV) ? L : 1];
-  if constexpr (MODE == X_STEP || MODE == X_BOOT) {
+  __shared__ T KX[(MODE == X_STEP || MODE == X_BOOT || MODE == X_INV || MODE == X_MUL2) ? L : 1];
+  if constexpr (MODE == X_STEP || MODE == X_BOOT || MODE == X_MUL2) {
     for (int i = tid; i < L; i += NT) KX[i] = p.kx[i];
     __syncthreads();  // a one-pass plan has no barrier of its own before the update reads KX
   } else if constexpr (MODE == X_INV) {
@@ -250,6 +250,27 @@ __device__ __forceinline__ void xpass_tile(const cplx<T>* in, cplx<T>* out, cplx
   if constexpr (MODE != X_INV) {
     FwdFFT<T, L, NCP, V, true>::run(v, S, col, tc, twl);
     const T kyv = p.ky[ky], kzv = p.kz[kz];
+    if constexpr (MODE == X_MUL2) {
+      // F^-1[m_kop F in] into out and F^-1[m_kop2 F in] into rbuf from this one forward
+      // transform (p.scale undoes the round trip), the scaled spectrum kept in registers between
+      // the two: the arithmetic of X_FWD_SCALED followed by X_INV with each multiplier
+      cplx<T> sp[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) sp[e] = p.scale * v[e];
+#pragma unroll
+      for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          v[e] = kapply(p, pass == 0 ? p.kop : p.kop2, KX[Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e)],
+                        kyv, kzv, sp[e]);
+        InvFFT<T, L, NCP, V, true>::run(v, S, col, tc, twl);
+        cplx<T>* dst = (pass == 0 ? out : rbuf) + plane;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          st_off(dst, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
+      }
+      return;
+    }
     if constexpr (ORD == 1 || ORD == 2) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
