@@ -2048,12 +2048,16 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
       CHK(ypass<double>(h, x_stage, h->D1, +1));
     }
   }
-  // psi, the crossing of every column and the index its fields are taken at; a Neumann grid whose
-  // z blocks are whole waves also gets psi_zz there from the same pass (k_dct_zinv_scan)
+  // psi, the crossing of every column and the index its fields are taken at; a grid whose z
+  // blocks are whole waves also gets psi_zz (Neumann) or psi_z (periodic) there from the same
+  // pass (k_dct_zinv_scan, k_zinv_scan)
   bool zfused = false;
-  if (h->neumann) {
-    hipError_t e = surf_dct_zinv_scan((const double*)h->D1, f, nxy, h->cfg.nz, t.twz, t.qtwz, t.kz, trig, opts->arm,
-                                      kcol, kev, ns, val + (size_t)SV_DZZ * ns * nxy, h->st);
+  {
+    hipError_t e =
+        h->neumann ? surf_dct_zinv_scan((const double*)h->D1, f, nxy, h->cfg.nz, t.twz, t.qtwz, t.kz, trig, opts->arm,
+                                        kcol, kev, ns, val + (size_t)SV_DZZ * ns * nxy, h->st)
+                   : surf_zinv_scan((const cplx<double>*)h->D1, f, nxy, h->cfg.nz, t.twz, t.kz, trig, opts->arm, kcol,
+                                    kev, ns, val + (size_t)SV_GZ * ns * nxy, h->st);
     if (e == hipSuccess) zfused = true;
     else if (e != hipErrorNotSupported) return fail(SMX_ERR_HIP, std::string("surfTrack z pass: ") + hipGetErrorString(e));
   }

@@ -518,6 +518,156 @@ inline hipError_t surf_dct_zinv_scan(const double* in, double* psi, int64_t nxy,
   return hipErrorInvalidValue;
 }
 
+// Periodic: psi, its crossings and psi_z at them in one pass over the y-inverted stage `in`
+// (complex, layout [kz][x][y], kz < nz/2 + 1).  The z inverse of each column pair as k_zpass
+// (Z_IN_SPEC -> Z_OUT_REAL: the Hermitian extension of the half spectrum, the imaginary parts of
+// kz = 0 and nz/2 dropped) writes psi (dense [x][y][z]); the column scan of k_surf_scan<true> (from
+// the bottom, armed above `arm`, first psi < 0) runs on the block's tile, one wave per column; and
+// psi_z at each column's evaluation index is summed from the raw rows the thread loaded, kept in
+// registers: each row kz <= nz/2 is the "low" entry (n = kz) of exactly one thread of the pair,
+//   y_n = sum_{k=0}^{nz/2} w_k Re(i k_z X_k e^{2 pi i k n / nz}),  w_0 = w_{nz/2} = 1, else 2
+// (the sum of k_surf_point<false> with KM_IKZ).  dz: ns parts of nxy (the sum in part 0, zeros in
+// the others).  The block's threads form whole waves.
+template <int L, int NC, int V>
+__global__ __launch_bounds__((NC / 2) * (L / PlanV<L, V>::E)) void k_zinv_scan(
+    const cplx<double>* __restrict__ in, double* __restrict__ psi, int64_t nxy, const cplx<double>* __restrict__ twz,
+    const double* __restrict__ kzt, const double* __restrict__ trig, double arm, int* __restrict__ kc_out,
+    int* __restrict__ ke_out, int ns, double* __restrict__ dz) {
+  using T = double;
+  constexpr int E = PlanV<L, V>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC, FP = NPR + 1;
+  static_assert(NT % 64 == 0, "the scan takes one whole wave per column");
+  static_assert(2 * NT <= 2 * L * FP, "the partial sums fit in the tile");
+  constexpr uint32_t CB = sizeof(cplx<T>);
+  using TL = Tile<T, L, E, FP>;
+  constexpr int RP = fwd_first<L, V>();
+  static_assert(RP == inv_last<L, V>(), "plan symmetry");
+  __shared__ cplx<T> F[L * FP];
+  __shared__ cplx<T> TW[L];
+  __shared__ int KE[NC];
+  const int tid = threadIdx.x, q = tid % NPR, tc = tid / NPR;
+  const cplx<T>* twl = tw_lds<L, NT>(TW, twz, tid);
+  const int64_t f0 = (int64_t)blockIdx.x * NC;
+  const uint32_t rowb = (uint32_t)nxy * CB, qofs = (uint32_t)q * 2 * CB;
+  const cplx2<T>* colp = reinterpret_cast<const cplx2<T>*>(in + f0);
+  const uint32_t tcr = (uint32_t)tc * rowb + qofs;
+  cplx2<T> ab[E];  // raw rows: entry e at n = tc + c_e reads row min(n, L - n)
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int c = TL::template pos_in<inv_first<L, V>()>(0, e);
+    const int n = tc + c;
+    const bool lo = (c + TPC - 1 <= L / 2) || (c <= L / 2 && n <= L / 2);  // n <= L/2
+    const uint32_t off = lo ? tcr + (uint32_t)c * rowb : (uint32_t)(L - c) * rowb - tcr + 2 * qofs;
+    ab[e] = ld_off(colp, off);
+  }
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = tc + TL::template pos_in<inv_first<L, V>()>(0, e);
+    cplx<T> A = ab[e].a, B = ab[e].b;
+    if (n == 0 || n == L / 2) {
+      A.y = 0;
+      B.y = 0;
+    }
+    v[e] = (n <= L / 2) ? mk(A.x - B.y, A.y + B.x) : mk(A.x + B.y, B.x - A.y);
+  }
+  InvFFT<T, L, FP, V>::run(v, F, q, tc, twl);
+  __syncthreads();
+  TL::template store<RP>(v, F, q, tc);
+  __syncthreads();
+  for (int i = tid; i < NPR * L; i += NT) {
+    const int qq = i / L, n = i - qq * L;
+    const cplx<T> w = F[n * FP + qq];
+    psi[(f0 + 2 * qq) * L + n] = w.x;
+    psi[(f0 + 2 * qq + 1) * L + n] = w.y;
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int c = wv; c < NC; c += NT / 64) {
+    const int qq = c >> 1;
+    const bool hi = (c & 1) != 0;
+    auto P = [&](int z) -> double {
+      const cplx<T> w = F[z * FP + qq];
+      return hi ? w.y : w.x;
+    };
+    bool armed = false;
+    int kc = -1;
+    for (int ch = 0; ch < (L + 63) / 64; ++ch) {
+      const int k = ch * 64 + lane;
+      const bool inr = k < L;
+      const double val = inr ? P(k) : 0.0;
+      const uint64_t am = __ballot(inr && val > arm);
+      uint64_t neg = __ballot(inr && val < 0.0);
+      if (kc < 0) {
+        if (!armed && am) {
+          neg &= lanes_from(__ffsll((unsigned long long)am) - 1);
+          armed = true;
+        }
+        if (armed && neg) kc = ch * 64 + (__ffsll((unsigned long long)neg) - 1);
+      }
+    }
+    if (lane == 0) {
+      int ke = kc;
+      if (kc >= 1 && fabs(P(kc)) > fabs(P(kc - 1))) ke = kc - 1;
+      kc_out[f0 + c] = kc;
+      ke_out[f0 + c] = ke;
+      KE[c] = ke;
+    }
+  }
+  __syncthreads();  // KE, and every read of the tile is done
+  const int na = KE[2 * q], nb = KE[2 * q + 1];
+  double aa = 0.0, bb = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = tc + TL::template pos_in<inv_first<L, V>()>(0, e);
+    if (n <= L / 2) {
+      const double w = (n == 0 || 2 * n == L) ? 1.0 : 2.0, k = kzt[n];
+      const cplx<T> A = ab[e].a, B = ab[e].b;
+      if (na >= 0) {
+        const uint32_t j = ((uint32_t)n * (uint32_t)na) % (uint32_t)L;
+        aa += w * ((-k * A.y) * trig[j] - (k * A.x) * trig[L + j]);
+      }
+      if (nb >= 0) {
+        const uint32_t j = ((uint32_t)n * (uint32_t)nb) % (uint32_t)L;
+        bb += w * ((-k * B.y) * trig[j] - (k * B.x) * trig[L + j]);
+      }
+    }
+  }
+  double* R = reinterpret_cast<double*>(F);
+  R[2 * (tc * NPR + q)] = aa;
+  R[2 * (tc * NPR + q) + 1] = bb;
+  __syncthreads();
+  if (tid < NC) {  // column f0 + tid: pair tid / 2, half tid % 2
+    const int qq = tid >> 1, h = tid & 1;
+    double sum = 0.0;
+    for (int t = 0; t < TPC; ++t) sum += R[2 * (t * NPR + qq) + h];
+    dz[f0 + tid] = sum;
+    for (int i = 1; i < ns; ++i) dz[(int64_t)i * nxy + f0 + tid] = 0.0;
+  }
+}
+
+// Launch of k_zinv_scan over nxy columns (16 per block, the E = 8 plan where there is one);
+// hipErrorNotSupported for a length whose block is not whole waves (the caller then runs the z
+// pass, k_surf_scan and k_surf_point).
+inline hipError_t surf_zinv_scan(const cplx<double>* in, double* psi, int64_t nxy, int nz, const cplx<double>* twz,
+                                 const double* kzt, const double* trig, double arm, int* kc, int* ke, int ns,
+                                 double* dz, hipStream_t st) {
+  constexpr int NC = 16;
+  if (nxy % NC != 0) return hipErrorInvalidValue;
+#define SMX_CASE(L)                                                                                       \
+  if (nz == L) {                                                                                          \
+    constexpr int V = HasPlanV<L, 1>::value ? 1 : 0;                                                      \
+    if constexpr (((NC / 2) * (L / PlanV<L, V>::E)) % 64 == 0) {                                          \
+      SMX_LAUNCH((k_zinv_scan<L, NC, V>), dim3((unsigned)(nxy / NC)), dim3((NC / 2) * (L / PlanV<L, V>::E)), 0, \
+                 st, in, psi, nxy, twz, kzt, trig, arm, kc, ke, ns, dz);                                  \
+      return hipGetLastError();                                                                           \
+    } else {                                                                                              \
+      return hipErrorNotSupported;                                                                        \
+    }                                                                                                     \
+  }
+  SMX_FOR_EACH_LENGTH(SMX_CASE)
+#undef SMX_CASE
+  return hipErrorInvalidValue;
+}
+
 // An FP32 Neumann stage read straight by surfTrack's FP64 passes (rows widened on load), so that
 // no FP64 copy of it is made: the scaled forward x transform into the FP64 spectrum (as
 // NeumannOps<double>::xpass X_FWD_SCALED) and the inverse y transform into FP64 (as

@@ -120,9 +120,9 @@ def test_surftrack_shapes_and_precisions(shape, bc, prec):
     """Every axis length role and both working precisions.  The field values come from one
     kernel per field that inverts y and sums the z transform at each column's index over kz
     splits (up to 16 of them for a short x axis): here the splits and the rounds of pencils
-    within them come out uneven and the last round is partly idle.  A Neumann grid with nz >= 96
-    scans its columns and sums psi_zz inside the z inverse; a shorter one runs the separate scan
-    and point kernels: both forms are here, and every kernel at the longest axis lengths (384 and
+    within them come out uneven and the last round is partly idle.  A grid whose z blocks are whole
+    waves (Neumann nz >= 96, periodic nz >= 64) scans its columns and sums psi_zz or psi_z inside
+    the z inverse; a shorter one runs the separate scan and point kernels: both forms are here, and every kernel at the longest axis lengths (384 and
     512 along each axis).  The oracle evaluates the GPU's own current field
     (get_psi), so an FP32-mixed state is compared on identical data."""
     nx, ny, nz = shape
