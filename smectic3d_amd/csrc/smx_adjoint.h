@@ -59,56 +59,61 @@ __device__ __forceinline__ void xpass_adj_tile(cplx<T>* stage, cplx<T>* __restri
   const uint32_t cofs = (uint32_t)ky * CB;
   cplx<T>* sb = stage + plane;
   cplx<T>* cb = carry + plane;
-  cplx<T> v[E];
+  const bool sums = psin != nullptr && acc != nullptr;
+  // Every read is issued first (the stage rows, then alpha_{n+1} and Psi_n at the entries the
+  // forward transform leaves in this thread) and every store comes last (the new stage, then
+  // alpha_n from registers): a wave's loads and stores retire in order, and the exchanges order
+  // LDS only (lds_barrier), so no barrier waits for a store and the carry reads overlap the
+  // forward transform.  In place: each stage row is consumed before the first exchange, each
+  // carry entry is read and written by this thread (or its partner value comes through the tile).
+  cplx<T> v[E], a1r[E], s0r[E];
 #pragma unroll
   for (int e = 0; e < E; ++e)
     v[e] = ld_off<cplx<T>>(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_first<L, V>()>(tc, e) * rowb + cofs);
-  FwdFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t o = (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs;
+    a1r[e] = init ? mk<T>(0, 0) : ld_off<cplx<T>>(cb, o);
+    s0r[e] = sums ? ld_off(psin + plane, o) : mk<T>(0, 0);
+  }
+  FwdFFT<T, L, NCP, V, true>::run(v, S, col, tc, twl);
   const T kyv = p.ky[ky], kzv = p.kz[kz];
   const double wk = herm ? 1.0 : 2.0;
-  const bool sums = psin != nullptr && acc != nullptr;
   double ge = 0.0, gq = 0.0;
-  // alpha_n goes to the LDS tile first and to HBM after the loop, so that the loop's reads
-  // (carry, Psi_n) are not issued behind writes (a wave's loads and stores retire in order)
-  __syncthreads();  // S receives alpha_n below; the transform's last reads must be done
+  cplx<T> a0r[E];
+  lds_barrier();  // S receives alpha_n below (self-conjugate planes); the transform's last reads must be done
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
-    const uint32_t o = (uint32_t)kxi * rowb + cofs;
     const T kxv = KX[kxi];
     const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
     const T d = k2 - p.q02;
     const T aLin = p.eps - p.alpha * d * d;
     const T C1 = (T)1 + p.dtd2 * aLin, C2 = (T)1 - p.dtd2 * aLin;
     const T ic2 = recip(C2);
-    const cplx<T> a1 = init ? mk<T>(0, 0) : ld_off<cplx<T>>(cb, o);
+    const cplx<T> a1 = a1r[e];
     const T g = C1 * ic2;
     const cplx<T> a0 = mk(g * a1.x + p.scale * v[e].x, g * a1.y + p.scale * v[e].y);
     if (sums) {  // per-element terms in the working precision, per-thread sums in FP64
-      const cplx<T> s0 = ld_off(psin + plane, o);
+      const cplx<T> s0 = s0r[e];
       T re = a1.x * s0.x + a1.y * s0.y;
       if (tail) re += a0.x * s0.x + a0.y * s0.y;
       const T base = re * p.dtd2 * ic2;
       ge += (double)base;
       gq += (double)(base * (k2 - p.q02));
     }
-    S[kxi * NCP + col] = a0;
+    if (herm) S[kxi * NCP + col] = a0;
+    a0r[e] = a0;
     const T m = p.dtd2 * ic2;
     v[e] = mk(m * (cnext * a0.x - a1.x), m * (cnext * a0.y - a1.y));
   }
   if (herm) {  // alpha(k) <- (alpha(k) + conj alpha(-k))/2
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
       const cplx<T> a = S[kxi * NCP + col], b = S[((L - kxi) % L) * NCP + pcol];
-      st_off(cb, (uint32_t)kxi * rowb + cofs, mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)));
-    }
-  } else {  // each thread reads back its own entries
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int kxi = Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e);
-      st_off(cb, (uint32_t)kxi * rowb + cofs, S[kxi * NCP + col]);
+      a0r[e] = mk((T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y));
     }
   }
   double be = 0.0, bq = 0.0;
@@ -116,10 +121,13 @@ __device__ __forceinline__ void xpass_adj_tile(cplx<T>* stage, cplx<T>* __restri
     be = block_sum(ge, red);
     bq = block_sum(gq, red);
   }
-  InvFFT<T, L, NCP, V>::run(v, S, col, tc, twl);
+  InvFFT<T, L, NCP, V, true>::run(v, S, col, tc, twl);
 #pragma unroll
   for (int e = 0; e < E; ++e)
     st_off(sb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<inv_last<L, V>()>(tc, e) * rowb + cofs, v[e]);
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    st_off(cb, (uint32_t)Tile<T, L, E, NCP>::template pos_in<fwd_last<L, V>()>(tc, e) * rowb + cofs, a0r[e]);
   if (sums && tid == 0) {  // the block's accumulators, read and written last
     const int b = by * gx + bx;
     acc[2 * b] += wk * be;
