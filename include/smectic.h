@@ -188,9 +188,9 @@ int smx_psi_avg(smx_handle *h, int nw, double *out, int64_t row_stride);
  * handle last stored for it (zero-initialised).  *found = number of columns
  * that crossed on this call.  The derivative fields are evaluated only at each
  * column's crossing (x and y transforms of the whole field, the z transform at one
- * point); the first call allocates the scratch (real psi, two FP64 stage arrays,
- * 8 values and 2 indices per column) and a mapped pinned host buffer of 4*nx*ny+1
- * doubles, both kept by the handle.  Synchronous. */
+ * point); the first call allocates the scratch (real psi, an FP64 copy of a periodic
+ * FP32 stage, up to 112 values and 2 indices per column, a 4*nz table) and a mapped
+ * pinned host buffer of 4*nx*ny+1 doubles, both kept by the handle.  Synchronous. */
 typedef struct smx_surf_opts {
   double arm;      /* arming threshold: 0.1 (surfTrack.cpp:775) or 0.7 (cosNoAdvMorphCurv.cpp:889) */
   double vel_sign; /* +1: v = +dtpsi/|grad psi| as in the reference; -1: v = -dtpsi/|grad psi| */

@@ -2074,11 +2074,11 @@ int smx_surftrack(smx_handle* h, const smx_surf_opts* opts, double* surfz, doubl
     HIPCHK(hipGetLastError());
   }
   // Derivative and rate fields, each evaluated only at the columns' evaluation indices.  A k_z
-  // multiplier goes on psi's own y inverse (still in D1) with the z transform at one point per
-  // column (k_surf_point); the others come from an x stage (x inverted), whose y inverse and
-  // point z transform are one kernel (surf_ypoint) that stores no y-inverted field, with a k_y
-  // multiplier applied there.  A multiplier that depends on k_x is applied by the x inverse of
-  // the full spectrum D2 into D1.
+  // multiplier goes on psi's own y inverse (in the fused z pass above, or k_surf_point on D1);
+  // the others come from an x stage (x inverted), whose y inverse and point z transform are one
+  // kernel (surf_ypoint) that stores no y-inverted field, with a k_y multiplier applied there.
+  // Multipliers that depend on k_x come in pairs from the x stage (X_MUL2 into D1 and D2), or,
+  // with advection, from the x inverse of the full spectrum D2 into D1.
   auto along_z = [&](int mode, int slot) -> int {
     const unsigned pblocks = (unsigned)((nxy + 63) / 64);
     double* o = val + (size_t)slot * ns * nxy;
