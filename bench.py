@@ -208,7 +208,10 @@ def run_neumann(steps, surf_every=20, compare=True):
         out[key] = {"steps_per_s_with_surftrack": steps / (t1 - t0),
                     "columns_found_per_call": found / max(1, steps // surf_every), "schedule": sched}
     r = out["fp32"]
+    # the calls' share of the run: the separately timed call cost times the calls, over the run
+    run_s = steps / r["steps_per_s_with_surftrack"]
     r.update({"steps": steps, "surftrack_every": surf_every, "surftrack_ms_per_call": 1e3 * t_surf,
+              "surftrack_share_of_run": ncalls * t_surf / run_s,
               "kernels": [{"name": k, "avg_ms": m, "GBs": b / (m * 1e-3) / 1e9} for k, m, b in ks]})
     if compare:
         v32, h32, p32 = samples[A.SMX_FP32_MIXED]
