@@ -1,8 +1,14 @@
-// smx_surf.h -- surfTrack's field values at one z index per column, evaluated straight from an
-// x stage (x inverted, y and z spectral): the inverse y transform of every pencil in LDS, then
-// each output entry's term of the inverse z transform at its column's evaluation index, summed
-// over kz in registers.  The y-inverted field is never written; a call reads its input once
-// and writes nxy doubles per kz split.
+// smx_surf.h -- surfTrack's kernels beyond the step's passes (all FP64 arithmetic):
+//   k_dct_ypoint / k_ypoint   a field's value at one z index per column, straight from an x stage
+//                             (x inverted, y and z spectral): the inverse y transform of every
+//                             pencil in LDS, then each output entry's term of the inverse z
+//                             transform at its column's index, summed over kz in registers; the
+//                             y-inverted field is never written (one read, nxy doubles per split)
+//   k_surf_point              the same sum from a y-inverted stage
+//   k_dct_zinv_scan / k_zinv_scan   psi's z inverse, the column scan and the k_z derivative at
+//                             each column's index in one pass
+//   surf_dct_xfwd_f32, surf_dct_yinv_f32, surf_dct_xmul2   Neumann x and y passes reading an
+//                             FP32 stage (widened on load) or taking two x multipliers at once
 #pragma once
 #include <algorithm>
 
