@@ -233,6 +233,35 @@ def advected_cfg(precision):
                          precision=precision)
 
 
+def run_surftrack_periodic(calls=20):
+    """Periodic surfTrack at 256^3 FP32-mixed, without advection (the field of the headline
+    workload after 20 steps) and with it (the advected workload's noise start after 20 steps):
+    milliseconds per call, the outputs written into caller-owned arrays."""
+    from smectic3d_amd import Solver
+    from smectic3d_amd import _abi as A
+    n = 256
+    bufs = np.empty((4, n * n))
+    bufs.fill(0.0)
+    out = {}
+    for name, cfg, psi0, arm in (
+            ("plain", workload_cfg(A.SMX_FP32_MIXED), hyperboloid_ic(n, 2 * math.pi / 8), 0.1),
+            ("advected", advected_cfg(A.SMX_FP32_MIXED), np.random.default_rng(4).uniform(-0.05, 0.05, (n, n, n)),
+             0.02)):
+        with Solver(cfg) as s:
+            s.set_psi(psi0)
+            s.set_l1_tracking(False)
+            s.step(20)
+            s.sync()
+            for _ in range(3):
+                s.surftrack(arm, 1.0, out=bufs)
+            t0 = time.perf_counter()
+            found = 0
+            for _ in range(calls):
+                found += s.surftrack(arm, 1.0, out=bufs)[1]
+            out[name] = {"ms_per_call": 1e3 * (time.perf_counter() - t0) / calls, "columns_found_per_call": found / calls}
+    return {"grid": [n, n, n], "precision": "fp32_mixed", "calls": calls, **out}
+
+
 def run_advected(steps, compare=True):
     """Advection-coupled periodic 256^3: mu grad psi force, spectral Stokes (eta=1), eps=0.5, q0=1,
     gamma=1, dt=0.02, psi0 ~ U[-0.05, 0.05]; FP32-mixed, with the same run in FP64 for its error."""
@@ -505,7 +534,7 @@ def main():
                     help="install the schedules of profiles/periodic_256_schedules.json instead of autotuning")
     ap.add_argument("--no-tune", action="store_true", help="default tile schedules (no autotuning)")
     ap.add_argument("--leg", choices=["neumann", "advected", "neumann_advected", "batch", "batch_adjoint",
-                                      "adjoint", "accuracy"],
+                                      "adjoint", "accuracy", "surftrack_periodic"],
                     help="run only this secondary workload (with --leg-steps steps) and print its result")
     ap.add_argument("--leg-steps", type=int, default=None)
     ap.add_argument("--schedules-from", default=None,
@@ -541,7 +570,8 @@ def main():
                 "batch": (lambda k: run_batch(k, tune=not a.no_tune), 500),
                 "batch_adjoint": (lambda k: run_batch_adjoint(k, 50), 500),
                 "adjoint": (lambda k: run_adjoint(k, 50), 2000),
-                "accuracy": (lambda k: run_accuracy(hyperboloid_ic(n, dx), k, f32, f64), 2000)}
+                "accuracy": (lambda k: run_accuracy(hyperboloid_ic(n, dx), k, f32, f64), 2000),
+                "surftrack_periodic": (run_surftrack_periodic, 20)}
         fn, default_steps = runs[a.leg]
         print(json.dumps({"leg": a.leg, "result": fn(a.leg_steps or default_steps)}, default=str))
         return
