@@ -138,3 +138,35 @@ def test_surftrack_shapes_and_precisions(shape, bc, prec):
         orc = O.Oracle(cfg)
         orc.set_psi(s.get_psi())
         _compare(s, orc, arm, 1.0)
+
+
+@pytest.mark.parametrize("bc", [A.SMX_BC_NEUMANN, A.SMX_BC_PERIODIC])
+def test_surftrack_at_workload_size(bc):
+    """The surfTrack workloads' own grids and precision: Neumann 256x256x384 and periodic 256^3,
+    FP32-mixed, from the layered start (200 and 20 steps: most columns cross), every column
+    against the oracle evaluating the GPU's own field (the evaluation kernels' splits, tables and
+    fused z inverse at full size)."""
+    dx = 2 * math.pi / 8
+    if bc == A.SMX_BC_NEUMANN:
+        nx, ny, nz = 256, 256, 384
+        cfg = A.make_config(nx, ny, nz, lx=nx * dx, ly=ny * dx, lz=nz * dx, bc=bc, eps=0.5, q0=1.0, beta=0.0,
+                            gamma=1.0, dt=0.05, nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED)
+        arm, vsign, steps = 0.7, -1.0, 200
+    else:
+        nx = ny = nz = 256
+        cfg = A.make_config(nx, lx=nx * dx, ly=ny * dx, lz=nz * dx, eps=0.3, q0=1.0, beta=0.0, gamma=1.0, dt=0.1,
+                            nl_model=A.SMX_NL_CUBIC, precision=A.SMX_FP32_MIXED)
+        arm, vsign, steps = 0.1, 1.0, 20
+    c = (np.arange(nx) - nx // 2) * dx
+    cz = (np.arange(nz) - nz // 2) * dx
+    r2 = np.maximum(c[:, None, None] ** 2 + c[None, :, None] ** 2 - cz[None, None, :] ** 2, 0.0)
+    psi0 = 0.4 * np.cos(np.sqrt(r2 + 4.0))
+    say("surftrack at workload size", bc)
+    with Solver(cfg) as s:
+        s.set_psi(psi0)
+        s.step(steps)
+        orc = O.Oracle(cfg)
+        orc.set_psi(s.get_psi())
+        (_, _, _, _), found = s.surftrack(arm, vsign)
+        assert found > nx * ny // 2
+        _compare(s, orc, arm, vsign)
