@@ -331,7 +331,9 @@ int smx_profile_step(smx_handle *h, int64_t nsteps, int max_slots, int *nslots, 
 
 /* Plain device copy kernel over nbytes (grid-stride float4 copy of hashed,
  * non-zero words), used to calibrate hardware byte counters against a kernel of
- * known traffic and as the achievable-bandwidth yardstick. */
+ * known traffic and as the achievable-bandwidth yardstick: avg_ms is the
+ * launch time of the fastest of three launch shapes (2048 and 8192 blocks of
+ * 256 threads, and one float4 per thread). */
 int smx_calibration_copy(smx_handle *h, int64_t nbytes, int reps, double *avg_ms);
 
 #ifdef __cplusplus

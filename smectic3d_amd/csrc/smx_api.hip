@@ -2602,15 +2602,24 @@ int smx_calibration_copy(smx_handle* h, int64_t nbytes, int reps, double* avg_ms
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  hipLaunchKernelGGL(k_copy_f4, dim3(2048), dim3(256), 0, h->st, (const float4*)a, (float4*)b, n4);
-  (void)hipEventRecord(e0, h->st);
-  for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(k_copy_f4, dim3(2048), dim3(256), 0, h->st, (const float4*)a, (float4*)b, n4);
-  (void)hipEventRecord(e1, h->st);
-  (void)hipEventSynchronize(e1);
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  *avg_ms = ms / reps;
+  // the fastest of three launch shapes: 2048 and 8192 blocks of 256 threads, and one float4
+  // per thread (on MI355X at 67.6 MB: 6.50, 6.89 and 6.82 TB/s), so the yardstick is the
+  // best plain copy
+  const int64_t one = (n4 + 255) / 256;
+  const int64_t grids[3] = {std::min<int64_t>(one, 2048), std::min<int64_t>(one, 8192), std::min<int64_t>(one, 1 << 30)};
+  float best = 0.f;
+  for (int64_t g : grids) {
+    hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)g), dim3(256), 0, h->st, (const float4*)a, (float4*)b, n4);
+    (void)hipEventRecord(e0, h->st);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)g), dim3(256), 0, h->st, (const float4*)a, (float4*)b, n4);
+    (void)hipEventRecord(e1, h->st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (best == 0.f || ms < best) best = ms;
+  }
+  *avg_ms = best / reps;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(a);
