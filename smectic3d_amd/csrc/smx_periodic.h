@@ -391,12 +391,15 @@ __device__ __forceinline__ void zpass_tile(const cplx<T>* in, cplx<T>* out, cons
   // The grid covers whole tiles: nx*ny is a multiple of NC (every plan length is a
   // multiple of 8 and the launcher checks it).
   constexpr int E = PlanV<L, V>::E, TPC = L / E, NPR = NC / 2, NT = NPR * TPC;
-  // Tile row pitch: rows of exactly 128 bytes (8 pairs in FP64, 16 in FP32) unpadded, others
-  // padded by one element.  Measured on the FP64 z pass at 256^3: the 144-byte padded pitch took
-  // three times the LDS bank-conflict cycles (3.8 M against 1.2 M per launch) and, at 42 KB of
-  // LDS per block, one block per CU fewer (46.7 against 44.7 us).  (The x pass is no faster
-  // unpadded.)
-  constexpr int NZH = L / 2 + 1, FP = NPR * sizeof(cplx<T>) == 128 ? NPR : NPR + 1;
+  // Tile row pitch: in the spectrum-to-spectrum form (the step's), rows of exactly 128 bytes
+  // (8 pairs in FP64, 16 in FP32) unpadded, others padded by one element.  Measured on the FP64 z
+  // pass at 256^3: the 144-byte padded pitch took three times the LDS bank-conflict cycles (3.8 M
+  // against 1.2 M per launch) and, at 42 KB of LDS per block, one block per CU fewer (46.7 against
+  // 44.7 us).  (The x pass is no faster unpadded.)  The real-field forms keep the pad: their
+  // lanes run down consecutive rows of one column, which an unpadded 128-byte pitch puts in
+  // the same banks.
+  constexpr bool UNPAD = IN == Z_IN_SPEC && OUT == Z_OUT_SPEC && NPR * sizeof(cplx<T>) == 128;
+  constexpr int NZH = L / 2 + 1, FP = UNPAD ? NPR : NPR + 1;
   constexpr uint32_t CB = sizeof(cplx<T>);
   __shared__ cplx<T> F[L * FP];
   __shared__ double red[64];
